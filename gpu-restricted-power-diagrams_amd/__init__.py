@@ -1,0 +1,420 @@
+"""Python binding of librpd (the MI355X restricted power diagram hot path).
+
+The product is the C-ABI shared library ``librpd.so`` built from ``csrc/``
+(hand-written HIP kernels for gfx950 + host C++).  This module only loads it
+with ctypes and mirrors the reference's ``VoroAlgo`` surface
+(voronoi.h:105-256) so that callers and tests read like the reference's own
+driver (test_voronoi.cpp:30-63).  Every compute call goes to librpd; when the
+library or a GPU is missing the calls raise -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librpd.so")
+
+# Status.h:4-16
+STATUS_NAMES = [
+    "vertex_overflow", "plane_overflow", "inconsistent_boundary",
+    "security_radius_not_reached", "success", "needs_exact_predicates",
+    "empty_cell", "find_another_beginning_vertex", "triangle_overflow",
+    "seed_on_border", "hessian_overflow",
+]
+SUCCESS = 4
+EMPTY_CELL = 6
+NO_CELL_LIMITS = -100000.0
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+PREC_FP32 = 2
+
+SHAPE_SPHERE = 0
+SHAPE_TORUS = 1
+SHAPE_CUBE = 2
+SHAPE_CUBE_MINUS_SPHERE = 3
+
+SEEDS_UNIFORM = 0
+SEEDS_LATTICE = 1
+SEEDS_BLUE_NOISE = 2
+
+
+class RpdError(RuntimeError):
+    pass
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_int32),
+        ("in_domain", ctypes.c_void_p),
+        ("points", ctypes.c_void_p),
+        ("num_points", ctypes.c_int32),
+        ("triangles", ctypes.c_void_p),
+        ("num_triangles", ctypes.c_int32),
+        ("tri_list", ctypes.c_void_p),
+        ("num_tri_list", ctypes.c_int32),
+        ("offsets", ctypes.c_void_p),
+        ("centre", ctypes.c_void_p),
+    ]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("K", "P", "V", "T", "precision", "max_extra_steps", "apex_mode", "reserved")]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("ms_bin", ctypes.c_double),
+        ("ms_knn", ctypes.c_double),
+        ("ms_clip", ctypes.c_double),
+        ("ms_fallback", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+        ("status_hist", ctypes.c_int32 * 11),
+        ("passes", ctypes.c_int32),
+        ("fallback_cells", ctypes.c_int32),
+        ("final_K", ctypes.c_int32),
+        ("final_P", ctypes.c_int32),
+        ("final_V", ctypes.c_int32),
+        ("final_T", ctypes.c_int32),
+        ("dim", ctypes.c_int32),
+        ("n", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "status_hist"}
+        d["status_hist"] = list(self.status_hist)
+        return d
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+_lib = None
+VP = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+
+
+def lib():
+    """Load librpd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RpdError("librpd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "rpd_voxel_mask": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, VP, VP]),
+        "rpd_grid_from_voxels": (ctypes.c_int, [ctypes.c_int, VP, ctypes.POINTER(VP)]),
+        "rpd_grid_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(VP)]),
+        "rpd_grid_save": (ctypes.c_int, [VP, ctypes.c_char_p]),
+        "rpd_grid_view": (ctypes.c_int, [VP, ctypes.POINTER(GridDesc)]),
+        "rpd_grid_free": (None, [VP]),
+        "rpd_grid_valid": (ctypes.c_int, [ctypes.POINTER(GridDesc)]),
+        "rpd_grid_volume": (ctypes.c_double, [ctypes.POINTER(GridDesc)]),
+        "rpd_grid_moments": (None, [ctypes.POINTER(GridDesc), VP]),
+        "rpd_generate_seeds": (ctypes.c_int, [ctypes.c_int, I64, ctypes.c_uint64, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, VP]),
+        "rpd_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(VP)]),
+        "rpd_destroy": (None, [VP]),
+        "rpd_last_error": (ctypes.c_char_p, [VP]),
+        "rpd_default_params": (None, [ctypes.POINTER(Params)]),
+        "rpd_domain_create": (ctypes.c_int, [VP, ctypes.POINTER(GridDesc), ctypes.POINTER(VP)]),
+        "rpd_domain_destroy": (None, [VP]),
+        "rpd_set_seeds": (ctypes.c_int, [VP, VP, I64]),
+        "rpd_set_seeds_device": (ctypes.c_int, [VP, VP, I64]),
+        "rpd_solve": (ctypes.c_int, [VP, VP, ctypes.POINTER(Params), ctypes.POINTER(Stats)]),
+        "rpd_read_moments": (ctypes.c_int, [VP, VP]),
+        "rpd_read_status": (ctypes.c_int, [VP, VP]),
+        "rpd_read_perm": (ctypes.c_int, [VP, VP]),
+        "rpd_moments_device": (ctypes.c_int, [VP, ctypes.POINTER(VP)]),
+        "rpd_knn": (ctypes.c_int, [VP, I32, I32, VP]),
+        "rpd_compact": (ctypes.c_int, [VP, ctypes.c_double, ctypes.POINTER(I64), VP, VP]),
+        "rpd_server_create": (ctypes.c_int, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
+        "rpd_server_destroy": (None, [VP]),
+        "rpd_server_submit": (ctypes.c_int, [VP, VP, VP, I64, ctypes.POINTER(Params), ctypes.POINTER(I64)]),
+        "rpd_server_poll": (ctypes.c_int, [VP, ctypes.c_int, ctypes.POINTER(I64), VP, I64,
+                                           ctypes.POINTER(ctypes.c_double)]),
+        "rpd_server_drain": (ctypes.c_int, [VP]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != 0:
+        msg = lib().rpd_last_error(ctx).decode() if ctx else ""
+        raise RpdError("%s failed (%d) %s" % (what, rc, msg))
+
+
+# ---------------------------------------------------------------------------
+# host helpers: domains and seeds
+# ---------------------------------------------------------------------------
+class Grid:
+    """Host domain grid (the reference's Grid, Grid.h:14-70) owned by librpd."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+        d = GridDesc()
+        _check(lib().rpd_grid_view(self._h, ctypes.byref(d)), what="rpd_grid_view")
+        self.desc = d
+        g = d.size
+        nv = g ** 3
+
+        def arr(ptr, ctype, n, dtype):
+            if n == 0 or not ptr:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), (n,)).astype(dtype, copy=True)
+
+        self.size = g
+        self.in_domain = arr(d.in_domain, ctypes.c_int8, nv, np.int8)
+        self.points = arr(d.points, ctypes.c_double, 4 * d.num_points, np.float64).reshape(-1, 4)
+        self.triangles = arr(d.triangles, ctypes.c_int32, 3 * d.num_triangles, np.int32)
+        self.tri_list = arr(d.tri_list, ctypes.c_int32, d.num_tri_list, np.int32)
+        self.offsets = arr(d.offsets, ctypes.c_int32, nv + 1, np.int32)
+        self.centre = arr(d.centre, ctypes.c_int8, nv, np.int8) if d.centre else None
+
+    @classmethod
+    def voxels(cls, mask):
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        g = int(round(mask.size ** (1.0 / 3.0)))
+        assert g ** 3 == mask.size
+        h = ctypes.c_void_p()
+        _check(lib().rpd_grid_from_voxels(g, _p(mask), ctypes.byref(h)), what="rpd_grid_from_voxels")
+        return cls(h.value)
+
+    @classmethod
+    def shape(cls, shape, g, params=()):
+        mask = voxel_mask(shape, g, params)
+        return cls.voxels(mask)
+
+    @classmethod
+    def load(cls, path):
+        h = ctypes.c_void_p()
+        _check(lib().rpd_grid_load(path.encode(), ctypes.byref(h)), what="rpd_grid_load")
+        return cls(h.value)
+
+    def save(self, path):
+        _check(lib().rpd_grid_save(self._h, path.encode()), what="rpd_grid_save")
+
+    def valid(self):
+        return bool(lib().rpd_grid_valid(ctypes.byref(self.desc)))
+
+    def volume(self):
+        return lib().rpd_grid_volume(ctypes.byref(self.desc))
+
+    def moments(self):
+        out = np.zeros(4)
+        lib().rpd_grid_moments(ctypes.byref(self.desc), _p(out))
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().rpd_grid_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def voxel_mask(shape, g, params=()):
+    p = np.zeros(8, np.float64)
+    p[: len(params)] = params
+    out = np.zeros(g ** 3, np.uint8)
+    _check(lib().rpd_voxel_mask(int(shape), int(g), _p(p), _p(out)), what="rpd_voxel_mask")
+    return out
+
+
+def generate_seeds(kind, n, seed=0, p0=0.0, p1=0.0, wmax=0.0):
+    out = np.zeros((int(n), 4), np.float64)
+    _check(lib().rpd_generate_seeds(int(kind), int(n), int(seed), float(p0), float(p1), float(wmax), _p(out)),
+           what="rpd_generate_seeds")
+    return out
+
+
+def default_params():
+    p = Params()
+    lib().rpd_default_params(ctypes.byref(p))
+    return p
+
+
+# ---------------------------------------------------------------------------
+# solver context
+# ---------------------------------------------------------------------------
+class Context:
+    """One librpd context: one HIP device, one stream (rpd_create)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().rpd_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise RpdError("rpd_create(%d) failed (%d): no usable HIP device" % (device, rc))
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rpd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc, what):
+        _check(rc, self._h, what)
+
+    def domain(self, grid):
+        h = ctypes.c_void_p()
+        self.check(lib().rpd_domain_create(self._h, ctypes.byref(grid.desc), ctypes.byref(h)), "rpd_domain_create")
+        return Domain(self, h, grid)
+
+    def set_seeds(self, xyzw):
+        xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
+        self.check(lib().rpd_set_seeds(self._h, _p(xyzw), xyzw.shape[0]), "rpd_set_seeds")
+        self.n = xyzw.shape[0]
+
+    def set_seeds_device(self, ptr, n):
+        self.check(lib().rpd_set_seeds_device(self._h, ctypes.c_void_p(ptr), int(n)), "rpd_set_seeds_device")
+        self.n = int(n)
+
+    def solve(self, domain=None, K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, apex_mode=1):
+        p = Params(int(K), int(P), int(V), int(T), int(precision), int(max_extra_steps), int(apex_mode), 0)
+        st = Stats()
+        self.check(lib().rpd_solve(self._h, domain._h if domain is not None else None, ctypes.byref(p),
+                                   ctypes.byref(st)), "rpd_solve")
+        return st.as_dict()
+
+    def moments(self):
+        out = np.zeros((self.n, 4), np.float64)
+        self.check(lib().rpd_read_moments(self._h, _p(out)), "rpd_read_moments")
+        return out
+
+    def moments_device_ptr(self):
+        p = ctypes.c_void_p()
+        self.check(lib().rpd_moments_device(self._h, ctypes.byref(p)), "rpd_moments_device")
+        return p.value
+
+    def status(self):
+        out = np.zeros(self.n, np.int32)
+        self.check(lib().rpd_read_status(self._h, _p(out)), "rpd_read_status")
+        return out
+
+    def perm(self):
+        out = np.zeros(self.n, np.int32)
+        self.check(lib().rpd_read_perm(self._h, _p(out)), "rpd_read_perm")
+        return out
+
+    def knn(self, K, precision=1):
+        out = np.zeros((self.n, int(K)), np.int32)
+        self.check(lib().rpd_knn(self._h, int(K), int(precision), _p(out)), "rpd_knn")
+        return out
+
+    def compact(self, keep_volume):
+        cnt = I64(0)
+        self.check(lib().rpd_compact(self._h, float(keep_volume), ctypes.byref(cnt), None, None), "rpd_compact")
+        out = np.zeros((cnt.value, 4), np.float64)
+        perm = np.zeros(cnt.value, np.int32)
+        self.check(lib().rpd_compact(self._h, float(keep_volume), ctypes.byref(cnt), _p(out), _p(perm)),
+                   "rpd_compact")
+        return out, perm
+
+
+class Domain:
+    """A domain grid resident in HBM (rpd_domain_create)."""
+
+    def __init__(self, ctx, handle, grid):
+        self.ctx = ctx
+        self._h = handle
+        self.grid = grid
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rpd_domain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class VoroAlgo:
+    """Mirror of the reference's VoroAlgo (voronoi.h:105-256) over librpd.
+
+    setInputPoints / setGrid / launch / getBarycenters / getPermutationToUser
+    keep the reference's meaning: launch computes the restricted power diagram
+    and calls saveData(self) while the outputs are available.  Outputs are the
+    used cells (volume >= m_keepVolume, ascending original index) in
+    (m*x, m*y, m*z, m) form, and getPermutationToUser maps them to the user's
+    seed indices (compressResult, voronoi.cpp:513-691).
+    """
+
+    def __init__(self, device=0, debug=False):
+        self._ctx = Context(device)
+        self._dom = None
+        self.m_K, self.m_P, self.m_V, self.m_T = 25, 35, 40, 100
+        self.m_keepVolume = 1.0
+        self.m_keepDistance = 0
+        self.precision = PREC_MIXED
+        self.max_extra_steps = 5
+        self.apex_mode = 1
+        self.m_debug = debug
+        self._bary = None
+        self._perm = None
+        self._input = None
+
+    def setGrid(self, grid):
+        self._dom = self._ctx.domain(grid)
+
+    def setInputPoints(self, seeds):
+        self._input = np.ascontiguousarray(seeds, dtype=np.float64).reshape(-1, 4)
+        self._ctx.set_seeds(self._input)
+
+    def getNumInputPoints(self):
+        return 0 if self._input is None else self._input.shape[0]
+
+    def launch(self, saveData=None, stats=None):
+        st = self._ctx.solve(self._dom, self.m_K, self.m_P, self.m_V, self.m_T, self.precision,
+                             self.max_extra_steps, self.apex_mode)
+        if stats is not None:
+            stats.append(st)
+        if self.m_keepDistance < 0 or self._dom is None:
+            self._bary = self._ctx.moments()
+            self._perm = np.arange(self._bary.shape[0], dtype=np.int32)
+        else:
+            self._bary, self._perm = self._ctx.compact(self.m_keepVolume)
+        ok = True if saveData is None else bool(saveData(self))
+        return ok
+
+    def getNumOutputPoints(self):
+        return 0 if self._bary is None else self._bary.shape[0]
+
+    def getBarycenters(self):
+        return self._bary
+
+    def getPermutationToUser(self):
+        return self._perm
+
+    def getHessians(self):
+        return np.zeros(0)
+
+    def getHessianIds(self):
+        return np.zeros(0, np.int32)

@@ -1,0 +1,472 @@
+// rpd_api.hip -- C-ABI of librpd: contexts, resident domains, seeds, the
+// solve with its status re-queue, outputs.
+//
+// Pass policy (replaces VoroAlgo::launch, voronoi.cpp:1244-1376):
+//   pass 1: every cell, caps K/P/V/T of rpd_params, in the requested form;
+//   then up to max_extra_steps re-queue passes over the failed cells only:
+//     * vertex/plane/K/triangle overflow -> the matching cap x1.5 (as the
+//       reference), same form;
+//     * needs_exact_predicates, inconsistent_boundary and
+//       find_another_beginning_vertex from an FP32 form -> FP64 form with the
+//       same caps (the reference drops cells flagged only for exact
+//       predicates, voronoi.cpp:1349-1356; here they are always recomputed);
+//     * empty_cell is final (volume 0).
+// Boundary cells are processed in the same pass as interior cells (the
+// reference defers them with a T = 0 first pass and recomputes them).
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rpd_internal.h"
+
+struct rpd_domain {
+    rpd_ctx *ctx = nullptr;
+    DomainDev dev;
+    DevBuf flags, centre, pts, tris, tlist, toffs;
+    int64_t bytes = 0;
+};
+
+struct rpd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int64_t n = 0;
+    DevBuf seeds;  // original order, FP64 xyzw
+    // binning (kNN precision bin_prec)
+    int bin_prec = -1;
+    int dim = 0;
+    DevBuf perm, counters, stored, stored64, bin_scratch;
+    // outputs, original order
+    DevBuf out, status, kused;
+    // pass work buffers
+    DevBuf knn, ids_a, ids_b, ids_c, cnt;
+    bool solved = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+int set_err(rpd_ctx *c, int code, const char *fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess) return set_err(ctx, RPD_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_));  \
+    } while (0)
+
+#define RPDCHK(ctx, call, what)                                          \
+    do {                                                                 \
+        int r_ = (call);                                                 \
+        if (r_ != RPD_OK) return set_err(ctx, r_, "%s failed (%d)", what, r_); \
+    } while (0)
+
+// failed cells of a list: compacts stored ids whose status (by original
+// index) is not success; counts statuses.
+__global__ void collect_failed_kernel(const int *__restrict__ ids, int64_t n_ids, const int *__restrict__ perm,
+                                      const int *__restrict__ status, int *__restrict__ exact_ids,
+                                      int *__restrict__ over_ids, int *__restrict__ cnt) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= n_ids) return;
+    const int sid = ids ? ids[i] : int(i);
+    const int st = status[perm[sid]];
+    if (st == RPD_SUCCESS) return;
+    atomicAdd(&cnt[2 + st], 1);
+    if (st == RPD_EMPTY_CELL) return;
+    if (st == RPD_NEEDS_EXACT_PREDICATES || st == RPD_INCONSISTENT_BOUNDARY ||
+        st == RPD_FIND_ANOTHER_BEGINNING_VERTEX) {
+        exact_ids[atomicAdd(&cnt[0], 1)] = sid;
+    } else {
+        over_ids[atomicAdd(&cnt[1], 1)] = sid;
+    }
+}
+
+__global__ void status_hist_kernel(const int *__restrict__ status, int64_t n, int *__restrict__ hist) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const int st = status[i];
+    if (st >= 0 && st < RPD_STATUS_NUM) atomicAdd(&hist[st], 1);
+}
+
+int ensure_binning(rpd_ctx *c, int prec) {
+    if (c->bin_prec == prec) return RPD_OK;
+    const int64_t n = c->n;
+    const int dim = rpd_knn_dim(n);
+    const int64_t nc = int64_t(dim) * dim * dim;
+    if (nc + 1 > INT32_MAX) return set_err(c, RPD_E_LIMIT, "kNN grid too large");
+    RPDCHK(c, c->perm.ensure(size_t(n) * 4), "alloc perm");
+    RPDCHK(c, c->counters.ensure(size_t(nc + 1) * 4), "alloc counters");
+    RPDCHK(c, c->stored.ensure(size_t(n) * (prec == 0 ? 16 : 32)), "alloc stored");
+    RPDCHK(c, c->stored64.ensure(size_t(n) * 32), "alloc stored64");
+    RPDCHK(c,
+           rpd_bin_seeds(c->stream, c->seeds.as<double4>(), n, prec, dim, c->bin_scratch, c->perm.as<int>(),
+                         c->counters.as<int>(), c->stored.ptr, c->stored64.as<double4>()),
+           "binning");
+    c->bin_prec = prec;
+    c->dim = dim;
+    return RPD_OK;
+}
+
+float elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpd_create(int device, rpd_ctx **out) {
+    if (!out) return RPD_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RPD_E_DEVICE;
+    if (device < 0 || device >= ndev) return RPD_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return RPD_E_DEVICE;
+    auto *c = new rpd_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return RPD_E_DEVICE;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete c;
+            return RPD_E_DEVICE;
+        }
+    *out = c;
+    return RPD_OK;
+}
+
+void rpd_destroy(rpd_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *rpd_last_error(const rpd_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void rpd_default_params(rpd_params *p) {
+    if (!p) return;
+    // VoroAlgo constructor defaults (voronoi.cpp:1041-1073)
+    p->K = 25;
+    p->P = 35;
+    p->V = 40;
+    p->T = 100;
+    p->precision = RPD_PREC_MIXED;
+    p->max_extra_steps = 5;
+    p->apex_mode = 1;
+    p->reserved = 0;
+}
+
+int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
+    if (!c || !g || !out || g->size <= 0) return set_err(c, RPD_E_ARG, "rpd_domain_create: bad arguments");
+    hipSetDevice(c->device);
+    auto *d = new rpd_domain();
+    d->ctx = c;
+    const size_t nv = size_t(g->size) * g->size * g->size;
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
+        if (bytes == 0) bytes = 4;
+        if (b.ensure(bytes) != RPD_OK) return RPD_E_NOMEM;
+        if (src && hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return RPD_E_DEVICE;
+        d->bytes += int64_t(bytes);
+        return RPD_OK;
+    };
+    std::vector<double> pts4(size_t(g->num_points) * 4);
+    for (int i = 0; i < g->num_points; ++i)
+        for (int k = 0; k < 4; ++k) pts4[size_t(i) * 4 + k] = k < 3 ? g->points[4 * size_t(i) + k] : 0.0;
+    int r = up(d->flags, g->in_domain, nv);
+    if (r == RPD_OK && g->centre) r = up(d->centre, g->centre, nv);
+    if (r == RPD_OK) r = up(d->pts, pts4.data(), pts4.size() * 8);
+    if (r == RPD_OK) r = up(d->tris, g->triangles, size_t(g->num_triangles) * 12);
+    if (r == RPD_OK) r = up(d->tlist, g->tri_list, size_t(g->num_tri_list) * 4);
+    if (r == RPD_OK) r = up(d->toffs, g->offsets, (nv + 1) * 4);
+    if (r == RPD_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = RPD_E_DEVICE;
+    if (r != RPD_OK) {
+        delete d;
+        return set_err(c, r, "rpd_domain_create: upload failed");
+    }
+    d->dev.g = g->size;
+    d->dev.vs = 1000.0 / double(g->size);
+    d->dev.flags = d->flags.as<int8_t>();
+    d->dev.centre = g->centre ? d->centre.as<int8_t>() : nullptr;
+    d->dev.pts = d->pts.as<double4>();
+    d->dev.tris = d->tris.as<int>();
+    d->dev.tlist = d->tlist.as<int>();
+    d->dev.toffs = d->toffs.as<int>();
+    for (size_t i = 0; i < nv; ++i)
+        if (g->in_domain[i] == 1) {
+            d->dev.has_border_vertices = true;
+            break;
+        }
+    *out = d;
+    return RPD_OK;
+}
+
+void rpd_domain_destroy(rpd_domain *d) {
+    if (!d) return;
+    hipSetDevice(d->ctx->device);
+    hipStreamSynchronize(d->ctx->stream);
+    delete d;
+}
+
+static int set_seeds_common(rpd_ctx *c, const double *src, int64_t n, hipMemcpyKind kind) {
+    if (!c || !src || n < 2) return set_err(c, RPD_E_ARG, "rpd_set_seeds: need at least 2 seeds");
+    if (n > INT32_MAX / 2) return set_err(c, RPD_E_LIMIT, "rpd_set_seeds: too many seeds");
+    hipSetDevice(c->device);
+    RPDCHK(c, c->seeds.ensure(size_t(n) * 32), "alloc seeds");
+    HIPCHK(c, hipMemcpyAsync(c->seeds.ptr, src, size_t(n) * 32, kind, c->stream));
+    c->n = n;
+    c->bin_prec = -1;
+    c->solved = false;
+    return RPD_OK;
+}
+
+int rpd_set_seeds(rpd_ctx *c, const double *xyzw, int64_t n) {
+    return set_seeds_common(c, xyzw, n, hipMemcpyHostToDevice);
+}
+
+int rpd_set_seeds_device(rpd_ctx *c, const double *d_xyzw, int64_t n) {
+    return set_seeds_common(c, d_xyzw, n, hipMemcpyDeviceToDevice);
+}
+
+int rpd_knn(rpd_ctx *c, int32_t K, int32_t prec, int32_t *out_nk) {
+    if (!c || c->n < 2 || K <= 0 || !out_nk || (prec != 0 && prec != 1)) return set_err(c, RPD_E_ARG, "rpd_knn");
+    if (K > 256) return set_err(c, RPD_E_LIMIT, "rpd_knn: K > 256");
+    hipSetDevice(c->device);
+    RPDCHK(c, ensure_binning(c, prec), "binning");
+    RPDCHK(c, c->knn.ensure(size_t(c->n) * K * 4), "alloc knn");
+    RPDCHK(c,
+           rpd_launch_knn(c->stream, prec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, nullptr,
+                          c->n, K, c->knn.as<int>()),
+           "knn launch");
+    HIPCHK(c, hipMemcpyAsync(out_nk, c->knn.ptr, size_t(c->n) * K * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPD_OK;
+}
+
+int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *stats) {
+    if (!c || !prm) return set_err(c, RPD_E_ARG, "rpd_solve: bad arguments");
+    if (c->n < 2) return set_err(c, RPD_E_STATE, "rpd_solve: no seeds");
+    if (prm->precision < 0 || prm->precision > 2) return set_err(c, RPD_E_ARG, "rpd_solve: bad precision");
+    if (prm->K <= 0 || prm->K > 256 || prm->P < 7 || prm->P > 255 || prm->V < 8 || prm->V > 256 || prm->T < 0)
+        return set_err(c, RPD_E_ARG, "rpd_solve: caps out of range (0<K<=256, 7<=P<=255, 8<=V<=256, T>=0)");
+    if (dom && dom->ctx != c) return set_err(c, RPD_E_ARG, "rpd_solve: domain of another context");
+    hipSetDevice(c->device);
+    const int64_t n = c->n;
+    const int form0 = prm->precision;
+    const int kprec = form0 == RPD_PREC_FP64 ? 1 : 0;
+    int K = std::min<int64_t>(prm->K, n - 1), P = prm->P, V = prm->V, T = prm->T;
+    rpd_stats st{};
+    st.n = n;
+    auto t_start = std::chrono::steady_clock::now();
+
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    RPDCHK(c, ensure_binning(c, kprec), "binning");
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    st.ms_bin = elapsed_ms(c->ev[0], c->ev[1]);
+
+    RPDCHK(c, c->out.ensure(size_t(n) * 32), "alloc out");
+    RPDCHK(c, c->status.ensure(size_t(n) * 4), "alloc status");
+    RPDCHK(c, c->kused.ensure(size_t(n) * 4), "alloc kused");
+    RPDCHK(c, c->ids_a.ensure(size_t(n) * 4), "alloc ids");
+    RPDCHK(c, c->ids_b.ensure(size_t(n) * 4), "alloc ids");
+    RPDCHK(c, c->ids_c.ensure(size_t(n) * 4), "alloc ids");
+    RPDCHK(c, c->cnt.ensure(64), "alloc cnt");
+
+    const DomainDev *dd = dom ? &dom->dev : nullptr;
+    auto run_pass = [&](int form, const int *ids, int64_t n_ids, int K_, int P_, int V_, int T_, double &ms_knn,
+                        double &ms_clip) -> int {
+        if (n_ids <= 0) return RPD_OK;
+        int r = c->knn.ensure(size_t(n_ids) * K_ * 4);
+        if (r != RPD_OK) return r;
+        hipEventRecord(c->ev[0], c->stream);
+        r = rpd_launch_knn(c->stream, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids,
+                           n_ids, K_, c->knn.as<int>());
+        if (r != RPD_OK) return r;
+        hipEventRecord(c->ev[1], c->stream);
+        ClipLaunch L{};
+        L.seeds = c->stored64.as<double4>();
+        L.perm = c->perm.as<int>();
+        L.ids = ids;
+        L.knn = c->knn.as<int>();
+        L.n_ids = n_ids;
+        L.K = K_;
+        L.P = P_;
+        L.V = V_;
+        L.T = T_;
+        L.apex_mode = prm->apex_mode;
+        L.dom = dd;
+        L.out = c->out.as<double4>();
+        L.status = c->status.as<int>();
+        L.kused = c->kused.as<int>();
+        r = rpd_launch_clip(c->stream, form, L);
+        if (r != RPD_OK) return r;
+        hipEventRecord(c->ev[2], c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return RPD_E_DEVICE;
+        ms_knn += elapsed_ms(c->ev[0], c->ev[1]);
+        ms_clip += elapsed_ms(c->ev[1], c->ev[2]);
+        return RPD_OK;
+    };
+
+    double ms_knn = 0, ms_clip = 0, ms_fb_knn = 0, ms_fb = 0;
+    RPDCHK(c, run_pass(form0, nullptr, n, K, P, V, T, ms_knn, ms_clip), "pass 1");
+    st.passes = 1;
+
+    // re-queue passes
+    const int *cur_ids = nullptr;
+    int64_t cur_n = n;
+    int cur_form = form0;
+    DevBuf *bufs[3] = {&c->ids_a, &c->ids_b, &c->ids_c};
+    for (int step = 0; step < std::max(0, prm->max_extra_steps); ++step) {
+        // split failures of the last pass into (exact -> ids_b) and (overflow -> ids_c)
+        int *exact_ids = bufs[1]->as<int>(), *over_ids = bufs[2]->as<int>();
+        HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));
+        const unsigned nb = unsigned((cur_n + 255) / 256);
+        collect_failed_kernel<<<nb, 256, 0, c->stream>>>(cur_ids, cur_n, c->perm.as<int>(), c->status.as<int>(),
+                                                         exact_ids, over_ids, c->cnt.as<int>());
+        int cnt[2 + RPD_STATUS_NUM];
+        HIPCHK(c, hipMemcpyAsync(cnt, c->cnt.ptr, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const int n_exact = cnt[0], n_over = cnt[1];
+        const int *h = cnt + 2;
+        int K2 = K, P2 = P, V2 = V, T2 = T;
+        if (h[RPD_VERTEX_OVERFLOW]) V2 = std::min(256, int(V * 1.5));
+        if (h[RPD_PLANE_OVERFLOW]) P2 = std::min(255, int(P * 1.5));
+        if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) K2 = int(std::min<int64_t>(n - 1, std::min(256, int(K * 1.5))));
+        if (h[RPD_TRIANGLE_OVERFLOW]) T2 = int(T * 1.5) + 1;
+        const bool caps_changed = K2 != K || P2 != P || V2 != V || T2 != T;
+        const bool exact_rerun = n_exact > 0 && cur_form != RPD_PREC_FP64;
+        const bool over_rerun = n_over > 0 && caps_changed;
+        if (!exact_rerun && !over_rerun) break;
+        K = K2;
+        P = P2;
+        V = V2;
+        T = T2;
+        // next failures are collected over the union of both lists: run the
+        // overflow cells first into ids_c then the exact cells into ids_b, and
+        // gather both lists into ids_a for the next collection.
+        int64_t next_n = 0;
+        if (over_rerun) {
+            RPDCHK(c, run_pass(cur_form, over_ids, n_over, K, P, V, T, ms_knn, ms_clip), "overflow pass");
+            st.passes++;
+        }
+        if (n_exact > 0) {
+            const int f = RPD_PREC_FP64;
+            if (exact_rerun || caps_changed) {
+                RPDCHK(c, run_pass(f, exact_ids, n_exact, K, P, V, T, ms_fb_knn, ms_fb), "fp64 pass");
+                st.passes++;
+                st.fallback_cells += n_exact;
+            }
+        }
+        // ids_a <- exact ++ over (only those that were re-run)
+        int *dst = bufs[0]->as<int>();
+        if (n_exact > 0 && (exact_rerun || caps_changed)) {
+            HIPCHK(c, hipMemcpyAsync(dst, exact_ids, size_t(n_exact) * 4, hipMemcpyDeviceToDevice, c->stream));
+            next_n += n_exact;
+        }
+        if (over_rerun) {
+            HIPCHK(c, hipMemcpyAsync(dst + next_n, over_ids, size_t(n_over) * 4, hipMemcpyDeviceToDevice, c->stream));
+            next_n += n_over;
+        }
+        cur_ids = dst;
+        cur_n = next_n;
+        // after an exact rerun the list is in FP64 for its exact part; the
+        // overflow part stays in the original form
+        cur_form = (n_over > 0 && over_rerun) ? form0 : RPD_PREC_FP64;
+        if (cur_n == 0) break;
+    }
+    // final status histogram
+    HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));
+    status_hist_kernel<<<unsigned((n + 255) / 256), 256, 0, c->stream>>>(c->status.as<int>(), n, c->cnt.as<int>());
+    HIPCHK(c, hipMemcpyAsync(st.status_hist, c->cnt.ptr, sizeof(st.status_hist), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    st.ms_knn = ms_knn + ms_fb_knn;
+    st.ms_clip = ms_clip;
+    st.ms_fallback = ms_fb;
+    st.final_K = K;
+    st.final_P = P;
+    st.final_V = V;
+    st.final_T = T;
+    st.dim = c->dim;
+    st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    c->solved = true;
+    if (stats) *stats = st;
+    return RPD_OK;
+}
+
+int rpd_read_moments(rpd_ctx *c, double *out4n) {
+    if (!c || !out4n) return set_err(c, RPD_E_ARG, "rpd_read_moments");
+    if (!c->solved) return set_err(c, RPD_E_STATE, "rpd_read_moments: no solve");
+    hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpyAsync(out4n, c->out.ptr, size_t(c->n) * 32, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPD_OK;
+}
+
+int rpd_moments_device(rpd_ctx *c, const double **p) {
+    if (!c || !p) return set_err(c, RPD_E_ARG, "rpd_moments_device");
+    if (!c->solved) return set_err(c, RPD_E_STATE, "rpd_moments_device: no solve");
+    *p = c->out.as<double>();
+    return RPD_OK;
+}
+
+int rpd_read_status(rpd_ctx *c, int32_t *out) {
+    if (!c || !out) return set_err(c, RPD_E_ARG, "rpd_read_status");
+    if (!c->solved) return set_err(c, RPD_E_STATE, "rpd_read_status: no solve");
+    hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpyAsync(out, c->status.ptr, size_t(c->n) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPD_OK;
+}
+
+int rpd_read_perm(rpd_ctx *c, int32_t *out) {
+    if (!c || !out || c->bin_prec < 0) return set_err(c, RPD_E_STATE, "rpd_read_perm: no binning yet");
+    hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpyAsync(out, c->perm.ptr, size_t(c->n) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPD_OK;
+}
+
+int rpd_compact(rpd_ctx *c, double keep_volume, int64_t *out_count, double *out4, int32_t *perm_to_user) {
+    if (!c || !out_count) return set_err(c, RPD_E_ARG, "rpd_compact");
+    if (!c->solved) return set_err(c, RPD_E_STATE, "rpd_compact: no solve");
+    std::vector<double> m(size_t(c->n) * 4);
+    int r = rpd_read_moments(c, m.data());
+    if (r != RPD_OK) return r;
+    int64_t k = 0;
+    for (int64_t i = 0; i < c->n; ++i) {
+        if (!(m[4 * i + 3] >= keep_volume)) continue;
+        if (out4)
+            for (int j = 0; j < 4; ++j) out4[4 * k + j] = m[4 * i + j];
+        if (perm_to_user) perm_to_user[k] = int32_t(i);
+        ++k;
+    }
+    *out_count = k;
+    return RPD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// rpd_internal.h -- declarations shared by the translation units of librpd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "rpd.h"
+
+// Growable device buffer (hipMalloc'ed, freed with the owner).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need) {
+        if (need <= bytes && ptr) return RPD_OK;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        if (need == 0) return RPD_OK;
+        if (hipMalloc(&ptr, need) != hipSuccess) {
+            ptr = nullptr;
+            return RPD_E_NOMEM;
+        }
+        bytes = need;
+        return RPD_OK;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(ptr);
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+};
+
+// ---- kNN (rpd_knn.hip) ----------------------------------------------------
+int rpd_knn_dim(int64_t n);
+int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch, int *perm,
+                  int *counters, void *stored, double4 *stored64);
+int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
+                   const int *ids, int64_t n_ids, int K, int *out);
+
+// ---- clip (rpd_clip.hip) --------------------------------------------------
+struct DomainDev {
+    int g = 0;
+    double vs = 0;
+    const int8_t *flags = nullptr;
+    const int8_t *centre = nullptr;
+    const double4 *pts = nullptr;
+    const int *tris = nullptr;
+    const int *tlist = nullptr;
+    const int *toffs = nullptr;
+    bool has_border_vertices = false;
+};
+
+struct ClipLaunch {
+    const double4 *seeds;  // stored order, FP64
+    const int *perm;       // stored -> original
+    const int *ids;        // stored ids to process (list position t)
+    const int *knn;        // n_ids * K stored ids
+    int64_t n_ids;
+    int K, P, V, T;
+    int apex_mode;
+    const DomainDev *dom;  // nullptr: no domain
+    double4 *out;          // original order
+    int *status;           // original order
+    int *kused;            // original order (nullable)
+};
+
+// form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32
+int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L);

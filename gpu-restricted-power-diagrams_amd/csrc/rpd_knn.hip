@@ -1,0 +1,463 @@
+// rpd_knn.hip -- seed binning and exact K-nearest-neighbour search on gfx950.
+//
+// Replaces the reference's KNearests (knearests.cpp:48-315) and
+// build_knearests (knearests.cl:36-78):
+//   * binning: cell = floor(p * dim / 1000) per axis (knearests.cpp:35-46) in
+//     the seed precision; a stable radix sort of (cell, original index) gives
+//     the permutation, so seeds keep original order inside a cell;
+//   * query: one wavefront per query seed over the sorted array.  Candidate
+//     cells are visited shell by shell (Chebyshev rings with per-axis bounds);
+//     each row of a shell is one contiguous range of the sorted array, so the
+//     64 lanes load consecutive seeds.  The running top-K is held across the
+//     lanes (lane i holds rank i, two registers per lane when K > 64) and is
+//     updated by a bitonic sort/merge or, for sparse batches, by ordered
+//     insertion.  The search stops once the K-th distance is below the lower
+//     bound of the next shell, so the result is the exact K nearest under the
+//     key (d^2, original index), d^2 = ((dx*dx + dy*dy) + dz*dz) unfused.
+
+#include <hip/hip_runtime.h>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "rpd_internal.h"
+#include "rpd_wave.h"
+
+namespace rpd {
+
+template <typename R>
+struct Vec4T;
+template <>
+struct Vec4T<float> {
+    using T = float4;
+};
+template <>
+struct Vec4T<double> {
+    using T = double4;
+};
+
+// floor(c * (R)dim / 1000) with the products and the quotient rounded once
+// each, clamped to [0, dim-1] (knearests.cpp:35-46).
+__device__ __forceinline__ int cell_coord(float c, int dim) {
+    const float t = __fdiv_rn(__fmul_rn(c, float(dim)), 1000.0f);
+    int i = int(floorf(t));
+    return max(0, min(i, dim - 1));
+}
+__device__ __forceinline__ int cell_coord(double c, int dim) {
+    const double t = __ddiv_rn(__dmul_rn(c, double(dim)), 1000.0);
+    int i = int(floor(t));
+    return max(0, min(i, dim - 1));
+}
+
+template <typename R>
+__global__ void bin_cells_kernel(const double4 *__restrict__ seeds, int64_t n, int dim, int *__restrict__ cell,
+                                 int *__restrict__ counts, int *__restrict__ orig) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const double4 s = seeds[i];
+    const R x = R(s.x), y = R(s.y), z = R(s.z);
+    const int c = cell_coord(x, dim) + cell_coord(y, dim) * dim + cell_coord(z, dim) * dim * dim;
+    cell[i] = c;
+    orig[i] = int(i);
+    atomicAdd(&counts[c], 1);
+}
+
+template <typename R>
+__global__ void gather_seeds_kernel(const double4 *__restrict__ seeds, const int *__restrict__ perm, int64_t n,
+                                    typename Vec4T<R>::T *__restrict__ stored, double4 *__restrict__ stored64) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const double4 s = seeds[perm[i]];
+    typename Vec4T<R>::T v;
+    v.x = R(s.x);
+    v.y = R(s.y);
+    v.z = R(s.z);
+    v.w = R(s.w);
+    stored[i] = v;
+    if (stored64) stored64[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// top-K keys: (d^2 bits, original index) ordered, stored index as payload
+// ---------------------------------------------------------------------------
+template <typename R>
+struct Key;
+
+template <>
+struct Key<float> {
+    uint64_t k;  // (bits(d2) << 32) | original index  (d2 >= 0: bit order == value order)
+    uint32_t s;  // stored index
+    __device__ static Key inf() { return Key{~0ull, 0xffffffffu}; }
+    __device__ static Key make(float d2, uint32_t orig, uint32_t st) {
+        return Key{(uint64_t(__float_as_uint(d2)) << 32) | orig, st};
+    }
+    __device__ bool operator<(const Key &o) const { return k < o.k; }
+    __device__ bool is_inf() const { return k == ~0ull; }
+    __device__ double dist2() const { return double(__uint_as_float(uint32_t(k >> 32))); }
+    __device__ Key xor_shfl(int m) const {
+        Key r;
+        r.k = __shfl_xor(k, m);
+        r.s = __shfl_xor(s, m);
+        return r;
+    }
+    __device__ Key up_shfl() const {
+        Key r;
+        r.k = __shfl_up(k, 1);
+        r.s = __shfl_up(s, 1);
+        return r;
+    }
+    __device__ Key lane(int l) const { return Key{rlane(k, l), rlane(s, l)}; }
+};
+
+template <>
+struct Key<double> {
+    uint64_t d;  // bits(d2)
+    uint32_t o;  // original index
+    uint32_t s;  // stored index
+    __device__ static Key inf() { return Key{~0ull, 0xffffffffu, 0xffffffffu}; }
+    __device__ static Key make(double d2, uint32_t orig, uint32_t st) {
+        return Key{uint64_t(__double_as_longlong(d2)), orig, st};
+    }
+    __device__ bool operator<(const Key &b) const { return d < b.d || (d == b.d && o < b.o); }
+    __device__ bool is_inf() const { return d == ~0ull && o == 0xffffffffu; }
+    __device__ double dist2() const { return __longlong_as_double((long long)d); }
+    __device__ Key xor_shfl(int m) const {
+        Key r;
+        r.d = __shfl_xor(d, m);
+        r.o = __shfl_xor(o, m);
+        r.s = __shfl_xor(s, m);
+        return r;
+    }
+    __device__ Key up_shfl() const {
+        Key r;
+        r.d = __shfl_up(d, 1);
+        r.o = __shfl_up(o, 1);
+        r.s = __shfl_up(s, 1);
+        return r;
+    }
+    __device__ Key lane(int l) const { return Key{rlane(d, l), rlane(o, l), rlane(s, l)}; }
+};
+
+template <typename K>
+__device__ __forceinline__ K kmin(const K &a, const K &b) { return b < a ? b : a; }
+template <typename K>
+__device__ __forceinline__ K kmax(const K &a, const K &b) { return b < a ? a : b; }
+
+// Bitonic network over N = 64*S elements, element e = s*64 + lane.
+// desc: sort descending instead of ascending.
+template <int S, typename K>
+__device__ __forceinline__ void bitonic_sort(K (&v)[S], bool desc) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int k = 2; k <= 64 * S; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const int js = j >> 6;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (s & js) continue;
+                    const int e = s * 64 + lane;
+                    const bool asc = ((e & k) == 0) != desc;
+                    const K lo = v[s], hi = v[s | js];
+                    v[s] = asc ? kmin(lo, hi) : kmax(lo, hi);
+                    v[s | js] = asc ? kmax(lo, hi) : kmin(lo, hi);
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int e = s * 64 + lane;
+                    const K o = v[s].xor_shfl(j);
+                    const bool asc = ((e & k) == 0) != desc;
+                    const bool keep_min = ((e & j) == 0) == asc;
+                    v[s] = keep_min ? kmin(v[s], o) : kmax(v[s], o);
+                }
+            }
+        }
+    }
+}
+
+// Bitonic merge of a bitonic sequence into ascending order.
+template <int S, typename K>
+__device__ __forceinline__ void bitonic_merge_asc(K (&v)[S]) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 32 * S; j > 0; j >>= 1) {
+        if (j >= 64) {
+            const int js = j >> 6;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (s & js) continue;
+                const K lo = v[s], hi = v[s | js];
+                v[s] = kmin(lo, hi);
+                v[s | js] = kmax(lo, hi);
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const K o = v[s].xor_shfl(j);
+                const bool keep_min = (lane & j) == 0;
+                v[s] = keep_min ? kmin(v[s], o) : kmax(v[s], o);
+            }
+        }
+    }
+}
+
+template <typename R>
+struct KnnArgs {
+    const typename Vec4T<R>::T *pts;  // stored order
+    const int *perm;                  // stored -> original
+    const int *counters;              // dim^3 + 1 exclusive prefix
+    const int *ids;                   // query stored ids (nullptr: 0..n_ids-1)
+    int64_t n_ids;
+    int dim;
+    int K;
+    int *out;  // n_ids * K stored ids, -1 padded
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename R>
+__device__ __forceinline__ R dist2_unfused(const typename Vec4T<R>::T &a, const typename Vec4T<R>::T &b) {
+#pragma clang fp contract(off)
+    const R dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+constexpr int kKnnWavesPerBlock = 4;
+
+template <typename R, int S>
+__global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> a) {
+    using KeyT = Key<R>;
+    __shared__ int seg_off_all[kKnnWavesPerBlock][64];
+    __shared__ int seg_a_all[kKnnWavesPerBlock][64];
+    const int lane = lane_id();
+    const int wv = threadIdx.x >> 6;
+    const int64_t qi = int64_t(blockIdx.x) * kKnnWavesPerBlock + wv;
+    if (qi >= a.n_ids) return;  // wave-uniform
+    int *seg_off = seg_off_all[wv];
+    int *seg_a = seg_a_all[wv];
+    const int q = a.ids ? a.ids[qi] : int(qi);
+    const typename Vec4T<R>::T p = a.pts[q];
+    const int dim = a.dim;
+    const int ci = cell_coord(R(p.x), dim), cj = cell_coord(R(p.y), dim), ck = cell_coord(R(p.z), dim);
+    const int K = a.K;
+    const double cs = 1000.0 / double(dim);
+
+    KeyT top[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) top[s] = KeyT::inf();
+    KeyT kth = KeyT::inf();
+    const int kl = (K - 1) & 63, ks = (K - 1) >> 6;
+
+    auto refresh_kth = [&]() {
+        KeyT t = top[0];
+#pragma unroll
+        for (int s = 1; s < S; ++s)
+            if (s == ks) t = top[s];
+        kth = t.lane(kl);
+    };
+
+    auto insert_one = [&](const KeyT &c) {
+        int pos = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) pos += __popcll(ballot(top[s] < c));
+        KeyT prev_last = KeyT::inf();
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            KeyT up = top[s].up_shfl();
+            if (lane == 0) up = prev_last;
+            prev_last = top[s].lane(63);
+            const int e = s * 64 + lane;
+            if (e == pos)
+                top[s] = c;
+            else if (e > pos)
+                top[s] = up;
+        }
+    };
+
+    auto merge_batch = [&](KeyT c) {
+        if (!(c < kth)) c = KeyT::inf();
+        uint64_t m = ballot(!c.is_inf());
+        if (m == 0) return;
+        if (__popcll(m) <= 6) {
+            while (m) {
+                const int l = lowest_bit(m);
+                m &= m - 1;
+                insert_one(c.lane(l));
+            }
+        } else {
+            KeyT cv[S];
+            cv[0] = c;
+#pragma unroll
+            for (int s = 1; s < S; ++s) cv[s] = KeyT::inf();
+            bitonic_sort<S>(cv, true);
+#pragma unroll
+            for (int s = 0; s < S; ++s) top[s] = kmin(top[s], cv[s]);
+            bitonic_merge_asc<S>(top);
+        }
+        refresh_kth();
+    };
+
+    for (int r = 1;; ++r) {
+        const int side = 2 * r + 1;
+        const int nrows = side * side;
+        const int nseg = 2 * nrows;
+        for (int chunk = 0; chunk < nseg; chunk += 64) {
+            const int u = chunk + lane;
+            int sa = 0, sb = 0;
+            if (u < nseg) {
+                const int t = u >> 1, part = u & 1;
+                const int dy = t % side - r, dz = t / side - r;
+                const int y = cj + dy, z = ck + dz;
+                if (y >= 0 && y < dim && z >= 0 && z < dim) {
+                    // the first shell (r == 1) is the whole 3x3x3 block
+                    const bool full = r == 1 || abs(dy) == r || abs(dz) == r;
+                    int xlo, xhi;
+                    if (full) {
+                        xlo = part == 0 ? max(0, ci - r) : 1;
+                        xhi = part == 0 ? min(dim - 1, ci + r) : 0;
+                    } else {
+                        xlo = xhi = part == 0 ? ci - r : ci + r;
+                    }
+                    if (xlo >= 0 && xhi < dim && xlo <= xhi) {
+                        const int row = y * dim + z * dim * dim;
+                        sa = a.counters[row + xlo];
+                        sb = a.counters[row + xhi + 1];
+                    }
+                }
+            }
+            const int len = sb - sa;
+            const int off = wave_excl_scan(len);
+            const int total = uni(rlane(off + len, 63));
+            seg_off[lane] = off;
+            seg_a[lane] = sa;
+            wave_lds_sync();
+            for (int base = 0; base < total; base += 64) {
+                const int g = base + lane;
+                KeyT c = KeyT::inf();
+                if (g < total) {
+                    // last segment whose offset is <= g (empty segments never win)
+                    int lo = 0, hi = 63;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (seg_off[mid] <= g)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    const int idx = seg_a[lo] + (g - seg_off[lo]);
+                    if (idx != q) {
+                        const typename Vec4T<R>::T cp = a.pts[idx];
+                        const R d2 = dist2_unfused<R>(p, cp);
+                        c = KeyT::make(d2, uint32_t(a.perm[idx]), uint32_t(idx));
+                    }
+                }
+                merge_batch(c);
+            }
+            wave_lds_sync();
+        }
+        // stop once the K-th key is closer than anything in the next shell
+        if (!kth.is_inf()) {
+            const double lb = double(r) * cs * (1.0 - 1e-4);
+            if (kth.dist2() < lb * lb * (1.0 - 1e-6)) break;
+        }
+        const int rn = r + 1;
+        if (ci - rn < 0 && ci + rn >= dim && cj - rn < 0 && cj + rn >= dim && ck - rn < 0 && ck + rn >= dim) break;
+    }
+    int *out = a.out + qi * int64_t(K);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int e = s * 64 + lane;
+        if (e < K) out[e] = top[s].is_inf() ? -1 : int(top[s].s);
+    }
+}
+
+}  // namespace rpd
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+using namespace rpd;
+
+int rpd_knn_dim(int64_t n) {
+    // knearests.cpp:63 -- round((N/3.1)^(1/3)), at least 1
+    const double d = std::cbrt(double(n) / 3.1);
+    return std::max(1, int(std::lround(d)));
+}
+
+// Bins n FP64 seeds in precision prec (0 float, 1 double).  Outputs (device):
+// perm[n], counters[dim^3+1], stored (float4 or double4 by prec) and stored64.
+int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch, int *perm,
+                  int *counters, void *stored, double4 *stored64) {
+    const int64_t nc = int64_t(dim) * dim * dim;
+    const size_t bytes_cell = size_t(n) * 4;
+    // scratch layout: cell_in, cell_out, orig_in, counts(nc+1), rocprim temp
+    size_t sort_bytes = 0, scan_bytes = 0;
+    int *null_i = nullptr;
+    unsigned end_bit = 1;
+    while ((int64_t(1) << end_bit) < nc) ++end_bit;
+    if (rocprim::radix_sort_pairs(nullptr, sort_bytes, null_i, null_i, null_i, null_i, n, 0u, end_bit, st) !=
+        hipSuccess)
+        return RPD_E_DEVICE;
+    if (rocprim::exclusive_scan(nullptr, scan_bytes, null_i, null_i, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
+        hipSuccess)
+        return RPD_E_DEVICE;
+    const size_t off_cell_out = bytes_cell, off_orig = 2 * bytes_cell, off_counts = 3 * bytes_cell;
+    const size_t off_tmp = (off_counts + size_t(nc + 1) * 4 + 255) & ~size_t(255);
+    const size_t need = off_tmp + std::max(sort_bytes, scan_bytes) + 256;
+    if (scratch.ensure(need) != RPD_OK) return RPD_E_NOMEM;
+    char *base = static_cast<char *>(scratch.ptr);
+    int *cell_in = reinterpret_cast<int *>(base);
+    int *cell_out = reinterpret_cast<int *>(base + off_cell_out);
+    int *orig = reinterpret_cast<int *>(base + off_orig);
+    int *counts = reinterpret_cast<int *>(base + off_counts);
+    void *tmp = base + off_tmp;
+    if (hipMemsetAsync(counts, 0, size_t(nc + 1) * 4, st) != hipSuccess) return RPD_E_DEVICE;
+    const int tb = 256;
+    const unsigned nb = unsigned((n + tb - 1) / tb);
+    if (prec == 0)
+        bin_cells_kernel<float><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, orig);
+    else
+        bin_cells_kernel<double><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, orig);
+    size_t sb = sort_bytes;
+    if (rocprim::radix_sort_pairs(tmp, sb, cell_in, cell_out, orig, perm, n, 0u, end_bit, st) != hipSuccess)
+        return RPD_E_DEVICE;
+    size_t scb = scan_bytes;
+    if (rocprim::exclusive_scan(tmp, scb, counts, counters, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
+        hipSuccess)
+        return RPD_E_DEVICE;
+    if (prec == 0)
+        gather_seeds_kernel<float><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<float4 *>(stored), stored64);
+    else
+        gather_seeds_kernel<double><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<double4 *>(stored), stored64);
+    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+}
+
+int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
+                   const int *ids, int64_t n_ids, int K, int *out) {
+    if (n_ids <= 0) return RPD_OK;
+    if (K <= 0 || K > 256) return RPD_E_LIMIT;
+    const unsigned blocks = unsigned((n_ids + kKnnWavesPerBlock - 1) / kKnnWavesPerBlock);
+    const dim3 tb(64 * kKnnWavesPerBlock);
+#define RPD_KNN_LAUNCH(R, V4)                                                                       \
+    do {                                                                                            \
+        KnnArgs<R> a{static_cast<const V4 *>(stored), perm, counters, ids, n_ids, dim, K, out};     \
+        if (K <= 64)                                                                                \
+            knn_kernel<R, 1><<<blocks, tb, 0, st>>>(a);                                             \
+        else if (K <= 128)                                                                          \
+            knn_kernel<R, 2><<<blocks, tb, 0, st>>>(a);                                             \
+        else                                                                                        \
+            knn_kernel<R, 4><<<blocks, tb, 0, st>>>(a);                                             \
+    } while (0)
+    if (prec == 0)
+        RPD_KNN_LAUNCH(float, float4);
+    else
+        RPD_KNN_LAUNCH(double, double4);
+#undef RPD_KNN_LAUNCH
+    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+}

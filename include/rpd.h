@@ -1,0 +1,226 @@
+/*
+ * rpd.h -- C-ABI of librpd, the MI355X (gfx950) restricted power diagram
+ * hot path: grid kNN -> per-seed power-cell clipping inside a voxel-gridded
+ * domain -> fused per-cell volume and first-moment integration.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success and
+ * a negative RPD_E* code on failure; rpd_last_error() gives the message.
+ *
+ * Each declaration names the reference interface it replaces
+ * (basselin7u/GPU-Restricted-Power-Diagrams, file:line).
+ */
+#ifndef RPD_H
+#define RPD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define RPD_OK 0
+#define RPD_E_ARG -1      /* invalid argument                               */
+#define RPD_E_STATE -2    /* call out of order (e.g. solve before seeds)    */
+#define RPD_E_DEVICE -3   /* HIP runtime error / no GPU                     */
+#define RPD_E_NOMEM -4    /* device allocation failed                       */
+#define RPD_E_IO -5       /* file could not be read or written             */
+#define RPD_E_LIMIT -6    /* request exceeds a stated limit                 */
+
+/* ---- per-cell status: identical numbering to Status.h:4-16 -------------- */
+enum rpd_status {
+    RPD_VERTEX_OVERFLOW = 0,
+    RPD_PLANE_OVERFLOW = 1,
+    RPD_INCONSISTENT_BOUNDARY = 2,
+    RPD_SECURITY_RADIUS_NOT_REACHED = 3,
+    RPD_SUCCESS = 4,
+    RPD_NEEDS_EXACT_PREDICATES = 5,
+    RPD_EMPTY_CELL = 6,
+    RPD_FIND_ANOTHER_BEGINNING_VERTEX = 7,
+    RPD_TRIANGLE_OVERFLOW = 8,
+    RPD_SEED_ON_BORDER = 9,
+    RPD_HESSIAN_OVERFLOW = 10,
+    RPD_STATUS_NUM = 11
+};
+
+/* NO_CELL_LIMITS of voronoi.h:53: a seed with w <= this is a "border seed"
+   (copied to the output, acts as a wall for its neighbours). */
+#define RPD_NO_CELL_LIMITS (-100000.0)
+
+/* ---- precision forms --------------------------------------------------- */
+enum rpd_precision {
+    RPD_PREC_FP64 = 0,   /* all-FP64 clip + integration (USE_DOUBLE build)    */
+    RPD_PREC_MIXED = 1,  /* FP32 seed-local clip with a certified error filter;
+                            flagged cells recomputed in FP64; FP64 moment
+                            accumulation across domain pieces                */
+    RPD_PREC_FP32 = 2    /* FP32 seed-local clip, no filter, no FP64 redo     */
+};
+
+/* ---- domain grid: content of Grid (Grid.h:14-70) ------------------------ */
+typedef struct {
+    int32_t size;                 /* voxels per axis g; voxel (i,j,k) id
+                                     i + j*g + k*g*g, frame [0,1000]^3        */
+    const int8_t *in_domain;      /* g^3 flags of each voxel's first corner:
+                                     0 outside, 1 on border, 2 inside         */
+    const double *points;         /* 4*num_points xyzw (w ignored)            */
+    int32_t num_points;
+    const int32_t *triangles;     /* 3*num_triangles point ids                */
+    int32_t num_triangles;
+    const int32_t *tri_list;      /* concatenated sorted per-voxel lists      */
+    int32_t num_tri_list;
+    const int32_t *offsets;       /* g^3+1 offsets into tri_list              */
+    const int8_t *centre;         /* nullable: g^3 classes of voxel centres,
+                                     0 outside / 2 inside / 1 unknown; used to
+                                     put the cone apex of the restriction at a
+                                     voxel centre, never on a lattice plane    */
+} rpd_grid_desc;
+
+/* Host-owned grid produced by the helpers below (free with rpd_grid_free). */
+typedef struct rpd_grid rpd_grid;
+
+/* Voxel domain shapes in the unit frame (scaled x1000). A voxel is kept iff
+   its centre is inside the shape. */
+enum rpd_shape {
+    RPD_SHAPE_SPHERE = 0,        /* p0..p2 centre, p3 radius                   */
+    RPD_SHAPE_TORUS = 1,         /* p0..p2 centre, p3 R, p4 r, axis z          */
+    RPD_SHAPE_CUBE = 2,          /* whole grid                                 */
+    RPD_SHAPE_CUBE_MINUS_SPHERE = 3 /* whole grid minus sphere p0..p2, p3     */
+};
+
+/* Voxel mask of an analytic shape: out_mask[g^3] = 1 kept, 0 not kept. */
+int rpd_voxel_mask(int shape, int g, const double *params, uint8_t *out_mask);
+
+/* Grid of a voxel union (replaces utils/createGrid/main.cpp:172-294 for
+   voxel domains): boundary faces between kept and non-kept voxels (grid
+   exterior counts as not kept), 2 triangles each, oriented so that
+   Grid::getVolume() > 0; corner flags 2/0/1 = all/none/some of the 8 incident
+   voxels kept; per-voxel sorted lists by exact closed-box overlap. */
+int rpd_grid_from_voxels(int g, const uint8_t *mask, rpd_grid **out);
+/* Grid::load / Grid::save (Grid.cpp:186-279, 290-344), reference text format. */
+int rpd_grid_load(const char *path, rpd_grid **out);
+int rpd_grid_save(const rpd_grid *grid, const char *path);
+/* Read-only view of a host grid (pointers stay valid until rpd_grid_free). */
+int rpd_grid_view(const rpd_grid *grid, rpd_grid_desc *out);
+void rpd_grid_free(rpd_grid *grid);
+/* Grid::valid (Grid.cpp:60-184) -> 1 valid, 0 invalid. */
+int rpd_grid_valid(const rpd_grid_desc *grid);
+/* Grid::getVolume (Grid.cpp:30-35) and first moments (Grid.cpp:37-53). */
+double rpd_grid_volume(const rpd_grid_desc *grid);
+void rpd_grid_moments(const rpd_grid_desc *grid, double out4[4]);
+
+/* ---- synthetic seeds (SplitMix64, u = (x>>11) * 2^-53) ------------------- */
+enum rpd_seed_kind {
+    RPD_SEEDS_UNIFORM = 0,        /* i.i.d. uniform in [0,1]^3                  */
+    RPD_SEEDS_LATTICE = 1,        /* m^3 lattice (i+1/2)/m + U(+-jitter/m),
+                                     first n in (k,j,i) order                  */
+    RPD_SEEDS_BLUE_NOISE = 2      /* dart throwing, min distance param        */
+};
+/* xyzw[4n] in the [0,1000]^3 frame; weights w_i ~ U[0, wmax] (already in
+   frame units, i.e. scaled x1e6 from the unit frame); p0 = lattice m or
+   dart min distance (unit frame), p1 = lattice jitter fraction. */
+int rpd_generate_seeds(int kind, int64_t n, uint64_t seed, double p0, double p1, double wmax, double *xyzw);
+
+/* ---- solver context ------------------------------------------------------ */
+typedef struct rpd_ctx rpd_ctx;
+typedef struct rpd_domain rpd_domain;
+
+/* Solve parameters: the public caps of VoroAlgo (voronoi.h:199-230). */
+typedef struct {
+    int32_t K, P, V, T;           /* caps of the first pass                   */
+    int32_t precision;            /* enum rpd_precision                       */
+    int32_t max_extra_steps;      /* re-queue passes with caps x1.5 (ref: 5)  */
+    int32_t apex_mode;            /* 0 reference corner apex, 1 (default)
+                                     centre of a triangle-free voxel first    */
+    int32_t reserved;
+} rpd_params;
+
+/* Per-solve statistics (VoroAlgoStat, voronoi.h:60-103). */
+typedef struct {
+    double ms_bin, ms_knn, ms_clip, ms_fallback, ms_total; /* device time   */
+    int32_t status_hist[RPD_STATUS_NUM];  /* final status per cell           */
+    int32_t passes;                 /* clip passes run                       */
+    int32_t fallback_cells;         /* cells recomputed in FP64 (mixed form) */
+    int32_t final_K, final_P, final_V, final_T;
+    int32_t dim;                    /* kNN grid resolution                   */
+    int64_t n;                      /* seeds                                 */
+} rpd_stats;
+
+/* Replaces OpenCLContext::createContext (openCL.cpp:151-207) + VoroAlgo ctor
+   (voronoi.cpp:1041-1073): one context = one HIP device + one stream. */
+int rpd_create(int device, rpd_ctx **out);
+void rpd_destroy(rpd_ctx *ctx);
+const char *rpd_last_error(const rpd_ctx *ctx);
+/* Default parameters (VoroAlgo ctor defaults K25 P35 V40 T100). */
+void rpd_default_params(rpd_params *out);
+
+/* Replaces VoroAlgo::setGrid / GridAccess::init (voronoi.cpp:1079-1082,
+   78-144): copy the grid to HBM; it stays resident until rpd_domain_destroy. */
+int rpd_domain_create(rpd_ctx *ctx, const rpd_grid_desc *grid, rpd_domain **out);
+void rpd_domain_destroy(rpd_domain *dom);
+
+/* Replaces VoroAlgo::setInputPoints(std::vector<real>) (voronoi.cpp:1098-1107)
+   and the KNearests constructor's binning (knearests.cpp:48-222): copies n
+   seeds xyzw (FP64, frame [0,1000]^3, w = power weight, sign as in
+   ConvexCell.cl:324) to HBM and bins them.  Seeds can be released on return. */
+int rpd_set_seeds(rpd_ctx *ctx, const double *xyzw, int64_t n);
+/* Same, from a device pointer already in HBM (setInputPoints(cl_mem,int),
+   voronoi.cpp:1088-1096). */
+int rpd_set_seeds_device(rpd_ctx *ctx, const double *d_xyzw, int64_t n);
+
+/* Replaces VoroAlgo::launch (voronoi.cpp:1244-1403): kNN + clip + restrict +
+   integrate with the status re-queue.  dom may be NULL (domain [0,1000]^3). */
+int rpd_solve(rpd_ctx *ctx, rpd_domain *dom, const rpd_params *prm, rpd_stats *stats);
+
+/* Outputs, valid until the next rpd_set_seeds/rpd_solve on this context. */
+/* Replaces getBarycenters (voronoi.cpp:1173-1183) in the per-original-seed
+   "mg" layout of test_voronoi.cpp:186-211: 4n doubles (m*x, m*y, m*z, m). */
+int rpd_read_moments(rpd_ctx *ctx, double *out4n);
+/* Final per-cell status, original seed order. */
+int rpd_read_status(rpd_ctx *ctx, int32_t *out_n);
+/* Binning permutation perm[stored] = original index (KNearests::getPermutation,
+   knearests.h:101-103). */
+int rpd_read_perm(rpd_ctx *ctx, int32_t *out_n);
+/* Device pointer of the per-original-seed moments (getBarycentersCL). */
+int rpd_moments_device(rpd_ctx *ctx, const double **out_ptr);
+
+/* kNN only (KNearests::buildKnearests, knearests.cpp:238-315): K nearest of
+   every seed, in stored order, ascending (d^2, original index); -1 padded.
+   knn_precision 0: FP32 seed copy, 1: FP64. out: n*K int32, stored ids. */
+int rpd_knn(rpd_ctx *ctx, int32_t K, int32_t knn_precision, int32_t *out_nk);
+
+/* Compaction of used cells (compressResult, voronoi.cpp:513-691) in a
+   deterministic order: cells with m >= keep_volume, ascending original index.
+   out_count receives the number of used cells; out4 (4*count) and
+   out_perm_to_user (count) may be NULL to query the count. */
+int rpd_compact(rpd_ctx *ctx, double keep_volume, int64_t *out_count, double *out4, int32_t *out_perm_to_user);
+
+/* ---- serving --------------------------------------------------------------
+   A request loop for streams of solves with different seed counts and
+   weights against resident domains (the inner step of Lloyd relaxation or
+   semi-discrete transport).  Requests that arrive while others run are
+   admitted as slots and HBM budget free up; admitted requests share the
+   device (continuous batching of their cell work into one queue). */
+typedef struct rpd_server rpd_server;
+typedef struct {
+    int64_t hbm_budget_bytes;     /* admission limit on request footprints   */
+    int32_t max_inflight;         /* requests resident at once               */
+    int32_t reserved;
+} rpd_server_config;
+
+int rpd_server_create(rpd_ctx *ctx, const rpd_server_config *cfg, rpd_server **out);
+void rpd_server_destroy(rpd_server *srv);
+/* Enqueue a request; seeds are copied. Returns a ticket >= 0 in *ticket. */
+int rpd_server_submit(rpd_server *srv, rpd_domain *dom, const double *xyzw, int64_t n, const rpd_params *prm,
+                      int64_t *ticket);
+/* Poll (wait=0) or block (wait=1) for a finished request; copies its 4n
+   moments into out4n (caller sized from the submitted n) and returns the
+   ticket in *done_ticket (-1 if none finished yet).  lat_ms: submit->done. */
+int rpd_server_poll(rpd_server *srv, int wait, int64_t *done_ticket, double *out4n, int64_t cap_n, double *lat_ms);
+/* Run the scheduler until every submitted request has finished. */
+int rpd_server_drain(rpd_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,201 @@
+"""ctypes binding of the CPU oracle (librpd_oracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, where it is the checker; the product
+library never uses it.  The oracle restates the reference algorithm
+(knearests.cpp/knearests.cl, ConvexCell.cl, voronoi.cpp pass logic) on the
+CPU; see rpd_oracle.cpp for the file:line map.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librpd_oracle.so")
+
+STATUS_NAMES = [
+    "vertex_overflow", "plane_overflow", "inconsistent_boundary",
+    "security_radius_not_reached", "success", "needs_exact_predicates",
+    "empty_cell", "find_another_beginning_vertex", "triangle_overflow",
+    "seed_on_border", "hessian_overflow",
+]
+SUCCESS = 4
+
+
+class OrcGrid(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_int),
+        ("in_domain", ctypes.c_void_p),
+        ("points", ctypes.c_void_p),
+        ("num_points", ctypes.c_int),
+        ("triangles", ctypes.c_void_p),
+        ("num_triangles", ctypes.c_int),
+        ("tri_list", ctypes.c_void_p),
+        ("num_tri_list", ctypes.c_int),
+        ("offsets", ctypes.c_void_p),
+        ("centre", ctypes.c_void_p),
+    ]
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "K", "P", "V", "T", "precision", "vhs0", "vhs1", "apex_mode",
+        "fix_exact_requeue", "max_extra_steps", "threads", "single_pass")]
+
+
+class OrcStats(ctypes.Structure):
+    _fields_ = [
+        ("n_pred", ctypes.c_longlong),
+        ("n_radius_vertex", ctypes.c_longlong),
+        ("n_planes", ctypes.c_longlong),
+        ("n_bbox_vertex", ctypes.c_longlong),
+        ("n_integrated_vertex", ctypes.c_longlong),
+        ("n_clip", ctypes.c_longlong),
+        ("status_hist", ctypes.c_int * 11),
+        ("passes", ctypes.c_int),
+        ("final_K", ctypes.c_int),
+        ("final_P", ctypes.c_int),
+        ("final_V", ctypes.c_int),
+        ("final_T", ctypes.c_int),
+        ("seconds_knn", ctypes.c_double),
+        ("seconds_clip", ctypes.c_double),
+        ("seconds_total", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "status_hist"}
+        d["status_hist"] = list(self.status_hist)
+        return d
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library not built: %s (run __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_knn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_knn_dim.argtypes = [ctypes.c_int]
+        L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcParams),
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(OrcStats)]
+        L.orc_clip_cell.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(OrcGrid), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_grid_volume.argtypes = [ctypes.POINTER(OrcGrid)]
+        L.orc_grid_volume.restype = ctypes.c_double
+        L.orc_grid_valid.argtypes = [ctypes.POINTER(OrcGrid)]
+        L.orc_grid_moments.argtypes = [ctypes.POINTER(OrcGrid), ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Grid:
+    """Numpy arrays of a domain grid (the content of the reference's Grid)."""
+
+    def __init__(self, size, in_domain, points, triangles, tri_list, offsets, centre=None):
+        self.size = int(size)
+        self.in_domain = np.ascontiguousarray(in_domain, dtype=np.int8)
+        self.points = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 4)
+        self.triangles = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1)
+        self.tri_list = np.ascontiguousarray(tri_list, dtype=np.int32)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.centre = None if centre is None else np.ascontiguousarray(centre, dtype=np.int8)
+
+    def orc(self):
+        g = OrcGrid()
+        g.size = self.size
+        g.in_domain = _ptr(self.in_domain)
+        g.points = _ptr(self.points)
+        g.num_points = self.points.shape[0]
+        g.triangles = _ptr(self.triangles)
+        g.num_triangles = self.triangles.size // 3
+        g.tri_list = _ptr(self.tri_list)
+        g.num_tri_list = self.tri_list.size
+        g.offsets = _ptr(self.offsets)
+        g.centre = _ptr(self.centre)
+        return g
+
+
+def knn_dim(n):
+    return lib().orc_knn_dim(int(n))
+
+
+def knn(xyzw, K, precision=1, method=0):
+    """Returns (perm, knn[n,K] stored ids, counters)."""
+    xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
+    n = xyzw.shape[0]
+    dim = knn_dim(n)
+    perm = np.empty(n, np.int32)
+    out = np.empty((n, K), np.int32)
+    counters = np.empty(dim ** 3 + 1, np.int32)
+    r = lib().orc_knn(_ptr(xyzw), n, K, precision, method, _ptr(perm), _ptr(out), _ptr(counters))
+    if r < 0:
+        raise RuntimeError("orc_knn failed %d" % r)
+    return perm, out, counters
+
+
+def params(K=25, P=35, V=40, T=100, precision=1, vhs0=2, vhs1=2, apex_mode=1, fix_exact_requeue=1,
+           max_extra_steps=5, threads=0, single_pass=0):
+    p = OrcParams()
+    for k, v in dict(K=K, P=P, V=V, T=T, precision=precision, vhs0=vhs0, vhs1=vhs1, apex_mode=apex_mode,
+                     fix_exact_requeue=fix_exact_requeue, max_extra_steps=max_extra_steps, threads=threads,
+                     single_pass=single_pass).items():
+        setattr(p, k, int(v))
+    return p
+
+
+def solve(xyzw, grid=None, **kw):
+    """Restricted power diagram; returns (mg[n,4] per original seed, status[n], kused[n], stats dict)."""
+    xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
+    n = xyzw.shape[0]
+    out = np.zeros((n, 4), np.float64)
+    status = np.zeros(n, np.int32)
+    kused = np.zeros(n, np.int32)
+    st = OrcStats()
+    p = params(**kw)
+    g = grid.orc() if grid is not None else None
+    r = lib().orc_solve(_ptr(xyzw), n, ctypes.byref(g) if g is not None else None, ctypes.byref(p), _ptr(out),
+                        _ptr(status), _ptr(kused), ctypes.byref(st))
+    if r < 0:
+        raise RuntimeError("orc_solve failed %d" % r)
+    return out, status, kused, st.as_dict()
+
+
+def clip_cell(xyzw, seed, neigs, K, P=64, V=128, T=0, precision=1, vhs=2, grid=None, apex_mode=1, out_domain=0):
+    xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
+    neigs = np.ascontiguousarray(neigs, dtype=np.int32)
+    out = np.zeros(4, np.float64)
+    nv = ctypes.c_int(0)
+    npl = ctypes.c_int(0)
+    g = grid.orc() if grid is not None else None
+    st = lib().orc_clip_cell(_ptr(xyzw), xyzw.shape[0], int(seed), _ptr(neigs), int(K), int(P), int(V), int(T),
+                             int(precision), int(vhs), ctypes.byref(g) if g is not None else None, int(apex_mode),
+                             int(out_domain), _ptr(out), ctypes.byref(nv), ctypes.byref(npl))
+    return st, out, nv.value, npl.value
+
+
+def grid_volume(grid):
+    g = grid.orc()
+    return lib().orc_grid_volume(ctypes.byref(g))
+
+
+def grid_valid(grid):
+    g = grid.orc()
+    return bool(lib().orc_grid_valid(ctypes.byref(g)))
+
+
+def grid_moments(grid):
+    g = grid.orc()
+    out = np.zeros(4, np.float64)
+    lib().orc_grid_moments(ctypes.byref(g), _ptr(out))
+    return out

@@ -1,0 +1,1140 @@
+// rpd_oracle.cpp -- CPU oracle for the restricted power diagram hot path.
+//
+// TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py as the checker.  The shipped library never
+// links or calls this file.
+//
+// It restates, in plain C++ on the CPU, the algorithm of the reference:
+//   * kNN binning and query    knearests.cpp:35-222, knearests.cl:12-78
+//   * convex-cell clipping     ConvexCell.cl:278-563 (CCInit, CCNewPlane,
+//                              CCClipByPlane, CCComputeBoundary, CCNewVertex,
+//                              CCVertexIsInHalfSpace, CCIsSecurityRadiusReached)
+//   * restriction to a grid    ConvexCell.cl:918-1116, merge :722-749,
+//                              CCClipTetFromPointsInfinite :565-575
+//   * integration              CCExportBaryVolume ConvexCell.cl:577-619
+//   * pass policy              voronoi.cpp:1244-1376 (first / tr_overflow /
+//                              extra passes, caps x1.5)
+//   * per-precision constants  voronoi.cpp:915-947
+// with these stated definitions where the reference is non-deterministic or
+// reads out of bounds:
+//   * kNN order is canonical: ascending (d^2, original index), self excluded,
+//     d^2 = ((dx*dx + dy*dy) + dz*dz) unfused in the seed precision; the
+//     search is exact (per-axis ring bounds, no row wrap-around, no ring cap).
+//   * binning is stable: inside a grid cell, seeds keep original order.
+//   * a kNN list shorter than K (fewer than K+1 seeds) is padded with -1 and a
+//     cell that consumed every other seed is complete.
+//   * GRID_VOXEL_SIZE is the exact 1000/g in the working precision (the
+//     reference injects it printed with 6 significant digits).
+// Build with -ffp-contract=off so every product and sum rounds separately.
+
+#include "rpd_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <set>
+#include <utility>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// Status numbering of Status.h:4-16.
+enum {
+    kVertexOverflow = 0,
+    kPlaneOverflow = 1,
+    kInconsistentBoundary = 2,
+    kSecurityRadiusNotReached = 3,
+    kSuccess = 4,
+    kNeedsExactPredicates = 5,
+    kEmptyCell = 6,
+    kFindAnotherBeginningVertex = 7,
+    kTriangleOverflow = 8,
+    kSeedOnBorder = 9,
+    kHessianOverflow = 10,
+    kNumStatus = 11
+};
+
+constexpr double kNoCellLimits = -100000.0;  // voronoi.h:53
+constexpr int kEndOfList = 255;              // ConvexCell.cl:132
+
+template <class R>
+struct R4 {
+    R x, y, z, w;
+};
+template <class R>
+inline R4<R> mk4(R x, R y, R z, R w) { return R4<R>{x, y, z, w}; }
+template <class R>
+inline R4<R> sub4(const R4<R> &a, const R4<R> &b) { return mk4<R>(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+template <class R>
+inline R4<R> add4(const R4<R> &a, const R4<R> &b) { return mk4<R>(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+template <class R>
+inline R4<R> scale4(R s, const R4<R> &a) { return mk4<R>(s * a.x, s * a.y, s * a.z, s * a.w); }
+template <class R>
+inline R dot3(const R4<R> &a, const R4<R> &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R>
+inline R dot4(const R4<R> &a, const R4<R> &b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+// OpenCL cross() on 4-vectors: xyz cross product, w = 0.
+template <class R>
+inline R4<R> cross3(const R4<R> &a, const R4<R> &b) {
+    return mk4<R>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x, R(0));
+}
+template <class R>
+inline R det2(R a11, R a12, R a21, R a22) { return a11 * a22 - a12 * a21; }
+// ConvexCell.cl:72-77 -- rows A, B, C.
+template <class R>
+inline R det3(R a11, R a12, R a13, R a21, R a22, R a23, R a31, R a32, R a33) {
+    return a11 * det2(a22, a23, a32, a33) - a21 * det2(a12, a13, a32, a33) + a31 * det2(a12, a13, a22, a23);
+}
+template <class R>
+inline R det3r4(const R4<R> &A, const R4<R> &B, const R4<R> &C) {
+    return det3<R>(A.x, A.y, A.z, B.x, B.y, B.z, C.x, C.y, C.z);
+}
+// ConvexCell.cl:79-97 (and det4x4D :104-122 for T = double).
+template <class T>
+inline T det4(T a11, T a12, T a13, T a14, T a21, T a22, T a23, T a24, T a31, T a32, T a33, T a34,
+              T a41, T a42, T a43, T a44) {
+    const T m12 = a21 * a12 - a11 * a22;
+    const T m13 = a31 * a12 - a11 * a32;
+    const T m14 = a41 * a12 - a11 * a42;
+    const T m23 = a31 * a22 - a21 * a32;
+    const T m24 = a41 * a22 - a21 * a42;
+    const T m34 = a41 * a32 - a31 * a42;
+    const T m123 = m23 * a13 - m13 * a23 + m12 * a33;
+    const T m124 = m24 * a13 - m14 * a23 + m12 * a43;
+    const T m134 = m34 * a13 - m14 * a33 + m13 * a43;
+    const T m234 = m34 * a23 - m24 * a33 + m23 * a43;
+    return (m234 * a14 - m134 * a24 + m124 * a34 - m123 * a44);
+}
+// ConvexCell.cl:125-129
+template <class R>
+inline R4<R> plane_from_points(const R4<R> &A, const R4<R> &B, const R4<R> &C) {
+    R4<R> p = cross3(sub4(B, A), sub4(C, A));
+    p.w = -dot3(p, A);
+    return p;
+}
+
+struct Counters {
+    long long n_pred = 0, n_radius_vertex = 0, n_planes = 0, n_bbox_vertex = 0, n_integrated_vertex = 0,
+              n_clip = 0;
+    void add(const Counters &o) {
+        n_pred += o.n_pred;
+        n_radius_vertex += o.n_radius_vertex;
+        n_planes += o.n_planes;
+        n_bbox_vertex += o.n_bbox_vertex;
+        n_integrated_vertex += o.n_integrated_vertex;
+        n_clip += o.n_clip;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Grid view in the working precision (GridAccess, voronoi.cpp:42-144)
+// ---------------------------------------------------------------------------
+template <class R>
+struct GridView {
+    int g = 0;
+    R vs = 0;  // voxel size 1000/g
+    const signed char *flags = nullptr;
+    std::vector<R4<R>> pts;
+    const int *tris = nullptr;
+    const int *list = nullptr;
+    const int *offs = nullptr;
+    const signed char *centre = nullptr;  // voxel-centre classes (nullable)
+    bool has_border_vertices = false;
+    int id(int x, int y, int z) const { return x + y * g + z * g * g; }
+};
+
+template <class R>
+GridView<R> make_grid_view(const orc_grid *gr) {
+    GridView<R> v;
+    if (!gr) return v;
+    v.g = gr->size;
+    v.vs = R(1000) / R(gr->size);
+    v.flags = gr->in_domain;
+    v.pts.resize(size_t(gr->num_points));
+    for (int i = 0; i < gr->num_points; ++i)
+        v.pts[size_t(i)] = mk4<R>(R(gr->points[4 * i]), R(gr->points[4 * i + 1]), R(gr->points[4 * i + 2]), R(0));
+    v.tris = gr->triangles;
+    v.list = gr->tri_list;
+    v.offs = gr->offsets;
+    v.centre = gr->centre;
+    size_t nv = size_t(v.g) * size_t(v.g) * size_t(v.g);
+    for (size_t i = 0; i < nv; ++i)
+        if (v.flags[i] == 1) {
+            v.has_border_vertices = true;
+            break;
+        }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Convex cell (ConvexCell.cl:134-149), one private instance per seed.
+// ---------------------------------------------------------------------------
+template <class R>
+struct Cell {
+    using V3 = std::array<uint8_t, 3>;
+    R4<R> seed;
+    const R4<R> *pts = nullptr;
+    int status = kSuccess;
+    int P = 0, V = 0, vhs = 2;
+    int nb_vertex = 0, nb_plane = 0;
+    R4<R> planes[256];
+    V3 verts[256];
+    uint8_t next[256];
+    Counters *ctr = nullptr;
+
+    // CCInit ConvexCell.cl:278-308
+    void init(const R4<R> *p_pts, int id, R cube_eps) {
+        pts = p_pts;
+        seed = p_pts[id];
+        status = kSuccess;
+        const R vmin = -cube_eps, vmax = R(1000) + cube_eps;
+        planes[0] = mk4<R>(1, 0, 0, -vmin);
+        planes[1] = mk4<R>(-1, 0, 0, vmax);
+        planes[2] = mk4<R>(0, 1, 0, -vmin);
+        planes[3] = mk4<R>(0, -1, 0, vmax);
+        planes[4] = mk4<R>(0, 0, 1, -vmin);
+        planes[5] = mk4<R>(0, 0, -1, vmax);
+        nb_plane = 6;
+        static const uint8_t cube[8][3] = {{2, 5, 0}, {5, 3, 0}, {1, 5, 2}, {5, 1, 3},
+                                           {4, 2, 0}, {4, 0, 3}, {2, 4, 1}, {4, 3, 1}};
+        for (int i = 0; i < 8; ++i) verts[i] = V3{cube[i][0], cube[i][1], cube[i][2]};
+        nb_vertex = 8;
+    }
+
+    // CCNewPlane ConvexCell.cl:314-333
+    int new_plane(int vid) {
+        if (nb_plane >= P) {
+            status = kPlaneOverflow;
+            return -1;
+        }
+        ctr->n_planes++;
+        const R4<R> B = pts[vid];
+        const R4<R> dir = sub4(seed, B);
+        const R4<R> ave2 = add4(seed, B);
+        R dt = dot3(ave2, dir) - (B.w - seed.w);
+        if (double(B.w) <= kNoCellLimits) dt = R(2) * dot3(B, dir);
+        const R nrm = std::sqrt(dot3(dir, dir));
+        planes[nb_plane] = mk4<R>(dir.x / nrm, dir.y / nrm, dir.z / nrm, -dt / R(2) / nrm);
+        return nb_plane++;
+    }
+
+    void new_vertex(int i, int j, int k) {  // CCNewVertex :335-342
+        if (nb_vertex >= V) {
+            status = kVertexOverflow;
+            return;
+        }
+        verts[nb_vertex++] = V3{uint8_t(i), uint8_t(j), uint8_t(k)};
+    }
+
+    // CCComputeVertexCoordinates :344-362
+    R4<R> vertex_coords(const V3 &t, bool persp) const {
+        const R4<R> p1 = planes[t[0]], p2 = planes[t[1]], p3 = planes[t[2]];
+        const R m12 = p2.x * p3.y - p2.y * p3.x;
+        const R m13 = p2.x * p3.z - p2.z * p3.x;
+        const R m14 = p2.x * p3.w - p2.w * p3.x;
+        const R m23 = p2.y * p3.z - p2.z * p3.y;
+        const R m24 = p2.y * p3.w - p2.w * p3.y;
+        const R m34 = p2.z * p3.w - p2.w * p3.z;
+        R4<R> r;
+        r.x = -p1.w * m23 - p1.y * m34 + p1.z * m24;
+        r.y = p1.x * m34 + p1.w * m13 - p1.z * m14;
+        r.z = -p1.x * m24 + p1.y * m14 - p1.w * m12;
+        r.w = p1.x * m23 - p1.y * m13 + p1.z * m12;
+        if (persp) return mk4<R>(r.x / r.w, r.y / r.w, r.z / r.w, R(1));
+        return r;
+    }
+
+    // CCIsSecurityRadiusReached :364-382 (weights ignored, as the reference)
+    bool radius_reached(const R4<R> &last) const {
+        R vd = 0;
+        for (int i = 0; i < nb_vertex; ++i) {
+            const R4<R> pc = vertex_coords(verts[i], true);
+            const R4<R> d = sub4(pc, seed);
+            const R d2 = dot3(d, d);
+            vd = std::max(d2, vd);
+        }
+        ctr->n_radius_vertex += nb_vertex;
+        const R4<R> d = sub4(last, seed);
+        const R d2 = dot3(d, d);
+        return d2 > R(4) * vd;
+    }
+
+    // CCVertexIsInHalfSpace :400-461
+    bool in_half_space(const V3 &t, const R4<R> &e) {
+        ctr->n_pred++;
+        const R4<R> a = planes[t[0]], b = planes[t[1]], c = planes[t[2]];
+        if (vhs == 2) {
+            const double d = det4<double>(double(a.x), double(b.x), double(c.x), double(e.x), double(a.y),
+                                          double(b.y), double(c.y), double(e.y), double(a.z), double(b.z),
+                                          double(c.z), double(e.z), double(a.w), double(b.w), double(c.w),
+                                          double(e.w));
+            return d > 0.0;
+        }
+        if (vhs == 1) {
+            // filtered real evaluation with the reference's magnitude bound
+            auto fmaxabs = [](R u, R v) { return std::max(std::fabs(u), std::fabs(v)); };
+            R tmp;
+            R m12 = b.x * a.y; tmp = -a.x * b.y; R M12 = fmaxabs(m12, tmp); m12 += tmp;
+            R m13 = c.x * a.y; tmp = -a.x * c.y; R M13 = fmaxabs(m13, tmp); m13 += tmp;
+            R m14 = e.x * a.y; tmp = -a.x * e.y; R M14 = fmaxabs(m14, tmp); m14 += tmp;
+            R m23 = c.x * b.y; tmp = -b.x * c.y; R M23 = fmaxabs(m23, tmp); m23 += tmp;
+            R m24 = e.x * b.y; tmp = -b.x * e.y; R M24 = fmaxabs(m24, tmp); m24 += tmp;
+            R m34 = e.x * c.y; tmp = -c.x * e.y; R M34 = fmaxabs(m34, tmp); m34 += tmp;
+            const R m123 = m23 * a.z - m13 * b.z + m12 * c.z;
+            const R M123 = std::max(std::max(std::fabs(M23 * a.z), std::fabs(M13 * b.z)), std::fabs(M12 * c.z));
+            const R m124 = m24 * a.z - m14 * b.z + m12 * e.z;
+            const R M124 = std::max(std::max(std::fabs(M24 * a.z), std::fabs(M14 * b.z)), std::fabs(M12 * e.z));
+            const R m134 = m34 * a.z - m14 * c.z + m13 * e.z;
+            const R M134 = std::max(std::max(std::fabs(M34 * a.z), std::fabs(M14 * c.z)), std::fabs(M13 * e.z));
+            const R m234 = m34 * b.z - m24 * c.z + m23 * e.z;
+            const R M234 = std::max(std::max(std::fabs(M34 * b.z), std::fabs(M24 * c.z)), std::fabs(M23 * e.z));
+            const R res = m234 * a.w - m134 * b.w + m124 * c.w - m123 * e.w;
+            const R M = std::max(std::max(std::fabs(M234 * a.w), std::fabs(M134 * b.w)),
+                                 std::max(std::fabs(M124 * c.w), std::fabs(M123 * e.w)));
+            if (std::fabs(res) < R(3.e-6) * M) {
+                status = kNeedsExactPredicates;
+                return res > R(-2.e-7) * M;
+            }
+            return res > R(0);
+        }
+        const R d = det4<R>(a.x, b.x, c.x, e.x, a.y, b.y, c.y, e.y, a.z, b.z, c.z, e.z, a.w, b.w, c.w, e.w);
+        return d > R(0);
+    }
+
+    // CCComputeBoundary :463-515 -- grows the removed region triangle by
+    // triangle, keeping a single simple boundary loop in next[].
+    int compute_boundary(int nb_removed) {
+        for (int i = 0; i < P; ++i) next[i] = kEndOfList;
+        int first = kEndOfList;
+        int nb_iter = 0;
+        int t = nb_vertex;
+        while (nb_removed > 0) {
+            if (nb_iter++ > 1000) {
+                status = kInconsistentBoundary;
+                return first;
+            }
+            const V3 tri = verts[t];
+            bool border[3], opp[3];
+            for (int e = 0; e < 3; ++e) border[e] = next[tri[e]] != kEndOfList;
+            for (int e = 0; e < 3; ++e) opp[e] = next[tri[(e + 1) % 3]] == tri[e];
+            bool simple = true;
+            for (int e = 0; e < 3; ++e)
+                if (!opp[e] && !opp[(e + 1) % 3] && border[(e + 1) % 3]) simple = false;
+            if (!opp[0] && !opp[1] && !opp[2]) {
+                if (first == kEndOfList) {
+                    for (int e = 0; e < 3; ++e) next[tri[e]] = tri[(e + 1) % 3];
+                    first = tri[0];
+                } else {
+                    simple = false;
+                }
+            }
+            if (!simple) {
+                t++;
+                if (t == nb_vertex + nb_removed) t = nb_vertex;
+                continue;
+            }
+            for (int e = 0; e < 3; ++e)
+                if (!opp[e]) next[tri[e]] = tri[(e + 1) % 3];
+            for (int e = 0; e < 3; ++e)
+                if (opp[e] && opp[(e + 1) % 3]) {
+                    const int p = tri[(e + 1) % 3];
+                    if (first == p) first = next[p];
+                    next[p] = kEndOfList;
+                }
+            std::swap(verts[t], verts[nb_vertex + nb_removed - 1]);
+            t = nb_vertex;
+            nb_removed--;
+        }
+        return first;
+    }
+
+    // CCClipByPlane :517-563
+    void clip_by_plane(int vid) {
+        if (status == kPlaneOverflow) return;
+        ctr->n_clip++;
+        const R4<R> eqn = planes[vid];
+        int nb_removed = 0;
+        int i = 0;
+        while (i < nb_vertex) {
+            if (in_half_space(verts[i], eqn)) {
+                nb_vertex--;
+                std::swap(verts[i], verts[nb_vertex]);
+                nb_removed++;
+            } else {
+                i++;
+            }
+        }
+        if (nb_vertex < 1) {
+            status = kEmptyCell;
+            return;
+        }
+        if (status == kNeedsExactPredicates) return;
+        if (nb_removed == 0) {
+            --nb_plane;
+            return;
+        }
+        const int first = compute_boundary(nb_removed);
+        if (status != kSuccess) return;
+        if (first == kEndOfList) return;
+        int cir = first;
+        do {
+            const int nc = next[cir];
+            new_vertex(vid, cir, nc);
+            if (status != kSuccess) return;
+            cir = nc;
+        } while (cir != first);
+    }
+
+    // CCClipTetFromPointsInfinite :565-575
+    void clip_tet(const R4<R> &A, const R4<R> &B, const R4<R> &C, const R4<R> &D) {
+        const R4<R> *tri[4][3] = {{&A, &B, &C}, {&A, &D, &B}, {&A, &C, &D}, {&C, &D, &B}};
+        for (int f = 0; f < 4; ++f) {
+            if (nb_plane >= P) {
+                status = kPlaneOverflow;
+                return;
+            }
+            ctr->n_planes++;
+            planes[nb_plane++] = plane_from_points(*tri[f][0], *tri[f][1], *tri[f][2]);
+            clip_by_plane(nb_plane - 1);
+            if (status != kSuccess) return;
+        }
+    }
+
+    // CCExportBaryVolume :577-619.  Accumulates factor/6 * sum into *out.
+    void export_bary_volume(R4<R> *out, R factor, R vol_eps2) {
+        R4<R> s = mk4<R>(0, 0, 0, 0);
+        const R4<R> C = seed;
+        ctr->n_integrated_vertex += nb_vertex;
+        for (int ve = 0; ve < nb_vertex; ++ve) {
+            const R4<R> A = vertex_coords(verts[ve], true);
+            const R4<R> AC = sub4(C, A);
+            R4<R> pl[3], onPlane[3], onLine[3];
+            for (int i = 0; i < 3; ++i) pl[i] = planes[verts[ve][i]];
+            for (int i = 0; i < 3; ++i) {
+                const R4<R> ni = pl[i];
+                onPlane[i] = sub4(AC, scale4(dot3(AC, ni) / dot3(ni, ni), ni));
+            }
+            for (int i = 0; i < 3; ++i) {
+                const R4<R> nn = cross3(pl[i], pl[(i + 1) % 3]);
+                const R nn2 = dot4(nn, nn);
+                if (nn2 < vol_eps2 * dot3(pl[i], pl[i]) * dot3(pl[(i + 1) % 3], pl[(i + 1) % 3]))
+                    onLine[i] = mk4<R>(0, 0, 0, 0);
+                else
+                    onLine[i] = scale4(dot4(AC, nn) / nn2, nn);
+            }
+            for (int i = 0; i < 3; ++i) {
+                for (int st = 0; st < 2; ++st) {
+                    const int j = (i + 3 - st) % 3;
+                    const R w = R(2 * st - 1) * det3r4(AC, onPlane[i], onLine[j]);
+                    s.w += w;
+                    const R4<R> sm = add4(add4(AC, onPlane[i]), onLine[j]);
+                    s.x += w * (A.x + R(0.25) * sm.x);
+                    s.y += w * (A.y + R(0.25) * sm.y);
+                    s.z += w * (A.z + R(0.25) * sm.z);
+                }
+            }
+        }
+        const R f = factor / R(6);
+        *out = add4(*out, scale4(f, s));
+    }
+};
+
+template <class R>
+struct CellResult {
+    int status = kSuccess;
+    int kused = 0;
+    int maxP = 0, maxV = 0, ntri = 0;
+};
+
+// Sorted union of two sorted unique lists (merge, ConvexCell.cl:722-749).
+// Returns false when the union would exceed cap entries.
+inline bool merge_union(const std::vector<int> &a, const int *b, int nb, int cap, std::vector<int> &out) {
+    out.clear();
+    size_t i = 0;
+    int j = 0;
+    while (i < a.size() || j < nb) {
+        int v;
+        if (j >= nb || (i < a.size() && a[i] < b[j])) {
+            v = a[i++];
+        } else if (i >= a.size() || b[j] < a[i]) {
+            v = b[j++];
+        } else {
+            v = a[i];
+            ++i;
+            ++j;
+        }
+        if (int(out.size()) >= cap) return false;
+        out.push_back(v);
+    }
+    return true;
+}
+
+// compute_voro_cell ConvexCell.cl:812-1152, one seed.  pts are in the
+// working precision; neigs holds K indices into pts (-1 = none).
+template <class R>
+CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, int P, int V, int T, int vhs,
+                           const GridView<R> *grid, int apex_mode, bool out_domain, R4<R> *out, Counters *ctr) {
+    CellResult<R> res;
+    const R vol_eps2 = sizeof(R) == sizeof(float) ? R(1.e-6) : R(1.e-12);  // voronoi.cpp:919-922
+    if (double(pts[seed].w) <= kNoCellLimits) {
+        *out = pts[seed];
+        return res;
+    }
+    *out = mk4<R>(0, 0, 0, 0);
+    Cell<R> cc;
+    cc.P = P;
+    cc.V = V;
+    cc.vhs = vhs;
+    cc.ctr = ctr;
+    cc.init(pts, seed, grid && grid->g > 0 ? R(0.1) : R(0));
+    bool complete = false;  // every other seed consumed
+    int i = 0;
+    for (; i < K; ++i) {
+        const int z = neigs[i];
+        if (z < 0) {
+            complete = true;
+            break;
+        }
+        const int cur = cc.new_plane(z);
+        cc.clip_by_plane(cur);
+        if (cc.status != kSuccess) {
+            res.status = cc.status;
+            res.kused = i + 1;
+            return res;
+        }
+        if (cc.radius_reached(pts[z])) break;
+    }
+    res.kused = std::min(i + 1, K);
+    if (complete) res.kused = i;
+    res.maxP = cc.nb_plane;
+    res.maxV = cc.nb_vertex;
+    if (!complete && !cc.radius_reached(pts[neigs[K - 1]])) {
+        res.status = kSecurityRadiusNotReached;
+        return res;
+    }
+    bool add_volume = true;
+    R4<R> Pt0 = mk4<R>(0, 0, 0, 1);
+    std::vector<int> tr, tmp;
+    if (grid && grid->g > 0) {
+        const int g = grid->g;
+        const R vs = grid->vs;
+        R bmin[3] = {1000, 1000, 1000}, bmax[3] = {0, 0, 0};
+        ctr->n_bbox_vertex += cc.nb_vertex;
+        for (int ve = 0; ve < cc.nb_vertex; ++ve) {
+            const R4<R> p = cc.vertex_coords(cc.verts[ve], true);
+            const R c[3] = {p.x, p.y, p.z};
+            for (int d = 0; d < 3; ++d) {
+                if (bmin[d] > c[d]) bmin[d] = c[d];
+                if (bmax[d] < c[d]) bmax[d] = c[d];
+            }
+        }
+        int vmin[3], vmax[3];
+        for (int d = 0; d < 3; ++d) vmin[d] = std::max(0, int(bmin[d] / vs));
+        for (int d = 0; d < 3; ++d) vmax[d] = std::min(g - 1, int(bmax[d] / vs));
+        int in_domain = 1;
+        // apex_mode 1: first voxel (z,y,x order) whose centre class is known;
+        // without an explicit class array, a triangle-free voxel takes the
+        // class of its first corner when that corner is not on the border.
+        int free_flag = 1;
+        R4<R> freePt = Pt0;
+        for (int z = vmin[2]; z <= vmax[2]; z++) {
+            for (int y = vmin[1]; y <= vmax[1]; y++) {
+                for (int x = vmin[0]; x <= vmax[0]; x++) {
+                    const int id = grid->id(x, y, z);
+                    const signed char f = grid->flags[id];
+                    if (in_domain == 1 && f != 1) {
+                        in_domain = f;
+                        Pt0 = mk4<R>(R(x) * vs, R(y) * vs, R(z) * vs, R(1));
+                    }
+                    const int o0 = grid->offs[id], o1 = grid->offs[id + 1];
+                    if (free_flag == 1) {
+                        const int cf = grid->centre ? grid->centre[id] : (o0 == o1 ? f : 1);
+                        if (cf != 1) {
+                            free_flag = cf;
+                            freePt = mk4<R>((R(x) + R(0.5)) * vs, (R(y) + R(0.5)) * vs, (R(z) + R(0.5)) * vs, R(1));
+                        }
+                    }
+                    if (o0 == o1) continue;
+                    if (T == 0) {
+                        res.status = kTriangleOverflow;
+                        return res;
+                    }
+                    if (!merge_union(tr, grid->list + o0, o1 - o0, T, tmp)) {
+                        res.status = kTriangleOverflow;
+                        return res;
+                    }
+                    tr.swap(tmp);
+                }
+            }
+        }
+        res.ntri = int(tr.size());
+        if (apex_mode == 1 && free_flag != 1) {
+            in_domain = free_flag;
+            Pt0 = freePt;
+        }
+        if (in_domain == 1) {
+            if (!out_domain) {
+                res.status = kFindAnotherBeginningVertex;
+                return res;
+            }
+            // GRID_OUT_DOMAIN fallback ConvexCell.cl:985-1051
+            auto flag = [&](int x, int y, int z) { return grid->flags[grid->id(x, y, z)]; };
+            auto corner = [&](int x, int y, int z) { return mk4<R>(R(x) * vs, R(y) * vs, R(z) * vs, R(1)); };
+            if (vmax[2] + 1 < g) {
+                const int z = vmax[2] + 1;
+                for (int y = vmin[1]; y <= vmax[1] + 1; y++) {
+                    if (y >= g) break;
+                    for (int x = vmin[0]; x <= vmax[0] + 1; x++) {
+                        if (x >= g) break;
+                        if (flag(x, y, z) != 1) {
+                            in_domain = flag(x, y, z);
+                            Pt0 = corner(x, y, z);
+                            break;
+                        }
+                    }
+                    if (in_domain != 1) break;
+                }
+            } else {
+                in_domain = 0;
+                Pt0 = corner(vmin[0], vmin[1], vmax[2] + 1);
+            }
+            if (in_domain == 1 && vmax[1] + 1 < g) {
+                const int y = vmax[1] + 1;
+                for (int z = vmin[2]; z <= vmax[2]; z++) {
+                    for (int x = vmin[0]; x <= vmax[0] + 1; x++) {
+                        if (x >= g) break;
+                        if (flag(x, y, z) != 1) {
+                            in_domain = flag(x, y, z);
+                            Pt0 = corner(x, y, z);
+                            break;
+                        }
+                    }
+                    if (in_domain != 1) break;
+                }
+            } else if (in_domain == 1) {
+                in_domain = 0;
+                Pt0 = corner(vmin[0], vmax[1] + 1, vmin[2]);
+            }
+            if (in_domain == 1 && vmax[0] + 1 < g) {
+                const int x = vmax[0] + 1;
+                for (int z = vmin[2]; z <= vmax[2]; z++) {
+                    for (int y = vmin[1]; y <= vmax[1]; y++) {
+                        if (flag(x, y, z) != 1) {
+                            in_domain = flag(x, y, z);
+                            Pt0 = corner(x, y, z);
+                            break;
+                        }
+                    }
+                    if (in_domain != 1) break;
+                }
+            } else if (in_domain == 1) {
+                in_domain = 0;
+                Pt0 = corner(vmax[0] + 1, vmin[1], vmin[2]);
+            }
+            if (in_domain == 1) {
+                res.status = kFindAnotherBeginningVertex;
+                return res;
+            }
+        }
+        add_volume = in_domain == 2;
+    }
+    if (add_volume) cc.export_bary_volume(out, R(1), vol_eps2);
+    if (!tr.empty()) {
+        // backup / restore ConvexCell.cl:692-715
+        const int bk_nv = cc.nb_vertex, bk_np = cc.nb_plane, bk_st = cc.status;
+        std::vector<typename Cell<R>::V3> bk(cc.verts, cc.verts + cc.nb_vertex);
+        for (size_t t = 0; t < tr.size(); ++t) {
+            const int tid = tr[t];
+            R4<R> P1 = grid->pts[size_t(grid->tris[3 * tid + 0])];
+            R4<R> P2 = grid->pts[size_t(grid->tris[3 * tid + 1])];
+            R4<R> P3 = grid->pts[size_t(grid->tris[3 * tid + 2])];
+            const bool go_out = det3r4(sub4(P1, Pt0), sub4(P2, Pt0), sub4(P3, Pt0)) > R(0);
+            if (!go_out) std::swap(P2, P3);
+            cc.clip_tet(Pt0, P1, P2, P3);
+            res.maxP = std::max(res.maxP, cc.nb_plane);
+            res.maxV = std::max(res.maxV, cc.nb_vertex);
+            if (cc.status == kEmptyCell) {
+                cc.nb_vertex = bk_nv;
+                cc.nb_plane = bk_np;
+                std::copy(bk.begin(), bk.end(), cc.verts);
+                cc.status = kSuccess;
+                continue;
+            }
+            if (cc.status != kSuccess) {
+                // the reference zeroes only .w here (ConvexCell.cl:1104-1107);
+                // the oracle clears all four moments of the failed cell
+                *out = mk4<R>(0, 0, 0, 0);
+                res.status = cc.status;
+                return res;
+            }
+            cc.export_bary_volume(out, go_out ? R(1) : R(-1), vol_eps2);
+            cc.nb_vertex = bk_nv;
+            cc.nb_plane = bk_np;
+            std::copy(bk.begin(), bk.end(), cc.verts);
+            cc.status = bk_st;
+        }
+    }
+    res.status = kSuccess;
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// kNN (knearests.cpp / knearests.cl)
+// ---------------------------------------------------------------------------
+int knn_dim(int n) {
+    // knearests.cpp:63, evaluated in double
+    return std::max(1, int(std::lround(std::cbrt(double(n) / 3.1))));
+}
+
+// cellFromPoint knearests.cpp:35-46: floor(p * (real)dim / 1000) per axis.
+template <class R>
+inline int cell_coord(R c, int dim) {
+    int i = int(std::floor(c * R(dim) / R(1000.f)));
+    return std::max(0, std::min(i, dim - 1));
+}
+
+template <class R>
+struct KnnGrid {
+    int dim = 0;
+    std::vector<int> counters;  // dim^3+1 exclusive prefix
+    std::vector<int> perm;      // stored -> original
+    std::vector<R4<R>> stored;  // sorted seeds
+    std::vector<int> cellOf;    // per stored point: cell index
+};
+
+template <class R>
+KnnGrid<R> build_grid(const std::vector<R4<R>> &pts) {
+    KnnGrid<R> kg;
+    const int n = int(pts.size());
+    const int dim = knn_dim(n);
+    kg.dim = dim;
+    const size_t nc = size_t(dim) * dim * dim;
+    std::vector<int> cell(static_cast<size_t>(n));
+    kg.counters.assign(nc + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        const int c = cell_coord(pts[size_t(i)].x, dim) + cell_coord(pts[size_t(i)].y, dim) * dim +
+                      cell_coord(pts[size_t(i)].z, dim) * dim * dim;
+        cell[size_t(i)] = c;
+        kg.counters[size_t(c)]++;
+    }
+    int acc = 0;
+    for (auto &c : kg.counters) {
+        const int v = c;
+        c = acc;
+        acc += v;
+    }
+    std::vector<int> cursor(kg.counters.begin(), kg.counters.end() - 1);
+    kg.perm.resize(size_t(n));
+    kg.stored.resize(size_t(n));
+    kg.cellOf.resize(size_t(n));
+    for (int i = 0; i < n; ++i) {  // stable: original order inside a cell
+        const int pos = cursor[size_t(cell[size_t(i)])]++;
+        kg.perm[size_t(pos)] = i;
+        kg.stored[size_t(pos)] = pts[size_t(i)];
+        kg.cellOf[size_t(pos)] = cell[size_t(i)];
+    }
+    return kg;
+}
+
+template <class R>
+inline R dist2(const R4<R> &a, const R4<R> &b) {
+    const R dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+struct KnnKey {
+    double d;
+    int orig;
+    int stored;
+    bool operator<(const KnnKey &o) const { return d < o.d || (d == o.d && orig < o.orig); }
+};
+
+// Exact K nearest of stored point q by rings of Chebyshev cell distance.
+template <class R>
+void knn_query_grid(const KnnGrid<R> &kg, int q, int K, int *out, std::vector<KnnKey> &heap) {
+    const int dim = kg.dim;
+    const int n = int(kg.stored.size());
+    const R4<R> p = kg.stored[size_t(q)];
+    const int c = kg.cellOf[size_t(q)];
+    const int ci = c % dim, cj = (c / dim) % dim, ck = c / (dim * dim);
+    heap.clear();
+    auto worse = [](const KnnKey &a, const KnnKey &b) { return a < b; };  // max-heap on key
+    const double cs = 1000.0 / dim;
+    auto visit_cell = [&](int x, int y, int z) {
+        const int id = x + y * dim + z * dim * dim;
+        for (int s = kg.counters[size_t(id)]; s < kg.counters[size_t(id) + 1]; ++s) {
+            if (s == q) continue;
+            const double d = double(dist2(p, kg.stored[size_t(s)]));
+            const KnnKey key{d, kg.perm[size_t(s)], s};
+            if (int(heap.size()) < K) {
+                heap.push_back(key);
+                std::push_heap(heap.begin(), heap.end(), worse);
+            } else if (key < heap.front()) {
+                std::pop_heap(heap.begin(), heap.end(), worse);
+                heap.back() = key;
+                std::push_heap(heap.begin(), heap.end(), worse);
+            }
+        }
+    };
+    const int rmax = dim;  // Chebyshev distance never exceeds dim-1
+    for (int r = 0; r <= rmax; ++r) {
+        if (r >= 1 && int(heap.size()) == K) {
+            // every point of ring r is at least (r-1) cells away along one axis
+            const double lb = double(r - 1) * cs * (1.0 - 1e-4);
+            const double lb2 = lb * lb * (1.0 - 1e-6);
+            if (heap.front().d < lb2) break;
+        }
+        if (r > 0 && ci - r < 0 && ci + r >= dim && cj - r < 0 && cj + r >= dim && ck - r < 0 && ck + r >= dim)
+            break;  // the ring lies entirely outside the grid
+        for (int dz = -r; dz <= r; ++dz) {
+            const int z = ck + dz;
+            if (z < 0 || z >= dim) continue;
+            for (int dy = -r; dy <= r; ++dy) {
+                const int y = cj + dy;
+                if (y < 0 || y >= dim) continue;
+                const bool full = std::abs(dz) == r || std::abs(dy) == r;
+                if (full) {
+                    for (int dx = -r; dx <= r; ++dx) {
+                        const int x = ci + dx;
+                        if (x < 0 || x >= dim) continue;
+                        visit_cell(x, y, z);
+                    }
+                } else {
+                    if (ci - r >= 0) visit_cell(ci - r, y, z);
+                    if (r > 0 && ci + r < dim) visit_cell(ci + r, y, z);
+                }
+            }
+        }
+    }
+    std::sort(heap.begin(), heap.end());
+    for (int i = 0; i < K; ++i) out[i] = i < int(heap.size()) ? heap[size_t(i)].stored : -1;
+    (void)n;
+}
+
+template <class R>
+void knn_query_brute(const KnnGrid<R> &kg, int q, int K, int *out, std::vector<KnnKey> &all) {
+    const int n = int(kg.stored.size());
+    const R4<R> p = kg.stored[size_t(q)];
+    all.clear();
+    for (int s = 0; s < n; ++s) {
+        if (s == q) continue;
+        all.push_back(KnnKey{double(dist2(p, kg.stored[size_t(s)])), kg.perm[size_t(s)], s});
+    }
+    const int k = std::min(K, int(all.size()));
+    std::partial_sort(all.begin(), all.begin() + k, all.end());
+    for (int i = 0; i < K; ++i) out[i] = i < k ? all[size_t(i)].stored : -1;
+}
+
+template <class R>
+std::vector<R4<R>> convert_pts(const double *xyzw, int n) {
+    std::vector<R4<R>> p(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+        p[size_t(i)] = mk4<R>(R(xyzw[4 * i]), R(xyzw[4 * i + 1]), R(xyzw[4 * i + 2]), R(xyzw[4 * i + 3]));
+    return p;
+}
+
+void set_threads(int t) {
+#ifdef _OPENMP
+    if (t > 0) omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
+}
+
+template <class R>
+int knn_all(const KnnGrid<R> &kg, int K, int method, const int *ids, int nids, int *knn_rows) {
+    const long long nn = nids;
+#pragma omp parallel
+    {
+        std::vector<KnnKey> buf;
+#pragma omp for schedule(dynamic, 256)
+        for (long long t = 0; t < nn; ++t) {
+            const int q = ids ? ids[t] : int(t);
+            if (method == 1)
+                knn_query_brute(kg, q, K, knn_rows + t * K, buf);
+            else
+                knn_query_grid(kg, q, K, knn_rows + t * K, buf);
+        }
+    }
+    return kg.dim;
+}
+
+// One clip pass over a list of stored ids (VoroAlgoComputeData::launch,
+// voronoi.cpp:861-1036): kNN with K, then one cell per id.
+template <class R>
+void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, int V, int T, int vhs,
+               const GridView<R> *grid, int apex_mode, bool out_domain, std::vector<R4<R>> &out,
+               std::vector<int> &status, std::vector<int> &kused, Counters &ctr, double &t_knn, double &t_clip) {
+    const long long nids = (long long)ids.size();
+    if (nids == 0) return;
+    std::vector<int> nb(size_t(nids) * size_t(K));
+    auto t0 = std::chrono::steady_clock::now();
+    knn_all(kg, K, 0, ids.data(), int(nids), nb.data());
+    auto t1 = std::chrono::steady_clock::now();
+    Counters total;
+#pragma omp parallel
+    {
+        Counters local;
+#pragma omp for schedule(dynamic, 256)
+        for (long long t = 0; t < nids; ++t) {
+            const int s = ids[size_t(t)];
+            CellResult<R> r = compute_cell<R>(kg.stored.data(), s, nb.data() + t * K, K, P, V, T, vhs, grid,
+                                              apex_mode, out_domain, &out[size_t(s)], &local);
+            status[size_t(s)] = r.status;
+            kused[size_t(s)] = r.kused;
+        }
+#pragma omp critical
+        total.add(local);
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    ctr.add(total);
+    t_knn += std::chrono::duration<double>(t1 - t0).count();
+    t_clip += std::chrono::duration<double>(t2 - t1).count();
+}
+
+template <class R>
+int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *prm, double *out_mg,
+               int *out_status, int *out_kused, orc_stats *st) {
+    auto tstart = std::chrono::steady_clock::now();
+    set_threads(prm->threads);
+    const std::vector<R4<R>> pts = convert_pts<R>(xyzw, n);
+    const KnnGrid<R> kg = build_grid(pts);
+    GridView<R> gv = make_grid_view<R>(gr);
+    const GridView<R> *grid = gr ? &gv : nullptr;
+    const bool dbl = sizeof(R) == sizeof(double);
+    std::vector<R4<R>> out(size_t(n), mk4<R>(0, 0, 0, 0));
+    std::vector<int> status(size_t(n), kSuccess), kused(size_t(n), 0);
+    Counters ctr;
+    double t_knn = 0, t_clip = 0;
+    int K = prm->K, P = prm->P, V = prm->V, T = prm->T;
+    int vhs = dbl ? 2 : prm->vhs0;
+    int passes = 0;
+    std::vector<int> all(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) all[size_t(i)] = i;
+
+    if (prm->single_pass) {
+        clip_pass(kg, all, K, P, V, T, vhs, grid, prm->apex_mode, grid && gv.has_border_vertices, out, status, kused,
+                  ctr, t_knn, t_clip);
+        passes = 1;
+    } else {
+        // first pass: T = 0 defers every boundary cell (voronoi.cpp:1294-1297)
+        clip_pass(kg, all, K, P, V, 0, vhs, grid, prm->apex_mode, false, out, status, kused, ctr, t_knn, t_clip);
+        passes = 1;
+        auto collect = [&](const std::vector<int> &from, auto cond, std::vector<int> &ids0, std::vector<int> &ids1) {
+            ids0.clear();
+            ids1.clear();
+            for (int s : from) {
+                if (status[size_t(s)] == kSuccess) continue;
+                ids0.push_back(s);
+                if (cond(status[size_t(s)])) ids1.push_back(s);
+            }
+        };
+        std::vector<int> ids0, ids1;
+        collect(all, [](int s) { return s == kTriangleOverflow; }, ids0, ids1);
+        if (!ids1.empty()) {  // tr_overflow pass (voronoi.cpp:1304-1310)
+            clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, false, out, status, kused, ctr, t_knn, t_clip);
+            passes++;
+        }
+        if (!ids0.empty()) {
+            // extra passes (voronoi.cpp:1311-1376)
+            vhs = dbl ? 2 : prm->vhs1;
+            const bool out_domain = grid && gv.has_border_vertices;
+            std::vector<int> cur = ids0;
+            collect(cur, [](int) { return true; }, ids0, ids1);
+            for (int step = 0; step < prm->max_extra_steps && !ids0.empty(); ++step) {
+                if (!dbl && step > 0 && vhs <= 1) break;
+                int hist[kNumStatus] = {0};
+                for (int s : ids0) hist[status[size_t(s)]]++;
+                bool relaunch = false;
+                if (hist[kVertexOverflow]) { relaunch = true; V = int(V * 1.5); }
+                if (hist[kPlaneOverflow]) { relaunch = true; P = int(P * 1.5); }
+                if (hist[kSecurityRadiusNotReached]) { relaunch = true; K = int(K * 1.5); }
+                if (hist[kTriangleOverflow]) { relaunch = true; T = int(T * 1.5); }
+                if (prm->fix_exact_requeue && (hist[kNeedsExactPredicates] || hist[kFindAnotherBeginningVertex] ||
+                                               hist[kInconsistentBoundary]))
+                    relaunch = true;
+                if (!relaunch) break;
+                if (P >= 256) P = 255;  // uchar plane ids (voronoi.cpp:867-870)
+                K = std::min(K, n - 1 > 0 ? n - 1 : 1);
+                clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, out_domain, out, status, kused, ctr, t_knn,
+                          t_clip);
+                passes++;
+                cur = ids0;
+                collect(cur, [](int s) { return s != kEmptyCell; }, ids0, ids1);
+            }
+        }
+    }
+    for (int s = 0; s < n; ++s) {
+        const int o = kg.perm[size_t(s)];
+        const R4<R> v = out[size_t(s)];
+        out_mg[4 * size_t(o) + 0] = double(v.x);
+        out_mg[4 * size_t(o) + 1] = double(v.y);
+        out_mg[4 * size_t(o) + 2] = double(v.z);
+        out_mg[4 * size_t(o) + 3] = double(v.w);
+        if (out_status) out_status[o] = status[size_t(s)];
+        if (out_kused) out_kused[o] = kused[size_t(s)];
+    }
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->n_pred = ctr.n_pred;
+        st->n_radius_vertex = ctr.n_radius_vertex;
+        st->n_planes = ctr.n_planes;
+        st->n_bbox_vertex = ctr.n_bbox_vertex;
+        st->n_integrated_vertex = ctr.n_integrated_vertex;
+        st->n_clip = ctr.n_clip;
+        for (int s = 0; s < n; ++s) st->status_hist[status[size_t(s)]]++;
+        st->passes = passes;
+        st->final_K = K;
+        st->final_P = P;
+        st->final_V = V;
+        st->final_T = T;
+        st->seconds_knn = t_knn;
+        st->seconds_clip = t_clip;
+        st->seconds_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstart).count();
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_knn_dim(int n) { return knn_dim(n); }
+
+int orc_knn(const double *xyzw, int n, int K, int precision, int method, int *perm, int *knn, int *counters_out) {
+    if (n <= 0 || K <= 0 || !xyzw) return -1;
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        const auto pts = convert_pts<R>(xyzw, n);
+        const KnnGrid<R> kg = build_grid(pts);
+        if (perm) std::copy(kg.perm.begin(), kg.perm.end(), perm);
+        if (counters_out) std::copy(kg.counters.begin(), kg.counters.end(), counters_out);
+        if (knn) knn_all(kg, K, method, nullptr, n, knn);
+        return kg.dim;
+    };
+    return precision == 0 ? run(float(0)) : run(double(0));
+}
+
+int orc_solve(const double *xyzw, int n, const orc_grid *grid, const orc_params *prm, double *out_mg,
+              int *out_status, int *out_kused, orc_stats *st) {
+    if (n <= 1 || !xyzw || !prm || !out_mg) return -1;
+    if (prm->P >= 256 || prm->K <= 0 || prm->V <= 0) return -2;
+    orc_params p = *prm;
+    p.K = std::min(p.K, n - 1);
+    return p.precision == 0 ? solve_impl<float>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st)
+                            : solve_impl<double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st);
+}
+
+int orc_clip_cell(const double *xyzw, int n, int seed, const int *neigs, int K, int P, int V, int T, int precision,
+                  int vhs, const orc_grid *grid, int apex_mode, int out_domain, double *out4, int *nb_vertex,
+                  int *nb_plane) {
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        const auto pts = convert_pts<R>(xyzw, n);
+        GridView<R> gv = make_grid_view<R>(grid);
+        Counters ctr;
+        R4<R> out = mk4<R>(0, 0, 0, 0);
+        CellResult<R> r = compute_cell<R>(pts.data(), seed, neigs, K, P, V, T, sizeof(R) == 8 ? 2 : vhs,
+                                          grid ? &gv : nullptr, apex_mode, out_domain != 0, &out, &ctr);
+        out4[0] = double(out.x);
+        out4[1] = double(out.y);
+        out4[2] = double(out.z);
+        out4[3] = double(out.w);
+        if (nb_vertex) *nb_vertex = r.maxV;
+        if (nb_plane) *nb_plane = r.maxP;
+        return r.status;
+    };
+    return precision == 0 ? run(float(0)) : run(double(0));
+}
+
+// Grid.cpp:18-35 (signed tet volumes against the origin, minus sign included)
+double orc_grid_volume(const orc_grid *g) {
+    double vol = 0;
+    for (int t = 0; t < g->num_triangles; ++t) {
+        const double *a = g->points + 4 * g->triangles[3 * t];
+        const double *b = g->points + 4 * g->triangles[3 * t + 1];
+        const double *c = g->points + 4 * g->triangles[3 * t + 2];
+        vol += -det3<double>(a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]) / 6;
+    }
+    return vol;
+}
+
+void orc_grid_moments(const orc_grid *g, double out[4]) {
+    out[0] = out[1] = out[2] = out[3] = 0;
+    for (int t = 0; t < g->num_triangles; ++t) {
+        const double *a = g->points + 4 * g->triangles[3 * t];
+        const double *b = g->points + 4 * g->triangles[3 * t + 1];
+        const double *c = g->points + 4 * g->triangles[3 * t + 2];
+        const double v = -det3<double>(a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]) / 6;
+        const double v4 = v / 4;
+        for (int d = 0; d < 3; ++d) out[d] += v4 * (a[d] + b[d] + c[d]);
+        out[3] += v;
+    }
+}
+
+// Grid::valid Grid.cpp:60-184
+int orc_grid_valid(const orc_grid *g) {
+    const long long nv = (long long)g->size * g->size * g->size;
+    for (long long i = 0; i < nv; ++i)
+        if (g->in_domain[i] & 0xfc) return 0;
+    for (int i = 0; i < 3 * g->num_triangles; ++i)
+        if (g->triangles[i] < 0 || g->triangles[i] >= g->num_points) return 0;
+    for (int i = 0; i < g->num_tri_list; ++i)
+        if (g->tri_list[i] < 0 || g->tri_list[i] >= g->num_triangles) return 0;
+    for (long long i = 0; i <= nv; ++i)
+        if (g->offsets[i] < 0 || g->offsets[i] > g->num_tri_list) return 0;
+    if (orc_grid_volume(g) <= 0) return 0;
+    for (int t = 0; t < g->num_triangles; ++t) {
+        const double *a = g->points + 4 * g->triangles[3 * t];
+        const double *b = g->points + 4 * g->triangles[3 * t + 1];
+        const double *c = g->points + 4 * g->triangles[3 * t + 2];
+        const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const double cr[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const double area2 = (cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) / 4;
+        if (area2 < 0.01 * 0.01) return 0;
+    }
+    std::set<std::pair<int, int>> edges;
+    for (int t = 0; t < g->num_triangles; ++t)
+        for (int j = 0; j < 3; ++j) {
+            auto e = std::make_pair(g->triangles[3 * t + j], g->triangles[3 * t + (j + 1) % 3]);
+            if (!edges.insert(e).second) return 0;
+        }
+    for (const auto &e : edges)
+        if (!edges.count(std::make_pair(e.second, e.first))) return 0;
+    // "no cell has 8 border corners" (Grid.cpp:147-168).  As written, the
+    // reference counts the border flags of the corners of voxel 0 (the probe
+    // index x+y*g+z*g*g has no p in it) and then rejects every border corner
+    // p off the upper faces when that count is 8; restated as such.
+    const int s = g->size;
+    if (s >= 2) {
+        int found = 0;
+        for (int dz = 0; dz <= 1; ++dz)
+            for (int dy = 0; dy <= 1; ++dy)
+                for (int dx = 0; dx <= 1; ++dx)
+                    if (g->in_domain[dx + dy * s + (long long)dz * s * s] == 1) ++found;
+        if (found == 8)
+            for (long long p = 0; p < nv; ++p) {
+                if (g->in_domain[p] != 1) continue;
+                const int x = int(p % s), y = int((p / s) % s), z = int(p / s / s);
+                if (x == s - 1 || y == s - 1 || z == s - 1) continue;
+                return 0;
+            }
+    }
+    for (long long o = 0; o < nv; ++o) {
+        if (g->offsets[o] > g->offsets[o + 1]) return 0;
+        for (int t = g->offsets[o]; t + 1 < g->offsets[o + 1]; ++t)
+            if (g->tri_list[t] >= g->tri_list[t + 1]) return 0;
+    }
+    return 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+/*
+ * rpd_oracle.h -- C interface of the CPU oracle for the restricted power
+ * diagram hot path (grid kNN -> power-cell clipping -> restricted
+ * volume/moment integration).
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle restates, on the CPU, the algorithm
+ * of the reference files knearests.cpp/knearests.cl, ConvexCell.cl and the
+ * pass logic of voronoi.cpp.  It is the checker used by tests/, by the
+ * smoke() entry point and by the CPU-baseline leg of bench.py.  The product
+ * library (librpd) never links, loads or calls it.
+ */
+#ifndef RPD_ORACLE_H
+#define RPD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Domain grid, same content as the reference's Grid (Grid.h:14-70). */
+typedef struct {
+    int size;                    /* voxels per axis g                          */
+    const signed char *in_domain;/* g^3 corner flags 0 out / 1 border / 2 in   */
+    const double *points;        /* 4*num_points, xyzw with w = 0              */
+    int num_points;
+    const int *triangles;        /* 3*num_triangles point ids                  */
+    int num_triangles;
+    const int *tri_list;         /* per-voxel sorted triangle ids              */
+    int num_tri_list;
+    const int *offsets;          /* g^3+1 offsets into tri_list                */
+    const signed char *centre;   /* nullable: g^3 voxel-centre classes
+                                    0 out / 2 in / 1 unknown                    */
+} orc_grid;
+
+typedef struct {
+    int K, P, V, T;              /* caps of the first pass (voronoi.h:199-205) */
+    int precision;               /* 0: float build, 1: double (USE_DOUBLE)     */
+    int vhs0, vhs1;              /* float build: half-space algo first/extra   */
+    int apex_mode;               /* 0: reference corner rule (ConvexCell.cl:950-
+                                    1056), 1: centre of the first voxel of the
+                                    bbox range whose centre class is known,
+                                    then the reference rule                    */
+    int fix_exact_requeue;       /* 1: relaunch cells flagged only with
+                                    needs_exact_predicates (reference drops them) */
+    int max_extra_steps;         /* extra passes (reference: 5)                */
+    int threads;                 /* OpenMP threads, <=0: all                   */
+    int single_pass;             /* 1: no re-queue, one pass with T = T        */
+} orc_params;
+
+typedef struct {
+    long long n_pred;            /* plane-side predicate evaluations           */
+    long long n_radius_vertex;   /* vertices visited by the security radius    */
+    long long n_planes;          /* planes built (bisectors + cone planes)     */
+    long long n_bbox_vertex;     /* vertices visited for the bbox              */
+    long long n_integrated_vertex;/* vertices integrated                       */
+    long long n_clip;            /* clip-by-plane calls                        */
+    int status_hist[11];         /* final status histogram                     */
+    int passes;                  /* number of clip passes run                  */
+    int final_K, final_P, final_V, final_T; /* caps of the last pass           */
+    double seconds_knn, seconds_clip, seconds_total;
+} orc_stats;
+
+/* Grid kNN.  perm[stored] = original index; knn[stored*K + i] = stored index
+   of the i-th neighbour, ascending by (d^2, original index), -1 padded.
+   method 0: exact grid ring search, 1: brute force.  Returns dim (>0) or <0. */
+int orc_knn(const double *xyzw, int n, int K, int precision, int method,
+            int *perm, int *knn, int *counters_out /* nullable, dim^3+1 */);
+
+/* kNN grid resolution used for n seeds. */
+int orc_knn_dim(int n);
+
+/* Full restricted power diagram with the reference pass policy.
+   out_mg: 4*n (m*x, m*y, m*z, m) per ORIGINAL seed index.
+   out_status: final status per original seed (nullable).
+   out_kused: neighbours consumed per original seed (nullable). */
+int orc_solve(const double *xyzw, int n, const orc_grid *grid,
+              const orc_params *prm, double *out_mg, int *out_status,
+              int *out_kused, orc_stats *st);
+
+/* One cell, one pass, caller-given neighbour list (original indices of
+   xyzw).  For unit tests of the clip.  Returns the status. */
+int orc_clip_cell(const double *xyzw, int n, int seed, const int *neigs, int K,
+                  int P, int V, int T, int precision, int vhs,
+                  const orc_grid *grid, int apex_mode, int out_domain,
+                  double *out4, int *nb_vertex, int *nb_plane);
+
+/* Grid helpers restating Grid.cpp. */
+double orc_grid_volume(const orc_grid *grid);              /* Grid.cpp:30-35 */
+int orc_grid_valid(const orc_grid *grid);                  /* Grid.cpp:60-184 */
+void orc_grid_moments(const orc_grid *grid, double out4[4]); /* Grid.cpp:37-53 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,61 @@
+"""Restricted power diagram on the GPU vs the FP64 oracle.
+
+Bars: FP64 form within 1e-12, mixed form within 1e-6, of the mean cell volume
+1e9/n (volume) and of mean cell volume x 1000 (first moments).
+"""
+import numpy as np
+import pytest
+
+from conftest import moment_errors, oracle_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(rpd, orc, ctx, x, grid, prec, K, P, V, T=100):
+    og = oracle_grid(orc, grid) if grid is not None else None
+    ref, ref_st, _, _ = orc.solve(x, og, K=K, P=P, V=V, T=T)
+    dom = ctx.domain(grid) if grid is not None else None
+    ctx.set_seeds(x)
+    st = ctx.solve(dom, K=K, P=P, V=V, T=T, precision=prec)
+    got = ctx.moments()
+    status = ctx.status()
+    return ref, ref_st, got, status, st
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-6)])
+def test_unrestricted_voronoi(rpd, orc, ctx, prec, tol):
+    n = 4000
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=11)
+    ref, ref_st, got, status, st = _run(rpd, orc, ctx, x, None, prec, 40, 40, 64)
+    ev, em = moment_errors(got, ref, n)
+    print("unrestricted", prec, st["status_hist"], ev, em, flush=True)
+    assert (status == 4).all()
+    assert ev <= tol and em <= tol
+    assert abs(got[:, 3].sum() - 1e9) / 1e9 < (1e-12 if prec == 0 else 1e-7)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-6)])
+def test_sphere_domain(rpd, orc, ctx, prec, tol):
+    n = 10000
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=0)
+    grid = rpd.Grid.shape(rpd.SHAPE_SPHERE, 32, (0.5, 0.5, 0.5, 0.45))
+    ref, ref_st, got, status, st = _run(rpd, orc, ctx, x, grid, prec, 50, 30, 40)
+    ev, em = moment_errors(got, ref, n)
+    print("sphere", prec, st, ev, em, flush=True)
+    assert (status == 4).all()
+    assert ev <= tol and em <= tol
+    vol = grid.volume()
+    assert abs(got[:, 3].sum() - vol) / vol < (1e-12 if prec == 0 else 1e-7)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-6)])
+def test_power_torus(rpd, orc, ctx, prec, tol):
+    n = 20000
+    h = (1.0 / n) ** (1 / 3)
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=2, wmax=0.25 * h * h * 1e6)
+    grid = rpd.Grid.shape(rpd.SHAPE_TORUS, 40, (0.5, 0.5, 0.5, 0.3, 0.12))
+    ref, ref_st, got, status, st = _run(rpd, orc, ctx, x, grid, prec, 64, 40, 60)
+    ev, em = moment_errors(got, ref, n)
+    print("torus power", prec, st, ev, em, flush=True)
+    assert np.array_equal(np.bincount(status, minlength=11), np.bincount(ref_st, minlength=11))
+    assert ev <= tol and em <= tol
