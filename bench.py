@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the restricted power diagram hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 1,000,000 seeds
+i.i.d. uniform in [0,1]^3 with power weights w ~ U[0, 0.25 h^2], h = N^-1/3,
+in the [0,1000]^3 frame; domain = solid torus R=0.3 r=0.12 (axis z, centred)
+voxelized at 128^3; caps K=64 P=40 V=60 T=100.  One step = one full solve
+(binning + kNN + clip/restrict/integrate + every re-queue pass) from seeds
+already resident in HBM, for the mixed form and for the all-FP64 form.
+
+Prints ONE JSON line.  `value` = mixed-form cells/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cells/sec @1M seeds mixed vs FP64 at ≤1e-6 rel err; p50/p99 req latency"
+PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (published)
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flops_per_cell(st, n_cells, mixed):
+    """Algorithmic flops per cell from the oracle's event counters:
+    F = c_pred*n_pred + 50*n_radius_vertex + 25*n_planes + 41*n_bbox_vertex
+        + 420*n_integrated_vertex   (op counts of ConvexCell.cl:79-97,
+    416-448, 344-373, 314-328, 930-935, 577-619).  Returns (fp32, fp64)."""
+    if mixed:
+        f32 = 109.0 * st["n_pred"] + 50.0 * st["n_radius_vertex"]
+        f64 = 25.0 * st["n_planes"] + 41.0 * st["n_bbox_vertex"] + 420.0 * st["n_integrated_vertex"]
+    else:
+        f32 = 0.0
+        f64 = (45.0 * st["n_pred"] + 50.0 * st["n_radius_vertex"] + 25.0 * st["n_planes"] +
+               41.0 * st["n_bbox_vertex"] + 420.0 * st["n_integrated_vertex"])
+    return f32 / n_cells, f64 / n_cells
+
+
+def rel_errors(got, ref, n):
+    v = 1e9 / n
+    ev = float(np.abs(got[:, 3] - ref[:, 3]).max() / v)
+    em = float(np.abs(got[:, :3] - ref[:, :3]).max() / (v * 1000.0))
+    big = ref[:, 3] > 0.01 * v
+    if big.any():
+        rv = float((np.abs(got[big, 3] - ref[big, 3]) / ref[big, 3]).max())
+        b_got = got[big, :3] / got[big, 3:4]
+        b_ref = ref[big, :3] / ref[big, 3:4]
+        rb = float((np.abs(b_got - b_ref).max(1) / np.cbrt(ref[big, 3])).max())
+    else:
+        rv = rb = 0.0
+    return {"vol_over_mean_cell": ev, "moment_over_mean_cell_x1000": em, "vol_rel_cells_gt_1pct": rv,
+            "bary_over_cell_size": rb}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--cpu-sample", type=int, default=100_000, help="cells timed on the CPU oracle")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the hot path is a single-device kernel; "
+              "multi-GPU sharding is described in DESIGN.md and not run). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as ge
+
+    rpd = ge.load_package()
+    n = args.n
+    h = (1.0 / n) ** (1.0 / 3.0)
+    wmax = 0.25 * h * h * 1e6  # frame units: lengths x1e3, weights x1e6
+    K, P, V, T = 64, 40, 60, 100
+    log("bench: generating %d seeds and the %d^3 torus" % (n, args.grid))
+    seeds = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=0, wmax=wmax)
+    grid = rpd.Grid.shape(rpd.SHAPE_TORUS, args.grid, (0.5, 0.5, 0.5, 0.3, 0.12))
+
+    ctx = rpd.Context(0)
+    dom = ctx.domain(grid)
+    dseeds = torch.from_numpy(seeds).to("cuda:0")
+    torch.cuda.synchronize()
+
+    def step(prec):
+        ctx.set_seeds_device(dseeds.data_ptr(), n)
+        return ctx.solve(dom, K=K, P=P, V=V, T=T, precision=prec)
+
+    res = {}
+    for name, prec in (("mixed", rpd.PREC_MIXED), ("fp64", rpd.PREC_FP64)):
+        for _ in range(args.warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = [step(prec) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) / args.steps * 1e3
+        res[name] = {"ms": ms, "stats": stats, "out": ctx.moments(), "status": ctx.status()}
+        s = stats[-1]
+        log("bench: %s %.2f ms/solve  pass1 knn %.2f ms clip %.2f ms  passes %d fallback %d hist %s" %
+            (name, ms, s["ms_knn_first"], s["ms_clip_first"], s["passes"], s["fallback_cells"],
+             s["status_hist"]))
+
+    err_mixed_vs_fp64 = rel_errors(res["mixed"]["out"], res["fp64"]["out"], n)
+    log("bench: mixed vs fp64", err_mixed_vs_fp64)
+
+    # ---- CPU oracle on a bounded sample of the same workload ----------------
+    cpu = None
+    flop = None
+    err_vs_oracle = None
+    if not args.no_cpu and args.cpu_sample > 0:
+        orc = ge.load_oracle()
+        og = orc.Grid(grid.size, grid.in_domain, grid.points, grid.triangles, grid.tri_list, grid.offsets,
+                      grid.centre)
+        stride = max(1, n // args.cpu_sample)
+        ids = np.arange(0, n, stride, dtype=np.int32)[: args.cpu_sample]
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        t0 = time.perf_counter()
+        ref, ref_st, _, ost = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads)
+        t1 = time.perf_counter()
+        cpu = {"value": float(ids.size / (t1 - t0)), "unit": "cells/s", "cores": int(threads), "kind": "port",
+               "sample": "cells of every %d-th seed (%d cells) of the 1M-seed torus workload, all 1M seeds as "
+                         "neighbours; FP64 (USE_DOUBLE) oracle with the reference pass policy; %.1f s" %
+                         (stride, ids.size, t1 - t0)}
+        log("bench: cpu oracle", cpu)
+        _, _, _, ost1 = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads, single_pass=1)
+        flop = {"mixed": flops_per_cell(ost1, ids.size, True), "fp64": flops_per_cell(ost1, ids.size, False)}
+        err_vs_oracle = {k: rel_errors(res[k]["out"][ids], ref, n) for k in ("mixed", "fp64")}
+        log("bench: vs oracle", err_vs_oracle)
+
+    # ---- roofline of the dominant kernel: pass-1 clip launch (mixed) ---------
+    sm = res["mixed"]["stats"]
+    t_clip = float(np.mean([s["ms_clip_first"] for s in sm])) * 1e-3
+    roofline = None
+    if flop is not None:
+        f32, f64 = flop["mixed"]
+        achieved = (f32 + f64) * n / t_clip / 1e12
+        peak = (f32 + f64) / (f32 / PEAK_FP32_TFLOPS + f64 / PEAK_FP64_TFLOPS)
+        roofline = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                    "frac": achieved / peak, "traffic": None,
+                    "kernel": "clip_kernel<float,double,true,1> (mixed form, pass 1)",
+                    "flops_per_cell": {"fp32": f32, "fp64": f64}, "launch_ms": t_clip * 1e3}
+    t_knn = float(np.mean([s["ms_knn_first"] for s in sm])) * 1e-3
+    knn_bytes = (60 + 4 * K) * n
+    roofline_knn = {"bound": "hbm", "achieved": knn_bytes / t_knn / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": knn_bytes / t_knn / 1e9 / PEAK_HBM_GBS, "bytes_per_seed": 60 + 4 * K,
+                    "launch_ms": t_knn * 1e3, "kernel": "knn_kernel<float,1>"}
+
+    ms_mixed = res["mixed"]["ms"]
+    ms_fp64 = res["fp64"]["ms"]
+    line = {
+        "metric": METRIC,
+        "value": n / (ms_mixed * 1e-3),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_mixed,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (SplitMix64 seeds, analytic voxel torus)",
+        "config": {"workload": "1M uniform power-weighted seeds (w~U[0,0.25h^2]), torus R0.3 r0.12 @128^3, "
+                               "K64 P40 V60 T100, mixed form; FP64 form timed beside it",
+                   "n": n, "grid": args.grid, "K": K, "P": P, "V": V, "T": T},
+        "fp64_cells_per_s": n / (ms_fp64 * 1e-3),
+        "fp64_ms_per_step": ms_fp64,
+        "mixed_over_fp64": ms_fp64 / ms_mixed,
+        "err_mixed_vs_fp64": err_mixed_vs_fp64,
+        "err_vs_oracle_sample": err_vs_oracle,
+        "fallback_fraction": res["mixed"]["stats"][-1]["fallback_cells"] / n,
+        "status_hist_mixed": res["mixed"]["stats"][-1]["status_hist"],
+        "roofline": roofline,
+        "roofline_knn": roofline_knn,
+        "cpu_baseline": cpu,
+        "serving": None,
+    }
+    dom.close()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -61,7 +61,8 @@ class GridDesc(ctypes.Structure):
 
 class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in
-                ("K", "P", "V", "T", "precision", "max_extra_steps", "apex_mode", "reserved")]
+                ("K", "P", "V", "T", "precision", "max_extra_steps", "apex_mode", "security_mode", "cull")] + \
+               [("reserved", ctypes.c_int32 * 3)]
 
 
 class Stats(ctypes.Structure):
@@ -71,6 +72,8 @@ class Stats(ctypes.Structure):
         ("ms_clip", ctypes.c_double),
         ("ms_fallback", ctypes.c_double),
         ("ms_total", ctypes.c_double),
+        ("ms_knn_first", ctypes.c_double),
+        ("ms_clip_first", ctypes.c_double),
         ("status_hist", ctypes.c_int32 * 11),
         ("passes", ctypes.c_int32),
         ("fallback_cells", ctypes.c_int32),
@@ -294,8 +297,10 @@ class Context:
         self.check(lib().rpd_set_seeds_device(self._h, ctypes.c_void_p(ptr), int(n)), "rpd_set_seeds_device")
         self.n = int(n)
 
-    def solve(self, domain=None, K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, apex_mode=1):
-        p = Params(int(K), int(P), int(V), int(T), int(precision), int(max_extra_steps), int(apex_mode), 0)
+    def solve(self, domain=None, K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, apex_mode=1,
+              security_mode=0, cull=1):
+        p = Params(int(K), int(P), int(V), int(T), int(precision), int(max_extra_steps), int(apex_mode),
+                   int(security_mode), int(cull))
         st = Stats()
         self.check(lib().rpd_solve(self._h, domain._h if domain is not None else None, ctypes.byref(p),
                                    ctypes.byref(st)), "rpd_solve")
@@ -376,6 +381,7 @@ class VoroAlgo:
         self.precision = PREC_MIXED
         self.max_extra_steps = 5
         self.apex_mode = 1
+        self.security_mode = 0
         self.m_debug = debug
         self._bary = None
         self._perm = None
@@ -393,7 +399,7 @@ class VoroAlgo:
 
     def launch(self, saveData=None, stats=None):
         st = self._ctx.solve(self._dom, self.m_K, self.m_P, self.m_V, self.m_T, self.precision,
-                             self.max_extra_steps, self.apex_mode)
+                             self.max_extra_steps, self.apex_mode, self.security_mode)
         if stats is not None:
             stats.append(st)
         if self.m_keepDistance < 0 or self._dom is None:

@@ -5,12 +5,13 @@
 //   pass 1: every cell, caps K/P/V/T of rpd_params, in the requested form;
 //   then up to max_extra_steps re-queue passes over the failed cells only:
 //     * vertex/plane/K/triangle overflow -> the matching cap x1.5 (as the
-//       reference), same form;
+//       reference);
 //     * needs_exact_predicates, inconsistent_boundary and
-//       find_another_beginning_vertex from an FP32 form -> FP64 form with the
-//       same caps (the reference drops cells flagged only for exact
-//       predicates, voronoi.cpp:1349-1356; here they are always recomputed);
-//     * empty_cell is final (volume 0).
+//       find_another_beginning_vertex from an FP32 form -> recomputed (the
+//       reference drops cells flagged only for exact predicates,
+//       voronoi.cpp:1349-1356; here they are always recomputed);
+//     * empty_cell is final (volume 0);
+//   every re-queue pass runs in the FP64 form over all re-queued cells.
 // Boundary cells are processed in the same pass as interior cells (the
 // reference defers them with a T = 0 first pass and recomputes them).
 
@@ -29,7 +30,7 @@
 struct rpd_domain {
     rpd_ctx *ctx = nullptr;
     DomainDev dev;
-    DevBuf flags, centre, pts, tris, tlist, toffs;
+    DevBuf flags, centre, pts, tris, tlist, toffs, dist;
     int64_t bytes = 0;
 };
 
@@ -46,7 +47,7 @@ struct rpd_ctx {
     // outputs, original order
     DevBuf out, status, kused;
     // pass work buffers
-    DevBuf knn, ids_a, ids_b, ids_c, cnt;
+    DevBuf knn, ids_a, ids_b, ids_c, cnt, wred;
     bool solved = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
@@ -78,30 +79,107 @@ int set_err(rpd_ctx *c, int code, const char *fmt, ...) {
     } while (0)
 
 // failed cells of a list: compacts stored ids whose status (by original
-// index) is not success; counts statuses.
-__global__ void collect_failed_kernel(const int *__restrict__ ids, int64_t n_ids, const int *__restrict__ perm,
-                                      const int *__restrict__ status, int *__restrict__ exact_ids,
-                                      int *__restrict__ over_ids, int *__restrict__ cnt) {
+// index) is not success into two lists, with one atomic per wave and list,
+// and counts statuses in LDS with one atomic per block and status.
+__global__ __launch_bounds__(256) void collect_failed_kernel(const int *__restrict__ ids, int64_t n_ids,
+                                                             const int *__restrict__ perm,
+                                                             const int *__restrict__ status,
+                                                             int *__restrict__ exact_ids, int *__restrict__ over_ids,
+                                                             int *__restrict__ cnt) {
+    __shared__ int hist[RPD_STATUS_NUM];
+    if (threadIdx.x < RPD_STATUS_NUM) hist[threadIdx.x] = 0;
+    __syncthreads();
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (i >= n_ids) return;
-    const int sid = ids ? ids[i] : int(i);
-    const int st = status[perm[sid]];
-    if (st == RPD_SUCCESS) return;
-    atomicAdd(&cnt[2 + st], 1);
-    if (st == RPD_EMPTY_CELL) return;
-    if (st == RPD_NEEDS_EXACT_PREDICATES || st == RPD_INCONSISTENT_BOUNDARY ||
-        st == RPD_FIND_ANOTHER_BEGINNING_VERTEX) {
-        exact_ids[atomicAdd(&cnt[0], 1)] = sid;
-    } else {
-        over_ids[atomicAdd(&cnt[1], 1)] = sid;
+    int sid = -1, st = RPD_SUCCESS;
+    if (i < n_ids) {
+        sid = ids ? ids[i] : int(i);
+        st = status[perm[sid]];
+        if (st != RPD_SUCCESS) atomicAdd(&hist[st], 1);
+    }
+    const bool ex = st == RPD_NEEDS_EXACT_PREDICATES || st == RPD_INCONSISTENT_BOUNDARY ||
+                    st == RPD_FIND_ANOTHER_BEGINNING_VERTEX;
+    const bool ov = st != RPD_SUCCESS && st != RPD_EMPTY_CELL && !ex;
+    const uint64_t me = __ballot(ex), mo = __ballot(ov);
+    const int lane = threadIdx.x & 63;
+    auto rank = [&](uint64_t m) {
+        return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+    };
+    int be = 0, bo = 0;
+    if (lane == 0) {
+        if (me) be = atomicAdd(&cnt[0], __popcll(me));
+        if (mo) bo = atomicAdd(&cnt[1], __popcll(mo));
+    }
+    be = __shfl(be, 0);
+    bo = __shfl(bo, 0);
+    if (ex) exact_ids[be + rank(me)] = sid;
+    if (ov) over_ids[bo + rank(mo)] = sid;
+    __syncthreads();
+    if (threadIdx.x < RPD_STATUS_NUM && hist[threadIdx.x]) atomicAdd(&cnt[2 + threadIdx.x], hist[threadIdx.x]);
+}
+
+// per-block minimum of the weights of non-border seeds
+__global__ __launch_bounds__(256) void wmin_kernel(const double4 *__restrict__ seeds, int64_t n,
+                                                   double *__restrict__ out) {
+    __shared__ double red[256];
+    double m = INFINITY;
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const double w = seeds[i].w;
+        if (w > RPD_NO_CELL_LIMITS) m = fmin(m, w);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Chebyshev (26-neighbour BFS) voxel distance to the voxels that hold
+// triangles or whose centre is not known to be outside, capped at 255.
+void domain_distance(const rpd_grid_desc *g, std::vector<uint8_t> &dist) {
+    const int G = g->size;
+    const size_t nv = size_t(G) * G * G;
+    dist.assign(nv, 255);
+    std::vector<int> q;
+    q.reserve(nv / 8 + 16);
+    for (size_t v = 0; v < nv; ++v) {
+        const bool tri = g->offsets[v + 1] > g->offsets[v];
+        const bool in = g->centre ? g->centre[v] != 0 : g->in_domain[v] != 0;
+        if (tri || in) {
+            dist[v] = 0;
+            q.push_back(int(v));
+        }
+    }
+    for (size_t h = 0; h < q.size(); ++h) {
+        const int v = q[h];
+        const int d = dist[size_t(v)];
+        if (d >= 254) continue;
+        const int x = v % G, y = (v / G) % G, z = v / (G * G);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int X = x + dx, Y = y + dy, Z = z + dz;
+                    if (X < 0 || Y < 0 || Z < 0 || X >= G || Y >= G || Z >= G) continue;
+                    const size_t u = size_t(X) + size_t(Y) * G + size_t(Z) * G * G;
+                    if (dist[u] != 255) continue;
+                    dist[u] = uint8_t(d + 1);
+                    q.push_back(int(u));
+                }
     }
 }
 
-__global__ void status_hist_kernel(const int *__restrict__ status, int64_t n, int *__restrict__ hist) {
-    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (i >= n) return;
-    const int st = status[i];
-    if (st >= 0 && st < RPD_STATUS_NUM) atomicAdd(&hist[st], 1);
+__global__ __launch_bounds__(256) void status_hist_kernel(const int *__restrict__ status, int64_t n,
+                                                          int *__restrict__ out) {
+    __shared__ int hist[RPD_STATUS_NUM];
+    if (threadIdx.x < RPD_STATUS_NUM) hist[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const int st = status[i];
+        if (st >= 0 && st < RPD_STATUS_NUM) atomicAdd(&hist[st], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < RPD_STATUS_NUM && hist[threadIdx.x]) atomicAdd(&out[threadIdx.x], hist[threadIdx.x]);
 }
 
 int ensure_binning(rpd_ctx *c, int prec) {
@@ -177,7 +255,9 @@ void rpd_default_params(rpd_params *p) {
     p->precision = RPD_PREC_MIXED;
     p->max_extra_steps = 5;
     p->apex_mode = 1;
-    p->reserved = 0;
+    p->security_mode = 0;
+    p->cull = 1;
+    p->reserved[0] = p->reserved[1] = p->reserved[2] = 0;
 }
 
 int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
@@ -203,6 +283,9 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     if (r == RPD_OK) r = up(d->tris, g->triangles, size_t(g->num_triangles) * 12);
     if (r == RPD_OK) r = up(d->tlist, g->tri_list, size_t(g->num_tri_list) * 4);
     if (r == RPD_OK) r = up(d->toffs, g->offsets, (nv + 1) * 4);
+    std::vector<uint8_t> dist;
+    domain_distance(g, dist);
+    if (r == RPD_OK) r = up(d->dist, dist.data(), nv);
     if (r == RPD_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = RPD_E_DEVICE;
     if (r != RPD_OK) {
         delete d;
@@ -216,6 +299,7 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     d->dev.tris = d->tris.as<int>();
     d->dev.tlist = d->tlist.as<int>();
     d->dev.toffs = d->toffs.as<int>();
+    d->dev.dist = d->dist.as<uint8_t>();
     for (size_t i = 0; i < nv; ++i)
         if (g->in_domain[i] == 1) {
             d->dev.has_border_vertices = true;
@@ -298,6 +382,18 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     RPDCHK(c, c->cnt.ensure(64), "alloc cnt");
 
     const DomainDev *dd = dom ? &dom->dev : nullptr;
+    double wmin = 0;
+    if (prm->security_mode == 0) {
+        const int nbk = 256;
+        RPDCHK(c, c->wred.ensure(nbk * sizeof(double)), "alloc wred");
+        wmin_kernel<<<nbk, 256, 0, c->stream>>>(c->seeds.as<double4>(), n, c->wred.as<double>());
+        double red[nbk];
+        HIPCHK(c, hipMemcpyAsync(red, c->wred.ptr, sizeof(red), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        wmin = INFINITY;
+        for (double v : red) wmin = std::min(wmin, v);
+        if (!(wmin < INFINITY)) wmin = 0;
+    }
     auto run_pass = [&](int form, const int *ids, int64_t n_ids, int K_, int P_, int V_, int T_, double &ms_knn,
                         double &ms_clip) -> int {
         if (n_ids <= 0) return RPD_OK;
@@ -319,6 +415,9 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.V = V_;
         L.T = T_;
         L.apex_mode = prm->apex_mode;
+        L.security_mode = prm->security_mode;
+        L.wmin = wmin;
+        L.cull = prm->cull != 0;
         L.dom = dd;
         L.out = c->out.as<double4>();
         L.status = c->status.as<int>();
@@ -335,15 +434,16 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     double ms_knn = 0, ms_clip = 0, ms_fb_knn = 0, ms_fb = 0;
     RPDCHK(c, run_pass(form0, nullptr, n, K, P, V, T, ms_knn, ms_clip), "pass 1");
     st.passes = 1;
+    st.ms_knn_first = ms_knn;
+    st.ms_clip_first = ms_clip;
 
-    // re-queue passes
+    // re-queue passes: the failed cells of the last pass, in one FP64 launch
+    // per step, with the caps grown x1.5 for each overflow kind seen
     const int *cur_ids = nullptr;
     int64_t cur_n = n;
     int cur_form = form0;
-    DevBuf *bufs[3] = {&c->ids_a, &c->ids_b, &c->ids_c};
     for (int step = 0; step < std::max(0, prm->max_extra_steps); ++step) {
-        // split failures of the last pass into (exact -> ids_b) and (overflow -> ids_c)
-        int *exact_ids = bufs[1]->as<int>(), *over_ids = bufs[2]->as<int>();
+        int *exact_ids = c->ids_b.as<int>(), *over_ids = c->ids_c.as<int>();
         HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));
         const unsigned nb = unsigned((cur_n + 255) / 256);
         collect_failed_kernel<<<nb, 256, 0, c->stream>>>(cur_ids, cur_n, c->perm.as<int>(), c->status.as<int>(),
@@ -359,49 +459,36 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) K2 = int(std::min<int64_t>(n - 1, std::min(256, int(K * 1.5))));
         if (h[RPD_TRIANGLE_OVERFLOW]) T2 = int(T * 1.5) + 1;
         const bool caps_changed = K2 != K || P2 != P || V2 != V || T2 != T;
-        const bool exact_rerun = n_exact > 0 && cur_form != RPD_PREC_FP64;
-        const bool over_rerun = n_over > 0 && caps_changed;
-        if (!exact_rerun && !over_rerun) break;
+        // cells an FP32 form could not certify always go to FP64; cells that
+        // failed in FP64 are retried only when a cap grew
+        const bool take_exact = n_exact > 0 && (cur_form != RPD_PREC_FP64 || caps_changed);
+        const bool take_over = n_over > 0 && caps_changed;
+        if (!take_exact && !take_over) break;
         K = K2;
         P = P2;
         V = V2;
         T = T2;
-        // next failures are collected over the union of both lists: run the
-        // overflow cells first into ids_c then the exact cells into ids_b, and
-        // gather both lists into ids_a for the next collection.
+        int *dst = c->ids_a.as<int>();
         int64_t next_n = 0;
-        if (over_rerun) {
-            RPDCHK(c, run_pass(cur_form, over_ids, n_over, K, P, V, T, ms_knn, ms_clip), "overflow pass");
-            st.passes++;
-        }
-        if (n_exact > 0) {
-            const int f = RPD_PREC_FP64;
-            if (exact_rerun || caps_changed) {
-                RPDCHK(c, run_pass(f, exact_ids, n_exact, K, P, V, T, ms_fb_knn, ms_fb), "fp64 pass");
-                st.passes++;
-                st.fallback_cells += n_exact;
-            }
-        }
-        // ids_a <- exact ++ over (only those that were re-run)
-        int *dst = bufs[0]->as<int>();
-        if (n_exact > 0 && (exact_rerun || caps_changed)) {
+        if (take_exact) {
             HIPCHK(c, hipMemcpyAsync(dst, exact_ids, size_t(n_exact) * 4, hipMemcpyDeviceToDevice, c->stream));
             next_n += n_exact;
+            if (cur_form != RPD_PREC_FP64) st.fallback_cells += n_exact;
         }
-        if (over_rerun) {
+        if (take_over) {
             HIPCHK(c, hipMemcpyAsync(dst + next_n, over_ids, size_t(n_over) * 4, hipMemcpyDeviceToDevice, c->stream));
             next_n += n_over;
         }
+        RPDCHK(c, run_pass(RPD_PREC_FP64, dst, next_n, K, P, V, T, ms_fb_knn, ms_fb), "re-queue pass");
+        st.passes++;
         cur_ids = dst;
         cur_n = next_n;
-        // after an exact rerun the list is in FP64 for its exact part; the
-        // overflow part stays in the original form
-        cur_form = (n_over > 0 && over_rerun) ? form0 : RPD_PREC_FP64;
-        if (cur_n == 0) break;
+        cur_form = RPD_PREC_FP64;
     }
     // final status histogram
     HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));
-    status_hist_kernel<<<unsigned((n + 255) / 256), 256, 0, c->stream>>>(c->status.as<int>(), n, c->cnt.as<int>());
+    status_hist_kernel<<<unsigned(std::min<int64_t>(1024, (n + 255) / 256)), 256, 0, c->stream>>>(
+        c->status.as<int>(), n, c->cnt.as<int>());
     HIPCHK(c, hipMemcpyAsync(st.status_hist, c->cnt.ptr, sizeof(st.status_hist), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     st.ms_knn = ms_knn + ms_fb_knn;

@@ -66,6 +66,9 @@ struct ClipArgs {
     int *kused;
     int lds_stride;
     int tl_cap;
+    int security_mode;    // 0 exact power stop, 1 reference stop
+    double wmin;          // minimum weight over non-border seeds
+    const uint8_t *dist;  // Chebyshev voxel distance to the nearest domain voxel (nullable)
 };
 
 __device__ __forceinline__ uint32_t pack3(uint32_t a, uint32_t b, uint32_t c) { return a | (b << 8) | (c << 16); }
@@ -189,6 +192,20 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
 
     for (int i = lane; i < 256; i += 64) nxt[i] = 0xff;
 
+    // Domain cull: a seed whose voxel is d >= 2 voxels (Chebyshev) from every
+    // voxel that holds triangles or lies inside has no point of the domain
+    // within (d - 1) * vs; once the cell's circumradius about the seed is
+    // below that, its restriction to the domain is empty (volume 0, as the
+    // reference's apex-outside / no-triangle case, ConvexCell.cl:1067-1079).
+    double cull_r2 = 0;
+    if (a.g > 0 && a.dist) {
+        const int g = a.g;
+        auto vox = [&](double c) { return int(fmin(fmax(c / a.vs, 0.0), double(g - 1))); };
+        const int d = a.dist[vox(S0.x) + vox(S0.y) * g + vox(S0.z) * g * g];
+        if (d >= 2) cull_r2 = double(d - 1) * a.vs * double(d - 1) * a.vs;
+    }
+    const double wgap = S0.w - a.wmin;  // >= 0 except for the lightest seeds
+
     // ---- CCInit: bounding cube in the local frame ---------------------------
     const double eps = a.g > 0 ? 0.1 : 0.0;  // CUBE_EPSILON, voronoi.cpp:929
     const double vmin = -eps, vmax = 1000.0 + eps;
@@ -246,8 +263,9 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
     R maxr2 = max_r2();
 
     // ---- one clip by plane e; returns kNoClip, kClipped or a status --------
-    auto clip = [&](const Pl<RP> &eP) -> int {
-        const Pl<R> e = cvt<R>(eP);
+    // e: the plane in the test precision (wave-uniform); the lane wl holds
+    // the same plane in the storage precision in eP (wl < 0: every lane does)
+    auto clip = [&](const Pl<R> &e, int wl, const Pl<RP> &eP) -> int {
         uint64_t rem[S];
         int nr = 0, nvalid = 0;
         uint64_t flag = 0;
@@ -268,7 +286,7 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
         if (nr == 0) return kNoClip;
         if (nr == nvalid) return RPD_EMPTY_CELL;
         const int pid = nb_plane;
-        if (lane == 0) planes[pid] = eP;
+        if (lane == (wl < 0 ? 0 : wl)) planes[pid] = eP;
         // removed triangles, in slot-major order
         int carry = 0;
 #pragma unroll
@@ -482,19 +500,40 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
     auto fail = [&](int st, int ku) { finish(st, 0.0, 0.0, 0.0, 0.0, ku); };
 
     // ---- bisector loop (ConvexCell.cl:882-913) --------------------------------
+    // The 64 neighbours of a chunk are loaded one per lane with their plane,
+    // distance |b| and plane offset t (signed distance of the bisector from
+    // the seed).  A plane with t >= R, R the distance from the seed to the
+    // farthest vertex, cannot cut the cell, so instead of visiting every
+    // neighbour the wave ballots for the next neighbour whose plane may cut
+    // (t < R) and for the first one where the stop test holds, and jumps to
+    // the earlier of the two.  Neighbours in between would neither clip nor
+    // stop, so the cell, the stop index and the plane-overflow rule (a full
+    // plane list fails at the next neighbour, ConvexCell.cl:316-319) are
+    // those of the one-by-one loop.
     const int *nrow = a.knn + t * int64_t(K);
     bool reached = false, complete = false;
     int kused = 0;
     Pl<RP> cpl;
-    R cd2 = R(0);
+    Pl<R> cplr;
+    double cdl = 0, ct = 0;
     int cnb = -1;
-    for (int i = 0; i < K; ++i) {
-        if ((i & 63) == 0) {
-            // planes of the next 64 neighbours, one per lane
-            const int j = i + lane;
-            cnb = j < K ? nrow[j] : -1;
+    int loaded = -1;
+    const double rmarg = sizeof(R) == 8 ? 1.0 + 1e-9 : 1.0 + 2e-4;  // FP32 R may be low by ~1e-4
+    double Rcur = sqrt(double(maxr2));
+    auto stop_at = [&](double dl, double Rm) -> bool {
+        const double Rs = Rm * rmarg;
+        if (a.security_mode == 1 || wgap <= 0) return dl * dl > 4.0 * Rs * Rs;
+        return dl * (dl - 2.0 * Rs) >= wgap;
+    };
+    int i = 0;
+    while (i < K) {
+        const int base = i & ~63;
+        if (base != loaded) {
+            const int j = base + lane;
+            cnb = j < K ? nrow[j] : -2;  // -1: list exhausted, -2: past K
             cpl = Pl<RP>{RP(0), RP(0), RP(0), RP(0)};
-            cd2 = R(0);
+            cdl = 0;
+            ct = 0;
             if (cnb >= 0) {
                 const double4 B = a.seeds[cnb];
                 const double bx = B.x - S0.x, by = B.y - S0.y, bz = B.z - S0.z;
@@ -506,33 +545,70 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
                 else
                     d = (b2 + (B.w - S0.w)) / (2.0 * bl);
                 cpl = Pl<RP>{RP(-bx / bl), RP(-by / bl), RP(-bz / bl), RP(d)};
-                cd2 = R(b2);
+                cdl = bl;
+                ct = d;
             }
+            cplr = cvt<R>(cpl);
+            loaded = base;
         }
-        const int l = i & 63;
-        if (rlane(cnb, l) < 0) {
+        const int l0 = i & 63;
+        const bool in = lane >= l0 && cnb != -2;
+        if (nb_plane >= P) {
+            // the next neighbour's plane cannot be stored
+            const uint64_t me = ballot(in && cnb == -1), mv = ballot(in && cnb >= 0);
+            if (mv && (!me || lowest_bit(mv) < lowest_bit(me))) {
+                fail(RPD_PLANE_OVERFLOW, base + lowest_bit(mv) + 1);
+                return;
+            }
+            if (me) {
+                complete = true;
+                break;
+            }
+            i = base + 64;
+            continue;
+        }
+        const double Rskip = Rcur * rmarg;
+        const uint64_t m_end = ballot(in && cnb == -1);
+        const uint64_t m_clip = ballot(in && cnb >= 0 && ct < Rskip);
+        const uint64_t m_stop = ballot(in && cnb >= 0 && stop_at(cdl, Rcur));
+        const int f_end = m_end ? lowest_bit(m_end) : 64;
+        const int f_clip = m_clip ? lowest_bit(m_clip) : 64;
+        const int f_stop = m_stop ? lowest_bit(m_stop) : 64;
+        if (f_end < f_clip && f_end < f_stop) {
             complete = true;  // every other seed has been used
+            kused = base + f_end;
             break;
         }
-        const Pl<RP> e = rlane_pl(cpl, l);
-        const R d2n = rlane(cd2, l);
-        if (nb_plane >= P) {
-            fail(RPD_PLANE_OVERFLOW, i + 1);
-            return;
+        if (f_stop < f_clip) {
+            reached = true;
+            kused = base + f_stop + 1;
+            break;
         }
-        const int res = clip(e);
-        kused = i + 1;
+        if (f_clip == 64) {
+            i = base + 64;
+            kused = min(K, i);
+            continue;
+        }
+        // clip by the plane of neighbour base + f_clip
+        const Pl<R> e = rlane_pl(cplr, f_clip);
+        const int res = clip(e, f_clip, cpl);
+        kused = base + f_clip + 1;
         if (res >= 0) {
             fail(res, kused);
             return;
         }
-        // CCIsSecurityRadiusReached; the mixed form keeps a relative margin
-        // for FP32 distances and FP32 neighbour order
-        const R margin = sizeof(R) == 8 ? R(1) : R(1.00001f);
-        if (d2n > R(4) * maxr2 * margin) {
+        if (res == kClipped) {
+            Rcur = sqrt(double(maxr2));
+            if (double(maxr2) * 1.001 < cull_r2) {
+                finish(RPD_SUCCESS, 0.0, 0.0, 0.0, 0.0, kused);
+                return;
+            }
+        }
+        if (stop_at(rlane(cdl, f_clip), Rcur)) {
             reached = true;
             break;
         }
+        i = base + f_clip + 1;
     }
     if (!reached && !complete) {
         fail(RPD_SECURITY_RADIUS_NOT_REACHED, kused);
@@ -762,7 +838,8 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
                     nz_ /= nl;
                     nd /= nl;
                 }
-                res = clip(Pl<RP>{RP(nx_), RP(ny_), RP(nz_), RP(nd)});
+                const Pl<RP> cp{RP(nx_), RP(ny_), RP(nz_), RP(nd)};
+                res = clip(cvt<R>(cp), -1, cp);
                 if (res >= 0) break;
             }
             if (res == RPD_EMPTY_CELL) {
@@ -815,7 +892,10 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
         a.tris = L.dom->tris;
         a.tlist = L.dom->tlist;
         a.toffs = L.dom->toffs;
+        a.dist = L.cull ? L.dom->dist : nullptr;
     }
+    a.security_mode = L.security_mode;
+    a.wmin = L.wmin;
     a.out = L.out;
     a.status = L.status;
     a.kused = L.kused;

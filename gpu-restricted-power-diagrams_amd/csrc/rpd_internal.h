@@ -59,6 +59,7 @@ struct DomainDev {
     const int *tris = nullptr;
     const int *tlist = nullptr;
     const int *toffs = nullptr;
+    const uint8_t *dist = nullptr;  // Chebyshev voxel distance to domain voxels
     bool has_border_vertices = false;
 };
 
@@ -70,6 +71,9 @@ struct ClipLaunch {
     int64_t n_ids;
     int K, P, V, T;
     int apex_mode;
+    int security_mode;
+    double wmin;
+    bool cull;
     const DomainDev *dom;  // nullptr: no domain
     double4 *out;          // original order
     int *status;           // original order

@@ -131,13 +131,22 @@ typedef struct {
     int32_t precision;            /* enum rpd_precision                       */
     int32_t max_extra_steps;      /* re-queue passes with caps x1.5 (ref: 5)  */
     int32_t apex_mode;            /* 0 reference corner apex, 1 (default)
-                                     centre of a triangle-free voxel first    */
-    int32_t reserved;
+                                     centre of the first voxel of the cell's
+                                     bbox whose centre class is known         */
+    int32_t security_mode;        /* 0 (default) exact power-diagram stop:
+                                     d*(d - 2R) >= w_s - w_min; 1 reference
+                                     stop d^2 > 4R^2 (ConvexCell.cl:364-382),
+                                     which ignores weights; both equal at w=0 */
+    int32_t cull;                 /* 1 (default): end a cell as soon as its
+                                     circumsphere provably misses the domain
+                                     (restricted volume 0); 0: clip it fully  */
+    int32_t reserved[3];
 } rpd_params;
 
 /* Per-solve statistics (VoroAlgoStat, voronoi.h:60-103). */
 typedef struct {
     double ms_bin, ms_knn, ms_clip, ms_fallback, ms_total; /* device time   */
+    double ms_knn_first, ms_clip_first; /* kNN and clip launch of pass 1     */
     int32_t status_hist[RPD_STATUS_NUM];  /* final status per cell           */
     int32_t passes;                 /* clip passes run                       */
     int32_t fallback_cells;         /* cells recomputed in FP64 (mixed form) */

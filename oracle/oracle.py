@@ -41,7 +41,7 @@ class OrcGrid(ctypes.Structure):
 class OrcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "K", "P", "V", "T", "precision", "vhs0", "vhs1", "apex_mode",
-        "fix_exact_requeue", "max_extra_steps", "threads", "single_pass")]
+        "fix_exact_requeue", "max_extra_steps", "threads", "single_pass", "security_mode")]
 
 
 class OrcStats(ctypes.Structure):
@@ -83,10 +83,13 @@ def lib():
         L.orc_knn_dim.argtypes = [ctypes.c_int]
         L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcParams),
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(OrcStats)]
+        L.orc_solve_ids.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcParams), ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(OrcStats)]
         L.orc_clip_cell.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                    ctypes.POINTER(OrcGrid), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
-                                    ctypes.c_void_p, ctypes.c_void_p]
+                                    ctypes.POINTER(OrcGrid), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_grid_volume.argtypes = [ctypes.POINTER(OrcGrid)]
         L.orc_grid_volume.restype = ctypes.c_double
         L.orc_grid_valid.argtypes = [ctypes.POINTER(OrcGrid)]
@@ -145,11 +148,11 @@ def knn(xyzw, K, precision=1, method=0):
 
 
 def params(K=25, P=35, V=40, T=100, precision=1, vhs0=2, vhs1=2, apex_mode=1, fix_exact_requeue=1,
-           max_extra_steps=5, threads=0, single_pass=0):
+           max_extra_steps=5, threads=0, single_pass=0, security_mode=0):
     p = OrcParams()
     for k, v in dict(K=K, P=P, V=V, T=T, precision=precision, vhs0=vhs0, vhs1=vhs1, apex_mode=apex_mode,
                      fix_exact_requeue=fix_exact_requeue, max_extra_steps=max_extra_steps, threads=threads,
-                     single_pass=single_pass).items():
+                     single_pass=single_pass, security_mode=security_mode).items():
         setattr(p, k, int(v))
     return p
 
@@ -171,7 +174,25 @@ def solve(xyzw, grid=None, **kw):
     return out, status, kused, st.as_dict()
 
 
-def clip_cell(xyzw, seed, neigs, K, P=64, V=128, T=0, precision=1, vhs=2, grid=None, apex_mode=1, out_domain=0):
+def solve_ids(xyzw, ids, grid=None, **kw):
+    """Cells of the seeds `ids` only (every seed still a neighbour)."""
+    xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    out = np.zeros((ids.size, 4), np.float64)
+    status = np.zeros(ids.size, np.int32)
+    kused = np.zeros(ids.size, np.int32)
+    st = OrcStats()
+    p = params(**kw)
+    g = grid.orc() if grid is not None else None
+    r = lib().orc_solve_ids(_ptr(xyzw), xyzw.shape[0], _ptr(ids), ids.size, ctypes.byref(g) if g is not None else None,
+                            ctypes.byref(p), _ptr(out), _ptr(status), _ptr(kused), ctypes.byref(st))
+    if r < 0:
+        raise RuntimeError("orc_solve_ids failed %d" % r)
+    return out, status, kused, st.as_dict()
+
+
+def clip_cell(xyzw, seed, neigs, K, P=64, V=128, T=0, precision=1, vhs=2, grid=None, apex_mode=1, out_domain=0,
+              security_mode=1, wmin=0.0):
     xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
     neigs = np.ascontiguousarray(neigs, dtype=np.int32)
     out = np.zeros(4, np.float64)
@@ -180,7 +201,8 @@ def clip_cell(xyzw, seed, neigs, K, P=64, V=128, T=0, precision=1, vhs=2, grid=N
     g = grid.orc() if grid is not None else None
     st = lib().orc_clip_cell(_ptr(xyzw), xyzw.shape[0], int(seed), _ptr(neigs), int(K), int(P), int(V), int(T),
                              int(precision), int(vhs), ctypes.byref(g) if g is not None else None, int(apex_mode),
-                             int(out_domain), _ptr(out), ctypes.byref(nv), ctypes.byref(npl))
+                             int(out_domain), int(security_mode), ctypes.c_double(wmin), _ptr(out), ctypes.byref(nv),
+                             ctypes.byref(npl))
     return st, out, nv.value, npl.value
 
 

@@ -184,6 +184,8 @@ struct Cell {
     const R4<R> *pts = nullptr;
     int status = kSuccess;
     int P = 0, V = 0, vhs = 2;
+    int security_mode = 1;  // 0 exact power stop, 1 reference stop
+    double wmin = 0;        // minimum weight over non-border seeds
     int nb_vertex = 0, nb_plane = 0;
     R4<R> planes[256];
     V3 verts[256];
@@ -264,7 +266,15 @@ struct Cell {
         ctr->n_radius_vertex += nb_vertex;
         const R4<R> d = sub4(last, seed);
         const R d2 = dot3(d, d);
-        return d2 > R(4) * vd;
+        const double c = double(seed.w) - wmin;
+        if (security_mode == 1 || c <= 0) return d2 > R(4) * vd;
+        // exact power stop: a seed B at distance d' >= dl has its bisector at
+        // distance (d'^2 + w_B - w_s) / (2 d') >= (d'^2 - c) / (2 d') from the
+        // seed, increasing in d'; no later bisector meets the ball of radius R
+        // around the seed, which holds the cell, iff dl (dl - 2R) >= c.
+        // Border seeds (plane through B, distance d' >= dl >= 2R) never do.
+        const double dl = std::sqrt(double(d2)), Rm = std::sqrt(double(vd));
+        return dl * (dl - 2.0 * Rm) >= c;
     }
 
     // CCVertexIsInHalfSpace :400-461
@@ -481,7 +491,8 @@ inline bool merge_union(const std::vector<int> &a, const int *b, int nb, int cap
 // working precision; neigs holds K indices into pts (-1 = none).
 template <class R>
 CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, int P, int V, int T, int vhs,
-                           const GridView<R> *grid, int apex_mode, bool out_domain, R4<R> *out, Counters *ctr) {
+                           const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
+                           R4<R> *out, Counters *ctr) {
     CellResult<R> res;
     const R vol_eps2 = sizeof(R) == sizeof(float) ? R(1.e-6) : R(1.e-12);  // voronoi.cpp:919-922
     if (double(pts[seed].w) <= kNoCellLimits) {
@@ -493,6 +504,8 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
     cc.P = P;
     cc.V = V;
     cc.vhs = vhs;
+    cc.security_mode = security_mode;
+    cc.wmin = wmin;
     cc.ctr = ctr;
     cc.init(pts, seed, grid && grid->g > 0 ? R(0.1) : R(0));
     bool complete = false;  // every other seed consumed
@@ -871,7 +884,8 @@ int knn_all(const KnnGrid<R> &kg, int K, int method, const int *ids, int nids, i
 // voronoi.cpp:861-1036): kNN with K, then one cell per id.
 template <class R>
 void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, int V, int T, int vhs,
-               const GridView<R> *grid, int apex_mode, bool out_domain, std::vector<R4<R>> &out,
+               const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
+               std::vector<R4<R>> &out,
                std::vector<int> &status, std::vector<int> &kused, Counters &ctr, double &t_knn, double &t_clip) {
     const long long nids = (long long)ids.size();
     if (nids == 0) return;
@@ -887,7 +901,7 @@ void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, 
         for (long long t = 0; t < nids; ++t) {
             const int s = ids[size_t(t)];
             CellResult<R> r = compute_cell<R>(kg.stored.data(), s, nb.data() + t * K, K, P, V, T, vhs, grid,
-                                              apex_mode, out_domain, &out[size_t(s)], &local);
+                                              apex_mode, out_domain, security_mode, wmin, &out[size_t(s)], &local);
             status[size_t(s)] = r.status;
             kused[size_t(s)] = r.kused;
         }
@@ -902,7 +916,7 @@ void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, 
 
 template <class R>
 int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *prm, double *out_mg,
-               int *out_status, int *out_kused, orc_stats *st) {
+               int *out_status, int *out_kused, orc_stats *st, const int *ids_orig = nullptr, int n_ids = 0) {
     auto tstart = std::chrono::steady_clock::now();
     set_threads(prm->threads);
     const std::vector<R4<R>> pts = convert_pts<R>(xyzw, n);
@@ -910,6 +924,11 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
     GridView<R> gv = make_grid_view<R>(gr);
     const GridView<R> *grid = gr ? &gv : nullptr;
     const bool dbl = sizeof(R) == sizeof(double);
+    const int sm = prm->security_mode;
+    double wmin = std::numeric_limits<double>::infinity();
+    for (const auto &p : pts)
+        if (double(p.w) > kNoCellLimits) wmin = std::min(wmin, double(p.w));
+    if (!(wmin < std::numeric_limits<double>::infinity())) wmin = 0;
     std::vector<R4<R>> out(size_t(n), mk4<R>(0, 0, 0, 0));
     std::vector<int> status(size_t(n), kSuccess), kused(size_t(n), 0);
     Counters ctr;
@@ -917,16 +936,25 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
     int K = prm->K, P = prm->P, V = prm->V, T = prm->T;
     int vhs = dbl ? 2 : prm->vhs0;
     int passes = 0;
-    std::vector<int> all(static_cast<size_t>(n));
-    for (int i = 0; i < n; ++i) all[size_t(i)] = i;
+    std::vector<int> all;
+    if (ids_orig) {
+        std::vector<int> inv(static_cast<size_t>(n));
+        for (int s = 0; s < n; ++s) inv[size_t(kg.perm[size_t(s)])] = s;
+        all.resize(size_t(n_ids));
+        for (int i = 0; i < n_ids; ++i) all[size_t(i)] = inv[size_t(ids_orig[i])];
+    } else {
+        all.resize(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) all[size_t(i)] = i;
+    }
 
     if (prm->single_pass) {
-        clip_pass(kg, all, K, P, V, T, vhs, grid, prm->apex_mode, grid && gv.has_border_vertices, out, status, kused,
-                  ctr, t_knn, t_clip);
+        clip_pass(kg, all, K, P, V, T, vhs, grid, prm->apex_mode, grid && gv.has_border_vertices, sm, wmin, out, status,
+                  kused, ctr, t_knn, t_clip);
         passes = 1;
     } else {
         // first pass: T = 0 defers every boundary cell (voronoi.cpp:1294-1297)
-        clip_pass(kg, all, K, P, V, 0, vhs, grid, prm->apex_mode, false, out, status, kused, ctr, t_knn, t_clip);
+        clip_pass(kg, all, K, P, V, 0, vhs, grid, prm->apex_mode, false, sm, wmin, out, status, kused, ctr, t_knn,
+                  t_clip);
         passes = 1;
         auto collect = [&](const std::vector<int> &from, auto cond, std::vector<int> &ids0, std::vector<int> &ids1) {
             ids0.clear();
@@ -940,7 +968,8 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
         std::vector<int> ids0, ids1;
         collect(all, [](int s) { return s == kTriangleOverflow; }, ids0, ids1);
         if (!ids1.empty()) {  // tr_overflow pass (voronoi.cpp:1304-1310)
-            clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, false, out, status, kused, ctr, t_knn, t_clip);
+            clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, false, sm, wmin, out, status, kused, ctr,
+                      t_knn, t_clip);
             passes++;
         }
         if (!ids0.empty()) {
@@ -964,21 +993,22 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
                 if (!relaunch) break;
                 if (P >= 256) P = 255;  // uchar plane ids (voronoi.cpp:867-870)
                 K = std::min(K, n - 1 > 0 ? n - 1 : 1);
-                clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, out_domain, out, status, kused, ctr, t_knn,
-                          t_clip);
+                clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, out_domain, sm, wmin, out, status, kused,
+                          ctr, t_knn, t_clip);
                 passes++;
                 cur = ids0;
                 collect(cur, [](int s) { return s != kEmptyCell; }, ids0, ids1);
             }
         }
     }
-    for (int s = 0; s < n; ++s) {
-        const int o = kg.perm[size_t(s)];
+    for (size_t k = 0; k < all.size(); ++k) {
+        const int s = all[k];
+        const size_t o = ids_orig ? k : size_t(kg.perm[size_t(s)]);
         const R4<R> v = out[size_t(s)];
-        out_mg[4 * size_t(o) + 0] = double(v.x);
-        out_mg[4 * size_t(o) + 1] = double(v.y);
-        out_mg[4 * size_t(o) + 2] = double(v.z);
-        out_mg[4 * size_t(o) + 3] = double(v.w);
+        out_mg[4 * o + 0] = double(v.x);
+        out_mg[4 * o + 1] = double(v.y);
+        out_mg[4 * o + 2] = double(v.z);
+        out_mg[4 * o + 3] = double(v.w);
         if (out_status) out_status[o] = status[size_t(s)];
         if (out_kused) out_kused[o] = kused[size_t(s)];
     }
@@ -990,7 +1020,7 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
         st->n_bbox_vertex = ctr.n_bbox_vertex;
         st->n_integrated_vertex = ctr.n_integrated_vertex;
         st->n_clip = ctr.n_clip;
-        for (int s = 0; s < n; ++s) st->status_hist[status[size_t(s)]]++;
+        for (int s : all) st->status_hist[status[size_t(s)]]++;
         st->passes = passes;
         st->final_K = K;
         st->final_P = P;
@@ -1033,9 +1063,22 @@ int orc_solve(const double *xyzw, int n, const orc_grid *grid, const orc_params 
                             : solve_impl<double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st);
 }
 
+int orc_solve_ids(const double *xyzw, int n, const int *ids_orig, int n_ids, const orc_grid *grid,
+                  const orc_params *prm, double *out_mg, int *out_status, int *out_kused, orc_stats *st) {
+    if (n <= 1 || !xyzw || !prm || !out_mg || !ids_orig || n_ids <= 0) return -1;
+    for (int i = 0; i < n_ids; ++i)
+        if (ids_orig[i] < 0 || ids_orig[i] >= n) return -1;
+    if (prm->P >= 256 || prm->K <= 0 || prm->V <= 0) return -2;
+    orc_params p = *prm;
+    p.K = std::min(p.K, n - 1);
+    return p.precision == 0
+               ? solve_impl<float>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, ids_orig, n_ids)
+               : solve_impl<double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, ids_orig, n_ids);
+}
+
 int orc_clip_cell(const double *xyzw, int n, int seed, const int *neigs, int K, int P, int V, int T, int precision,
-                  int vhs, const orc_grid *grid, int apex_mode, int out_domain, double *out4, int *nb_vertex,
-                  int *nb_plane) {
+                  int vhs, const orc_grid *grid, int apex_mode, int out_domain, int security_mode, double wmin,
+                  double *out4, int *nb_vertex, int *nb_plane) {
     auto run = [&](auto tag) -> int {
         using R = decltype(tag);
         const auto pts = convert_pts<R>(xyzw, n);
@@ -1043,7 +1086,8 @@ int orc_clip_cell(const double *xyzw, int n, int seed, const int *neigs, int K, 
         Counters ctr;
         R4<R> out = mk4<R>(0, 0, 0, 0);
         CellResult<R> r = compute_cell<R>(pts.data(), seed, neigs, K, P, V, T, sizeof(R) == 8 ? 2 : vhs,
-                                          grid ? &gv : nullptr, apex_mode, out_domain != 0, &out, &ctr);
+                                          grid ? &gv : nullptr, apex_mode, out_domain != 0, security_mode, wmin,
+                                          &out, &ctr);
         out4[0] = double(out.x);
         out4[1] = double(out.y);
         out4[2] = double(out.z);

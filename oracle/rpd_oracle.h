@@ -44,6 +44,9 @@ typedef struct {
     int max_extra_steps;         /* extra passes (reference: 5)                */
     int threads;                 /* OpenMP threads, <=0: all                   */
     int single_pass;             /* 1: no re-queue, one pass with T = T        */
+    int security_mode;           /* 0: exact power stop d(d-2R) >= w_s - w_min;
+                                    1: reference stop d^2 > 4R^2 (weights
+                                    ignored, ConvexCell.cl:364-382)            */
 } orc_params;
 
 typedef struct {
@@ -76,11 +79,19 @@ int orc_solve(const double *xyzw, int n, const orc_grid *grid,
               const orc_params *prm, double *out_mg, int *out_status,
               int *out_kused, orc_stats *st);
 
+/* Same as orc_solve for the cells of a subset of seeds only (ids_orig:
+   original indices); every seed still acts as a neighbour.  out_mg, out_status
+   and out_kused are indexed like ids_orig.  Used for bounded CPU timings. */
+int orc_solve_ids(const double *xyzw, int n, const int *ids_orig, int n_ids,
+                  const orc_grid *grid, const orc_params *prm, double *out_mg,
+                  int *out_status, int *out_kused, orc_stats *st);
+
 /* One cell, one pass, caller-given neighbour list (original indices of
    xyzw).  For unit tests of the clip.  Returns the status. */
 int orc_clip_cell(const double *xyzw, int n, int seed, const int *neigs, int K,
                   int P, int V, int T, int precision, int vhs,
                   const orc_grid *grid, int apex_mode, int out_domain,
+                  int security_mode, double wmin,
                   double *out4, int *nb_vertex, int *nb_plane);
 
 /* Grid helpers restating Grid.cpp. */
