@@ -25,6 +25,7 @@ STATUS_NAMES = [
 SUCCESS = 4
 EMPTY_CELL = 6
 NO_CELL_LIMITS = -100000.0
+MAX_PASSES = 8  # RPD_MAX_PASSES
 
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -74,6 +75,8 @@ class Stats(ctypes.Structure):
         ("ms_total", ctypes.c_double),
         ("ms_knn_first", ctypes.c_double),
         ("ms_clip_first", ctypes.c_double),
+        ("ms_cells_first", ctypes.c_double),
+        ("ms_finish_first", ctypes.c_double),
         ("status_hist", ctypes.c_int32 * 11),
         ("passes", ctypes.c_int32),
         ("fallback_cells", ctypes.c_int32),
@@ -82,12 +85,23 @@ class Stats(ctypes.Structure):
         ("final_V", ctypes.c_int32),
         ("final_T", ctypes.c_int32),
         ("dim", ctypes.c_int32),
+        ("queued_first", ctypes.c_int32),
         ("n", ctypes.c_int64),
+        ("pass_cells", ctypes.c_int32 * MAX_PASSES),
+        ("pass_caps", (ctypes.c_int32 * 4) * MAX_PASSES),
+        ("pass_ms_knn", ctypes.c_double * MAX_PASSES),
+        ("pass_ms_clip", ctypes.c_double * MAX_PASSES),
     ]
 
     def as_dict(self):
-        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "status_hist"}
+        arrays = ("status_hist", "pass_cells", "pass_caps", "pass_ms_knn", "pass_ms_clip")
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in arrays}
         d["status_hist"] = list(self.status_hist)
+        np_ = min(self.passes, MAX_PASSES)
+        d["pass_cells"] = list(self.pass_cells)[:np_]
+        d["pass_caps"] = [list(c) for c in self.pass_caps][:np_]
+        d["pass_ms_knn"] = list(self.pass_ms_knn)[:np_]
+        d["pass_ms_clip"] = list(self.pass_ms_clip)[:np_]
         return d
 
 

@@ -47,7 +47,7 @@ struct rpd_ctx {
     // outputs, original order
     DevBuf out, status, kused;
     // pass work buffers
-    DevBuf knn, ids_a, ids_b, ids_c, cnt, wred;
+    DevBuf knn, ids_a, ids_b, ids_c, cnt, wred, rec, qcount;
     bool solved = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
@@ -394,10 +394,16 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         for (double v : red) wmin = std::min(wmin, v);
         if (!(wmin < INFINITY)) wmin = 0;
     }
+    double ms_cells = 0, ms_finish = 0;
+    int n_queued = 0;
     auto run_pass = [&](int form, const int *ids, int64_t n_ids, int K_, int P_, int V_, int T_, double &ms_knn,
                         double &ms_clip) -> int {
         if (n_ids <= 0) return RPD_OK;
         int r = c->knn.ensure(size_t(n_ids) * K_ * 4);
+        if (r != RPD_OK) return r;
+        r = c->rec.ensure(size_t(n_ids) * rpd_clip_record_bytes(form, P_, V_));
+        if (r != RPD_OK) return r;
+        r = c->qcount.ensure(sizeof(int));
         if (r != RPD_OK) return r;
         hipEventRecord(c->ev[0], c->stream);
         r = rpd_launch_knn(c->stream, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids,
@@ -410,6 +416,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.ids = ids;
         L.knn = c->knn.as<int>();
         L.n_ids = n_ids;
+        L.n_seeds = n;
         L.K = K_;
         L.P = P_;
         L.V = V_;
@@ -422,12 +429,27 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.out = c->out.as<double4>();
         L.status = c->status.as<int>();
         L.kused = c->kused.as<int>();
+        L.rec = c->rec.ptr;
+        L.qcount = c->qcount.as<int>();
+        L.ev_mid = c->ev[3];
+        L.n_queued = &n_queued;
         r = rpd_launch_clip(c->stream, form, L);
         if (r != RPD_OK) return r;
         hipEventRecord(c->ev[2], c->stream);
         if (hipStreamSynchronize(c->stream) != hipSuccess) return RPD_E_DEVICE;
-        ms_knn += elapsed_ms(c->ev[0], c->ev[1]);
-        ms_clip += elapsed_ms(c->ev[1], c->ev[2]);
+        const double mk = elapsed_ms(c->ev[0], c->ev[1]), mc = elapsed_ms(c->ev[1], c->ev[2]);
+        ms_knn += mk;
+        ms_clip += mc;
+        ms_cells = elapsed_ms(c->ev[1], c->ev[3]);
+        ms_finish = elapsed_ms(c->ev[3], c->ev[2]);
+        const int pi = std::min(st.passes, RPD_MAX_PASSES - 1);
+        st.pass_cells[pi] += int32_t(n_ids);
+        st.pass_caps[pi][0] = K_;
+        st.pass_caps[pi][1] = P_;
+        st.pass_caps[pi][2] = V_;
+        st.pass_caps[pi][3] = T_;
+        st.pass_ms_knn[pi] += mk;
+        st.pass_ms_clip[pi] += mc;
         return RPD_OK;
     };
 
@@ -436,6 +458,9 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     st.passes = 1;
     st.ms_knn_first = ms_knn;
     st.ms_clip_first = ms_clip;
+    st.ms_cells_first = ms_cells;
+    st.ms_finish_first = ms_finish;
+    st.queued_first = n_queued;
 
     // re-queue passes: the failed cells of the last pass, in one FP64 launch
     // per step, with the caps grown x1.5 for each overflow kind seen

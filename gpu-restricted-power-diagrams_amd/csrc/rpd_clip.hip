@@ -18,6 +18,16 @@
 // and places one new vertex (plane, a, b) per boundary edge a->b into free
 // lanes -- the same vertex set CCComputeBoundary + CCNewVertex build.
 //
+// Two kernels per pass:
+//   clip_cells_kernel   the bisector loop for every cell (light, high
+//                       occupancy).  A cell that provably misses the domain
+//                       ends there with volume 0; a finished cell is appended
+//                       to a queue as a compact record (FP64 planes + vertex
+//                       triples);
+//   finish_cells_kernel restriction to the domain (bbox, voxels, apex, cone
+//                       clips per boundary triangle) and the integration, for
+//                       the queued cells only.
+//
 // Everything is in the seed-local frame (origin at the seed, FP64 inputs):
 // planes n.y + d with d = (|b|^2 + w_B - w_s) / (2|b|), b = B - s.
 //
@@ -51,6 +61,7 @@ struct ClipArgs {
     const int *ids;
     const int *knn;
     int64_t n_ids;
+    int64_t n_seeds;
     int K, P, V, T;
     int apex_mode;
     int g;
@@ -64,11 +75,15 @@ struct ClipArgs {
     double4 *out;
     int *status;
     int *kused;
-    int lds_stride;
+    int lds_stride;   // LDS bytes of one wave
+    int cell_stride;  // LDS bytes shared by the waves of one cell (finish kernel)
     int tl_cap;
     int security_mode;    // 0 exact power stop, 1 reference stop
     double wmin;          // minimum weight over non-border seeds
     const uint8_t *dist;  // Chebyshev voxel distance to the nearest domain voxel (nullable)
+    char *rec;            // queue of finished cells (records of rec_stride bytes)
+    int rec_stride;
+    int *qcount;          // number of queued records
 };
 
 __device__ __forceinline__ uint32_t pack3(uint32_t a, uint32_t b, uint32_t c) { return a | (b << 8) | (c << 16); }
@@ -80,11 +95,6 @@ __device__ __forceinline__ void wsync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <typename R>
-__device__ __forceinline__ R rabs(R v) {
-    return v < R(0) ? -v : v;
 }
 
 template <typename R>
@@ -137,150 +147,95 @@ constexpr float kFilterKappa = 0x1.0p-18f;
 // ill-conditioned in FP32 and sent to FP64
 constexpr float kFilterMinW = 1.0e-3f;
 
-template <typename R, bool FILTER, int S>
-struct CellState {
-    uint32_t tri[S];
-    R X[S], Y[S], Z[S], W[S];
-    float MX[S], MY[S], MZ[S], MW[S];
-    R r2[S];
-    uint64_t valid[S];
-};
+// apex_mode 1 puts the cone apex at this fraction of a voxel whose interior
+// has one known class (in or out).  Any interior point is valid; a generic
+// one keeps the cone planes off the voxel lattice and off the CUBE_EPSILON
+// box (the voxel centre lies on planes such as y + z = 1000 that contain box
+// edges such as y = 1000.1, z = -0.1).
+__device__ constexpr double kApexFrac[3] = {0.530901699437495, 0.520710678118655, 0.536602540378444};
 
-// R: precision of the plane-side tests and of the vertex registers;
-// RP: precision of the stored planes and of the cell geometry (bbox, cone
-// planes, integration).  FP64 form <double, double>, mixed <float, double>,
-// FP32 form <float, float>.
+// CCInit vertex triples (ConvexCell.cl:299-306), one byte per vertex:
+// vertex i has planes (kCube0 >> 8i, kCube1 >> 8i, kCube2 >> 8i) & 0xff.
+constexpr uint64_t kCube0 = 0x0402040405010502ull;  // 2 5 1 5 4 4 2 4
+constexpr uint64_t kCube1 = 0x0304000201050305ull;  // 5 3 5 1 2 0 4 3
+constexpr uint64_t kCube2 = 0x0101030003020000ull;  // 0 0 2 3 0 3 1 1
+
+// ---------------------------------------------------------------------------
+// The cell of one wavefront: vertices in lanes, planes in LDS.
+// ---------------------------------------------------------------------------
 template <typename R, typename RP, bool FILTER, int S>
-__global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
-    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
-    const int lane = lane_id();
-    const int wv = threadIdx.x >> 6;
-    const int wpb = blockDim.x >> 6;
-    const int64_t t = int64_t(blockIdx.x) * wpb + wv;
-    if (t >= a.n_ids) return;  // wave-uniform
+struct WaveCell {
+    Pl<RP> *planes;
+    uint32_t *rt;
+    uint32_t *nvs;
+    uint8_t *nxt;
+    int P, V, nb_plane, lane;
+    uint32_t tri[S];
+    R X[S], Y[S], Z[S], W[S], r2[S];
+    float MX[S], MY[S], MZ[S], MW[S];
+    uint64_t valid[S];
+    R maxr2;
 
-    char *L = lds_raw + size_t(wv) * a.lds_stride;
-    Pl<RP> *planes = reinterpret_cast<Pl<RP> *>(L);
-    size_t off = (size_t(a.P) * sizeof(Pl<RP>) + 15) & ~size_t(15);
-    uint32_t *rt = reinterpret_cast<uint32_t *>(L + off);
-    off += 4 * 64 * S;
-    uint32_t *nvs = reinterpret_cast<uint32_t *>(L + off);
-    off += 4 * 64 * S;
-    uint8_t *nxt = reinterpret_cast<uint8_t *>(L + off);
-    off += 256;
-    int *tl = reinterpret_cast<int *>(L + off);
-    off += 4 * size_t(a.tl_cap);
-    int *tu = reinterpret_cast<int *>(L + off);
-
-    const int sid = a.ids ? a.ids[t] : int(t);
-    const double4 S0 = a.seeds[sid];
-    const int orig = a.perm[sid];
-    const int K = a.K, P = a.P, V = a.V;
-
-    auto finish = [&](int st, double m0, double m1, double m2, double m3, int ku) {
-        if (lane == 0) {
-            a.out[orig] = make_double4(m0, m1, m2, m3);
-            a.status[orig] = st;
-            if (a.kused) a.kused[orig] = ku;
-        }
-    };
-
-    if (S0.w <= RPD_NO_CELL_LIMITS) {  // border seed: copied (ConvexCell.cl:873-876)
-        finish(RPD_SUCCESS, S0.x, S0.y, S0.z, S0.w, 0);
-        return;
+    __device__ void set_vertex(int s, uint32_t tr, const Pl<R> &p1, const Pl<R> &p2, const Pl<R> &p3) {
+        tri[s] = tr;
+        vertex_from_planes<R, FILTER>(p1, p2, p3, X[s], Y[s], Z[s], W[s], MX[s], MY[s], MZ[s], MW[s]);
+        const R iw = R(1) / W[s];
+        const R ix = X[s] * iw, iy = Y[s] * iw, iz = Z[s] * iw;
+        r2[s] = ix * ix + iy * iy + iz * iz;
     }
 
-    for (int i = lane; i < 256; i += 64) nxt[i] = 0xff;
-
-    // Domain cull: a seed whose voxel is d >= 2 voxels (Chebyshev) from every
-    // voxel that holds triangles or lies inside has no point of the domain
-    // within (d - 1) * vs; once the cell's circumradius about the seed is
-    // below that, its restriction to the domain is empty (volume 0, as the
-    // reference's apex-outside / no-triangle case, ConvexCell.cl:1067-1079).
-    double cull_r2 = 0;
-    if (a.g > 0 && a.dist) {
-        const int g = a.g;
-        auto vox = [&](double c) { return int(fmin(fmax(c / a.vs, 0.0), double(g - 1))); };
-        const int d = a.dist[vox(S0.x) + vox(S0.y) * g + vox(S0.z) * g * g];
-        if (d >= 2) cull_r2 = double(d - 1) * a.vs * double(d - 1) * a.vs;
-    }
-    const double wgap = S0.w - a.wmin;  // >= 0 except for the lightest seeds
-
-    // ---- CCInit: bounding cube in the local frame ---------------------------
-    const double eps = a.g > 0 ? 0.1 : 0.0;  // CUBE_EPSILON, voronoi.cpp:929
-    const double vmin = -eps, vmax = 1000.0 + eps;
-    if (lane < 6) {
-        const int ax = lane >> 1;
-        const double sc = ax == 0 ? S0.x : (ax == 1 ? S0.y : S0.z);
-        const double sgn = (lane & 1) ? -1.0 : 1.0;
-        const double d = (lane & 1) ? vmax - sc : sc - vmin;
-        Pl<RP> p;
-        p.x = ax == 0 ? RP(sgn) : RP(0);
-        p.y = ax == 1 ? RP(sgn) : RP(0);
-        p.z = ax == 2 ? RP(sgn) : RP(0);
-        p.w = RP(d);
-        planes[lane] = p;
-    }
-    wsync();
-    int nb_plane = 6;
-
-    CellState<R, FILTER, S> cs;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        cs.valid[s] = 0;
-        cs.tri[s] = 0;
-        cs.X[s] = cs.Y[s] = cs.Z[s] = R(0);
-        cs.W[s] = R(1);
-        cs.MX[s] = cs.MY[s] = cs.MZ[s] = cs.MW[s] = 0.f;
-        cs.r2[s] = R(0);
-    }
-    {
-        // CCInit vertex triples (ConvexCell.cl:299-306)
-        const uint32_t cube_tri[8] = {pack3(2, 5, 0), pack3(5, 3, 0), pack3(1, 5, 2), pack3(5, 1, 3),
-                                      pack3(4, 2, 0), pack3(4, 0, 3), pack3(2, 4, 1), pack3(4, 3, 1)};
-        uint32_t tr = 0;
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (lane == i) tr = cube_tri[i];
-        if (lane < 8) {
-            cs.tri[0] = tr;
-            vertex_from_planes<R, FILTER>(cvt<R>(planes[t0(tr)]), cvt<R>(planes[t1(tr)]), cvt<R>(planes[t2(tr)]),
-                                          cs.X[0], cs.Y[0], cs.Z[0], cs.W[0], cs.MX[0], cs.MY[0], cs.MZ[0],
-                                          cs.MW[0]);
-            const R ix = cs.X[0] / cs.W[0], iy = cs.Y[0] / cs.W[0], iz = cs.Z[0] / cs.W[0];
-            cs.r2[0] = ix * ix + iy * iy + iz * iz;
-        }
-        cs.valid[0] = 0xffull;
+    __device__ void set_vertex_lds(int s, uint32_t tr) {
+        set_vertex(s, tr, cvt<R>(planes[t0(tr)]), cvt<R>(planes[t1(tr)]), cvt<R>(planes[t2(tr)]));
     }
 
-    auto max_r2 = [&]() -> R {
+    __device__ void refresh_max_r2() {
         R m = R(0);
 #pragma unroll
         for (int s = 0; s < S; ++s)
-            if ((cs.valid[s] >> lane) & 1) m = cs.r2[s] > m ? cs.r2[s] : m;
-        return wave_max(m);
-    };
-    R maxr2 = max_r2();
+            if ((valid[s] >> lane) & 1) m = r2[s] > m ? r2[s] : m;
+        maxr2 = wave_max(m);
+    }
 
-    // ---- one clip by plane e; returns kNoClip, kClipped or a status --------
-    // e: the plane in the test precision (wave-uniform); the lane wl holds
-    // the same plane in the storage precision in eP (wl < 0: every lane does)
-    auto clip = [&](const Pl<R> &e, int wl, const Pl<RP> &eP) -> int {
+    // True when plane e removes every vertex with a margin: e.v above the
+    // certified FP32 error bound (mixed form) or above kSepTau times the sum
+    // of |terms| of e.v (FP64 and FP32 forms).
+    __device__ bool removes_all(const Pl<R> &e) const {
+        bool keep = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const bool v = (valid[s] >> lane) & 1;
+            const R sv = e.x * X[s] + e.y * Y[s] + e.z * Z[s] + e.w * W[s];
+            R m;
+            if (FILTER)
+                m = R(kFilterKappa * (fabsf(float(e.x)) * MX[s] + fabsf(float(e.y)) * MY[s] +
+                                      fabsf(float(e.z)) * MZ[s] + fabsf(float(e.w)) * MW[s]));
+            else
+                m = R(sizeof(R) == 8 ? 0x1.0p-33 : 0x1.0p-16) *
+                    (fabs(e.x * X[s]) + fabs(e.y * Y[s]) + fabs(e.z * Z[s]) + fabs(e.w * W[s]));
+            keep |= v && !(sv > m);
+        }
+        return ballot(keep) == 0;
+    }
+
+    // One clip by plane e (wave-uniform, test precision).  Lane wl holds the
+    // same plane in the storage precision in eP (wl < 0: every lane does).
+    // Returns kNoClip, kClipped or a failure status.
+    __device__ int clip(const Pl<R> &e, int wl, const Pl<RP> &eP) {
         uint64_t rem[S];
         int nr = 0, nvalid = 0;
         uint64_t flag = 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const bool v = (cs.valid[s] >> lane) & 1;
-            const R sv = e.x * cs.X[s] + e.y * cs.Y[s] + e.z * cs.Z[s] + e.w * cs.W[s];
+            const bool v = (valid[s] >> lane) & 1;
+            const R sv = e.x * X[s] + e.y * Y[s] + e.z * Z[s] + e.w * W[s];
             rem[s] = ballot(v && sv > R(0));
             if (FILTER) {
-                const float bound = kFilterKappa * (fabsf(float(e.x)) * cs.MX[s] + fabsf(float(e.y)) * cs.MY[s] +
-                                                    fabsf(float(e.z)) * cs.MZ[s] + fabsf(float(e.w)) * cs.MW[s]);
+                const float bound = kFilterKappa * (fabsf(float(e.x)) * MX[s] + fabsf(float(e.y)) * MY[s] +
+                                                    fabsf(float(e.z)) * MZ[s] + fabsf(float(e.w)) * MW[s]);
                 flag |= ballot(v && fabsf(float(sv)) <= bound);
             }
             nr += __popcll(rem[s]);
-            nvalid += __popcll(cs.valid[s]);
+            nvalid += __popcll(valid[s]);
         }
         if (FILTER && flag) return RPD_NEEDS_EXACT_PREDICATES;
         if (nr == 0) return kNoClip;
@@ -291,19 +246,18 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
         int carry = 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if ((rem[s] >> lane) & 1) rt[carry + mbcnt(rem[s])] = cs.tri[s];
+            if ((rem[s] >> lane) & 1) rt[carry + mbcnt(rem[s])] = tri[s];
             carry += __popcll(rem[s]);
         }
         wsync();
         // boundary edges: an edge u->v of a removed triangle whose reverse
         // v->u belongs to no removed triangle
         uint32_t bm[S];
-        int cnt_lane = 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             bm[s] = 0;
             if ((rem[s] >> lane) & 1) {
-                const uint32_t tr = cs.tri[s];
+                const uint32_t tr = tri[s];
                 const uint32_t a0 = t0(tr), a1 = t1(tr), a2 = t2(tr);
                 const uint32_t r01 = (a1 << 8) | a0, r12 = (a2 << 8) | a1, r20 = (a0 << 8) | a2;
                 bool h01 = false, h12 = false, h20 = false;
@@ -316,7 +270,6 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
                 }
                 bm[s] = (h01 ? 0u : 1u) | (h12 ? 0u : 2u) | (h20 ? 0u : 4u);
             }
-            cnt_lane += __popc(bm[s]);
         }
         // slot-major exclusive offsets of the new vertices
         int nb = 0;
@@ -324,28 +277,33 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int c = __popc(bm[s]);
-            const int o = wave_excl_scan(c);
-            obase[s] = nb + o;
-            nb += uni(rlane(o + c, 63));
+            const int incl = wave_incl_scan_op(c, 0, [](int x, int y) { return x + y; });
+            obase[s] = nb + incl - c;
+            nb += uni(rlane(incl, 63));
         }
         int nkeep = 0;
 #pragma unroll
-        for (int s = 0; s < S; ++s) nkeep += __popcll(cs.valid[s] & ~rem[s]);
+        for (int s = 0; s < S; ++s) nkeep += __popcll(valid[s] & ~rem[s]);
         if (nkeep + nb > V) return RPD_VERTEX_OVERFLOW;
         // stage new triples (plane, a, b) and the boundary successor map
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             if (bm[s]) {
-                const uint32_t tr = cs.tri[s];
-                const uint32_t v3[3] = {t0(tr), t1(tr), t2(tr)};
+                const uint32_t tr = tri[s];
+                const uint32_t v0 = t0(tr), v1 = t1(tr), v2 = t2(tr);
                 int o = obase[s];
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    if (bm[s] & (1u << k)) {
-                        const uint32_t ea = v3[k], eb = v3[(k + 1) % 3];
-                        nvs[o++] = pack3(uint32_t(pid), ea, eb);
-                        nxt[ea] = uint8_t(eb);
-                    }
+                if (bm[s] & 1u) {
+                    nvs[o++] = pack3(uint32_t(pid), v0, v1);
+                    nxt[v0] = uint8_t(v1);
+                }
+                if (bm[s] & 2u) {
+                    nvs[o++] = pack3(uint32_t(pid), v1, v2);
+                    nxt[v1] = uint8_t(v2);
+                }
+                if (bm[s] & 4u) {
+                    nvs[o++] = pack3(uint32_t(pid), v2, v0);
+                    nxt[v2] = uint8_t(v0);
+                }
             }
         }
         wsync();
@@ -355,11 +313,11 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             if (bm[s]) {
-                const uint32_t tr = cs.tri[s];
-                const uint32_t v3[3] = {t0(tr), t1(tr), t2(tr)};
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    if (bm[s] & (1u << k)) bad |= nxt[v3[k]] != v3[(k + 1) % 3];
+                const uint32_t tr = tri[s];
+                const uint32_t v0 = t0(tr), v1 = t1(tr), v2 = t2(tr);
+                if (bm[s] & 1u) bad |= nxt[v0] != v1;
+                if (bm[s] & 2u) bad |= nxt[v1] != v2;
+                if (bm[s] & 4u) bad |= nxt[v2] != v0;
             }
         }
         bool inconsistent = ballot(bad) != 0;
@@ -377,7 +335,7 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             if (bm[s]) {
-                const uint32_t tr = cs.tri[s];
+                const uint32_t tr = tri[s];
                 if (bm[s] & 1u) nxt[t0(tr)] = 0xff;
                 if (bm[s] & 2u) nxt[t1(tr)] = 0xff;
                 if (bm[s] & 4u) nxt[t2(tr)] = 0xff;
@@ -386,120 +344,147 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
         if (inconsistent) return RPD_INCONSISTENT_BOUNDARY;
         // new vertices fill the lowest free slots
         carry = 0;
-        uint64_t newm[S];
+        uint64_t ill = 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            cs.valid[s] &= ~rem[s];
-            const uint64_t fr = ~cs.valid[s];
+            valid[s] &= ~rem[s];
+            const uint64_t fr = ~valid[s];
             const int q = carry + mbcnt(fr);
             const bool take = ((fr >> lane) & 1) && q < nb;
-            newm[s] = ballot(take);
+            const uint64_t newm = ballot(take);
             if (take) {
                 const uint32_t tr = nvs[q];
-                cs.tri[s] = tr;
-                const Pl<R> pa = cvt<R>(planes[t1(tr)]), pb = cvt<R>(planes[t2(tr)]);
-                vertex_from_planes<R, FILTER>(e, pa, pb, cs.X[s], cs.Y[s], cs.Z[s], cs.W[s], cs.MX[s], cs.MY[s],
-                                              cs.MZ[s], cs.MW[s]);
-                const R ix = cs.X[s] / cs.W[s], iy = cs.Y[s] / cs.W[s], iz = cs.Z[s] / cs.W[s];
-                cs.r2[s] = ix * ix + iy * iy + iz * iz;
+                set_vertex(s, tr, e, cvt<R>(planes[t1(tr)]), cvt<R>(planes[t2(tr)]));
             }
+            if (FILTER) ill |= ballot(take && fabsf(float(W[s])) < kFilterMinW * MW[s]);
             carry += __popcll(fr);
-            cs.valid[s] |= newm[s];
+            valid[s] |= newm;
         }
         nb_plane = pid + 1;
-        if (FILTER) {
-            uint64_t ill = 0;
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-                ill |= ballot(((newm[s] >> lane) & 1) && fabsf(float(cs.W[s])) < kFilterMinW * cs.MW[s]);
-            if (ill) return RPD_NEEDS_EXACT_PREDICATES;
-        }
-        maxr2 = max_r2();
+        if (FILTER && ill) return RPD_NEEDS_EXACT_PREDICATES;
+        refresh_max_r2();
         return kClipped;
-    };
+    }
+};
 
-    // ---- fused integration (CCExportBaryVolume) into FP64 accumulators -----
-    // vertex coordinates in the geometry precision RP: the registers when
-    // R == RP, else recomputed from the stored RP planes
-    auto vertex_rp = [&](int s, RP &Ax, RP &Ay, RP &Az) {
-        if (sizeof(R) == sizeof(RP)) {
-            Ax = RP(cs.X[s] / cs.W[s]);
-            Ay = RP(cs.Y[s] / cs.W[s]);
-            Az = RP(cs.Z[s] / cs.W[s]);
-        } else {
-            const uint32_t tr = cs.tri[s];
-            RP X, Y, Z, W;
-            float d0, d1, d2, d3;
-            vertex_from_planes<RP, false>(planes[t0(tr)], planes[t1(tr)], planes[t2(tr)], X, Y, Z, W, d0, d1, d2,
-                                          d3);
-            Ax = X / W;
-            Ay = Y / W;
-            Az = Z / W;
+// LDS of one wave: planes[P] | rt[64S] | nvs[64S] | nxt[256] | (finish
+// kernel only) tl[tl_cap] | tu[T] | cone planes cpl[64] | go_out flags[16]
+template <typename R, typename RP, bool FILTER, int S>
+__device__ __forceinline__ void bind_lds(char *L, int P, int tl_cap, int T, WaveCell<R, RP, FILTER, S> &c,
+                                         int **tl = nullptr, int **tu = nullptr, Pl<RP> **cpl = nullptr,
+                                         int **goo = nullptr) {
+    c.planes = reinterpret_cast<Pl<RP> *>(L);
+    size_t off = (size_t(P) * sizeof(Pl<RP>) + 15) & ~size_t(15);
+    c.rt = reinterpret_cast<uint32_t *>(L + off);
+    off += 4 * 64 * S;
+    c.nvs = reinterpret_cast<uint32_t *>(L + off);
+    off += 4 * 64 * S;
+    c.nxt = reinterpret_cast<uint8_t *>(L + off);
+    off += 256;
+    if (tl) *tl = reinterpret_cast<int *>(L + off);
+    off += 4 * size_t(tl_cap);
+    if (tu) *tu = reinterpret_cast<int *>(L + off);
+    off += 4 * size_t(T);
+    off = (off + 15) & ~size_t(15);
+    if (cpl) *cpl = reinterpret_cast<Pl<RP> *>(L + off);
+    off += 64 * sizeof(Pl<RP>);
+    if (goo) *goo = reinterpret_cast<int *>(L + off);
+}
+
+// LDS bytes of one wave of the finish kernel
+__host__ __device__ constexpr size_t finish_wave_lds(int P, int S, int tl_cap, int T, size_t pl) {
+    return ((((((P * pl + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) + 256 + 4 * size_t(tl_cap) + 4 * size_t(T)) +
+              15) & ~size_t(15)) + 64 * pl + 64 + 15) & ~size_t(15);
+}
+
+// Record of a finished cell in the queue: header (sid, nb_plane, nvalid,
+// kused), planes[P] in the storage precision, vertex triples[64 * S].
+struct RecHeader {
+    int sid, nb_plane, nvalid, kused;
+};
+
+// ---------------------------------------------------------------------------
+// Kernel A: the bisector loop (ConvexCell.cl:850-917)
+// ---------------------------------------------------------------------------
+template <typename R, typename RP, bool FILTER, int S>
+__global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+    const int lane = lane_id();
+    const int wv = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int64_t t = int64_t(blockIdx.x) * wpb + wv;
+    if (t >= a.n_ids) return;  // wave-uniform
+
+    WaveCell<R, RP, FILTER, S> c;
+    bind_lds(lds_raw + size_t(wv) * a.lds_stride, a.P, 0, 0, c);
+    c.P = a.P;
+    c.V = a.V;
+    c.lane = lane;
+
+    const int sid = a.ids ? a.ids[t] : int(t);
+    const double4 S0 = a.seeds[sid];
+    const int orig = a.perm[sid];
+    const int K = a.K, P = a.P;
+
+    auto finish = [&](int st, double m0, double m1, double m2, double m3, int ku) {
+        if (lane == 0) {
+            a.out[orig] = make_double4(m0, m1, m2, m3);
+            a.status[orig] = st;
+            if (a.kused) a.kused[orig] = ku;
         }
     };
-    double acc_m = 0, acc_x = 0, acc_y = 0, acc_z = 0;
-    const RP vol_eps2 = sizeof(RP) == 8 ? RP(1e-12) : RP(1e-6);  // voronoi.cpp:919-922
-    auto integrate = [&](double factor) -> bool {
-        RP sw = 0, sx = 0, sy = 0, sz = 0;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (!((cs.valid[s] >> lane) & 1)) continue;
-            const uint32_t tr = cs.tri[s];
-            RP Ax, Ay, Az;
-            vertex_rp(s, Ax, Ay, Az);
-            const RP ACx = -Ax, ACy = -Ay, ACz = -Az;  // seed at the origin
-            Pl<RP> pl[3] = {planes[t0(tr)], planes[t1(tr)], planes[t2(tr)]};
-            RP opx[3], opy[3], opz[3], olx[3], oly[3], olz[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const RP nn = pl[i].x * pl[i].x + pl[i].y * pl[i].y + pl[i].z * pl[i].z;
-                const RP f = (ACx * pl[i].x + ACy * pl[i].y + ACz * pl[i].z) / nn;
-                opx[i] = ACx - f * pl[i].x;
-                opy[i] = ACy - f * pl[i].y;
-                opz[i] = ACz - f * pl[i].z;
-            }
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const Pl<RP> &p = pl[i], &q = pl[(i + 1) % 3];
-                const RP cx = p.y * q.z - p.z * q.y, cy = p.z * q.x - p.x * q.z, cz = p.x * q.y - p.y * q.x;
-                const RP c2 = cx * cx + cy * cy + cz * cz;
-                const RP pp = p.x * p.x + p.y * p.y + p.z * p.z, qq = q.x * q.x + q.y * q.y + q.z * q.z;
-                if (c2 < vol_eps2 * pp * qq) {
-                    olx[i] = oly[i] = olz[i] = RP(0);
-                } else {
-                    const RP f = (ACx * cx + ACy * cy + ACz * cz) / c2;
-                    olx[i] = f * cx;
-                    oly[i] = f * cy;
-                    olz[i] = f * cz;
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-#pragma unroll
-                for (int st = 0; st < 2; ++st) {
-                    const int j = (i + 3 - st) % 3;
-                    // det3 of rows AC, onPlane_i, onLine_j (ConvexCell.cl:72-77)
-                    const RP d = ACx * (opy[i] * olz[j] - opz[i] * oly[j]) - opx[i] * (ACy * olz[j] - ACz * oly[j]) +
-                                 olx[j] * (ACy * opz[i] - ACz * opy[i]);
-                    const RP w = st ? d : -d;
-                    sw += w;
-                    sx += w * (Ax + RP(0.25) * (ACx + opx[i] + olx[j]));
-                    sy += w * (Ay + RP(0.25) * (ACy + opy[i] + oly[j]));
-                    sz += w * (Az + RP(0.25) * (ACz + opz[i] + olz[j]));
-                }
-            }
-        }
-        const double f = factor / 6.0;
-        acc_m += f * wave_sum(double(sw));
-        acc_x += f * wave_sum(double(sx));
-        acc_y += f * wave_sum(double(sy));
-        acc_z += f * wave_sum(double(sz));
-        return true;
-    };
-
     auto fail = [&](int st, int ku) { finish(st, 0.0, 0.0, 0.0, 0.0, ku); };
 
-    // ---- bisector loop (ConvexCell.cl:882-913) --------------------------------
+    if (S0.w <= RPD_NO_CELL_LIMITS) {  // border seed: copied (ConvexCell.cl:873-876)
+        finish(RPD_SUCCESS, S0.x, S0.y, S0.z, S0.w, 0);
+        return;
+    }
+    for (int i = lane; i < 256; i += 64) c.nxt[i] = 0xff;
+
+    // Domain cull: a seed whose voxel is d >= 2 voxels (Chebyshev) from every
+    // voxel that holds triangles or lies inside has no point of the domain
+    // within (d - 1) * vs; once the cell's circumradius about the seed is
+    // below that, its restriction to the domain is empty (volume 0, as the
+    // reference's apex-outside / no-triangle case, ConvexCell.cl:1067-1079).
+    double cull_r2 = 0;
+    if (a.g > 0 && a.dist) {
+        const int g = a.g;
+        auto vox = [&](double x) { return int(fmin(fmax(x / a.vs, 0.0), double(g - 1))); };
+        const int d = a.dist[vox(S0.x) + vox(S0.y) * g + vox(S0.z) * g * g];
+        if (d >= 2) cull_r2 = double(d - 1) * a.vs * double(d - 1) * a.vs;
+    }
+    const double wgap = S0.w - a.wmin;  // >= 0 except for the lightest seeds
+
+    // ---- CCInit: bounding cube in the local frame ---------------------------
+    const double eps = a.g > 0 ? 0.1 : 0.0;  // CUBE_EPSILON, voronoi.cpp:929
+    if (lane < 6) {
+        const int ax = lane >> 1;
+        const double sc = ax == 0 ? S0.x : (ax == 1 ? S0.y : S0.z);
+        const double sgn = (lane & 1) ? -1.0 : 1.0;
+        const double d = (lane & 1) ? (1000.0 + eps) - sc : sc + eps;
+        c.planes[lane] = Pl<RP>{ax == 0 ? RP(sgn) : RP(0), ax == 1 ? RP(sgn) : RP(0), ax == 2 ? RP(sgn) : RP(0),
+                                RP(d)};
+    }
+    wsync();
+    c.nb_plane = 6;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        c.valid[s] = 0;
+        c.tri[s] = 0;
+        c.X[s] = c.Y[s] = c.Z[s] = R(0);
+        c.W[s] = R(1);
+        c.r2[s] = R(0);
+        c.MX[s] = c.MY[s] = c.MZ[s] = c.MW[s] = 0.f;
+    }
+    if (lane < 8) {
+        const uint32_t sh = 8 * uint32_t(lane);
+        c.set_vertex_lds(0, pack3(uint32_t(kCube0 >> sh) & 0xff, uint32_t(kCube1 >> sh) & 0xff,
+                                  uint32_t(kCube2 >> sh) & 0xff));
+    }
+    c.valid[0] = 0xffull;
+    c.refresh_max_r2();
+
+    // ---- neighbour loop -----------------------------------------------------
     // The 64 neighbours of a chunk are loaded one per lane with their plane,
     // distance |b| and plane offset t (signed distance of the bisector from
     // the seed).  A plane with t >= R, R the distance from the seed to the
@@ -519,8 +504,10 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
     int cnb = -1;
     int loaded = -1;
     const double rmarg = sizeof(R) == 8 ? 1.0 + 1e-9 : 1.0 + 2e-4;  // FP32 R may be low by ~1e-4
-    double Rcur = sqrt(double(maxr2));
+    double Rcur = sqrt(double(c.maxr2));
     auto stop_at = [&](double dl, double Rm) -> bool {
+        // CCIsSecurityRadiusReached (reference stop, weights ignored) or the
+        // exact power stop d (d - 2R) >= w_s - w_min
         const double Rs = Rm * rmarg;
         if (a.security_mode == 1 || wgap <= 0) return dl * dl > 4.0 * Rs * Rs;
         return dl * (dl - 2.0 * Rs) >= wgap;
@@ -539,12 +526,13 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
                 const double bx = B.x - S0.x, by = B.y - S0.y, bz = B.z - S0.z;
                 const double b2 = bx * bx + by * by + bz * bz;
                 const double bl = sqrt(b2);
+                const double ib = 1.0 / bl;
                 double d;
                 if (B.w <= RPD_NO_CELL_LIMITS)
                     d = bl;  // plane through B (ConvexCell.cl:325)
                 else
-                    d = (b2 + (B.w - S0.w)) / (2.0 * bl);
-                cpl = Pl<RP>{RP(-bx / bl), RP(-by / bl), RP(-bz / bl), RP(d)};
+                    d = (b2 + (B.w - S0.w)) * (0.5 * ib);
+                cpl = Pl<RP>{RP(-bx * ib), RP(-by * ib), RP(-bz * ib), RP(d)};
                 cdl = bl;
                 ct = d;
             }
@@ -553,7 +541,7 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
         }
         const int l0 = i & 63;
         const bool in = lane >= l0 && cnb != -2;
-        if (nb_plane >= P) {
+        if (c.nb_plane >= P) {
             // the next neighbour's plane cannot be stored
             const uint64_t me = ballot(in && cnb == -1), mv = ballot(in && cnb >= 0);
             if (mv && (!me || lowest_bit(mv) < lowest_bit(me))) {
@@ -590,16 +578,15 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
             continue;
         }
         // clip by the plane of neighbour base + f_clip
-        const Pl<R> e = rlane_pl(cplr, f_clip);
-        const int res = clip(e, f_clip, cpl);
+        const int res = c.clip(rlane_pl(cplr, f_clip), f_clip, cpl);
         kused = base + f_clip + 1;
         if (res >= 0) {
             fail(res, kused);
             return;
         }
         if (res == kClipped) {
-            Rcur = sqrt(double(maxr2));
-            if (double(maxr2) * 1.001 < cull_r2) {
+            Rcur = sqrt(double(c.maxr2));
+            if (double(c.maxr2) * 1.001 < cull_r2) {
                 finish(RPD_SUCCESS, 0.0, 0.0, 0.0, 0.0, kused);
                 return;
             }
@@ -610,12 +597,194 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
         }
         i = base + f_clip + 1;
     }
+    // a list that holds every other seed leaves nothing to test: the cell is
+    // exact once the list is used up (the reference reads past the seeds there)
+    if (!reached && !complete && int64_t(K) >= a.n_seeds - 1) complete = true;
     if (!reached && !complete) {
         fail(RPD_SECURITY_RADIUS_NOT_REACHED, kused);
         return;
     }
 
-    // ---- restriction to the domain grid (ConvexCell.cl:918-1116) -------------
+    // ---- queue the finished cell for restriction + integration -------------
+    int slot = 0;
+    if (lane == 0) slot = atomicAdd(a.qcount, 1);
+    slot = rlane(slot, 0);
+    char *rec = a.rec + size_t(slot) * size_t(a.rec_stride);
+    Pl<RP> *rpl = reinterpret_cast<Pl<RP> *>(rec + sizeof(RecHeader));
+    uint32_t *rtr = reinterpret_cast<uint32_t *>(rec + sizeof(RecHeader) + size_t(P) * sizeof(Pl<RP>));
+    int nv = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) nv += __popcll(c.valid[s]);
+    if (lane == 0) *reinterpret_cast<RecHeader *>(rec) = RecHeader{sid, c.nb_plane, nv, kused};
+    for (int p = lane; p < c.nb_plane; p += 64) rpl[p] = c.planes[p];
+    int carry = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if ((c.valid[s] >> lane) & 1) rtr[carry + mbcnt(c.valid[s])] = c.tri[s];
+        carry += __popcll(c.valid[s]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel B: restriction to the domain (ConvexCell.cl:918-1116) and the
+// integration (CCExportBaryVolume, ConvexCell.cl:577-619) of queued cells.
+//
+// W waves share one cell (W = 1 for large queues, up to 8 for the short
+// re-queue passes whose run time is set by their heaviest cell): every wave
+// rebuilds the cell and its triangle list, wave w takes the triangles
+// k = w, w + W, ...  Each triangle starts from the same backed-up cell, so its
+// contribution does not depend on which wave computes it; contributions go to
+// LDS by triangle index and wave 0 sums them in one fixed order, so the result
+// of a cell is the same bit for bit for every W and every batch it is in.
+//
+// Cone planes are built lane-parallel, 16 triangles x 4 planes per batch.  A
+// triangle one of whose four planes removes every vertex of the cell with a
+// margin contributes nothing: the reference clips it and ends with
+// empty_cell at that plane at the latest (vertices created by earlier cone
+// planes lie in the convex hull of the removed ones).  The shortcut is taken
+// only when the caps guarantee that those earlier clips could not overflow
+// either (at most 3 more planes, at most 2 more vertices per clip), so the
+// outcome is the reference's.
+// ---------------------------------------------------------------------------
+template <typename R, typename RP, bool FILTER, int S>
+__global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, int W) {
+    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+    const int lane = lane_id();
+    const int wv = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int cpb = wpb / W;  // cells per block; W > 1 implies cpb == 1
+    const int ci = wv / W, sub = wv - ci * W;
+    const int slot = int(blockIdx.x) * cpb + ci;
+    if (slot >= nq) return;  // uniform over the waves of a cell
+
+    char *cell_l = lds_raw + size_t(ci) * a.cell_stride;
+    double4 *contrib = reinterpret_cast<double4 *>(cell_l);              // [T]
+    int *fails = reinterpret_cast<int *>(cell_l + 32 * size_t(a.T));  // [W][2]
+    WaveCell<R, RP, FILTER, S> c;
+    int *tl = nullptr, *tu = nullptr, *goo = nullptr;
+    Pl<RP> *cpl = nullptr;
+    bind_lds(lds_raw + size_t(cpb) * a.cell_stride + size_t(wv) * a.lds_stride, a.P, a.tl_cap, a.T, c, &tl, &tu,
+             &cpl, &goo);
+    c.P = a.P;
+    c.V = a.V;
+    c.lane = lane;
+    const int P = a.P;
+
+    const char *rec = a.rec + size_t(slot) * size_t(a.rec_stride);
+    const RecHeader hd = *reinterpret_cast<const RecHeader *>(rec);
+    const int sid = uni(hd.sid);
+    const int kused = uni(hd.kused);
+    const int nvalid = uni(hd.nvalid);
+    c.nb_plane = uni(hd.nb_plane);
+    const Pl<RP> *rpl = reinterpret_cast<const Pl<RP> *>(rec + sizeof(RecHeader));
+    const uint32_t *rtr = reinterpret_cast<const uint32_t *>(rec + sizeof(RecHeader) + size_t(P) * sizeof(Pl<RP>));
+    for (int p = lane; p < c.nb_plane; p += 64) c.planes[p] = rpl[p];
+    for (int i = lane; i < 256; i += 64) c.nxt[i] = 0xff;
+    wsync();
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int j = s * 64 + lane;
+        c.tri[s] = 0;
+        c.X[s] = c.Y[s] = c.Z[s] = R(0);
+        c.W[s] = R(1);
+        c.r2[s] = R(0);
+        c.MX[s] = c.MY[s] = c.MZ[s] = c.MW[s] = 0.f;
+        if (j < nvalid) c.set_vertex_lds(s, rtr[j]);
+        c.valid[s] = ballot(j < nvalid);
+    }
+    c.refresh_max_r2();
+
+    const double4 S0 = a.seeds[sid];
+    const int orig = a.perm[sid];
+    auto finish = [&](int st, double m0, double m1, double m2, double m3) {
+        if (sub == 0 && lane == 0) {
+            a.out[orig] = make_double4(m0, m1, m2, m3);
+            a.status[orig] = st;
+            if (a.kused) a.kused[orig] = kused;
+        }
+    };
+    auto fail = [&](int st) { finish(st, 0.0, 0.0, 0.0, 0.0); };
+
+    // vertex coordinates in the geometry precision RP: the registers when
+    // R == RP, else recomputed from the stored RP planes
+    auto vertex_rp = [&](int s, RP &Ax, RP &Ay, RP &Az) {
+        if (sizeof(R) == sizeof(RP)) {
+            const RP iw = RP(1) / RP(c.W[s]);
+            Ax = RP(c.X[s]) * iw;
+            Ay = RP(c.Y[s]) * iw;
+            Az = RP(c.Z[s]) * iw;
+        } else {
+            const uint32_t tr = c.tri[s];
+            RP X, Y, Z, W;
+            float d0, d1, d2, d3;
+            vertex_from_planes<RP, false>(c.planes[t0(tr)], c.planes[t1(tr)], c.planes[t2(tr)], X, Y, Z, W, d0, d1,
+                                          d2, d3);
+            const RP iw = RP(1) / W;
+            Ax = X * iw;
+            Ay = Y * iw;
+            Az = Z * iw;
+        }
+    };
+
+    // ---- integration (CCExportBaryVolume): factor/6 * (volume, moments) of
+    // the current cell in the local frame, FP64 wave sums --------------------
+    const RP vol_eps2 = sizeof(RP) == 8 ? RP(1e-12) : RP(1e-6);  // voronoi.cpp:919-922
+    auto integrate = [&](double factor) -> double4 {
+        RP sw = 0, sx = 0, sy = 0, sz = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (!((c.valid[s] >> lane) & 1)) continue;
+            const uint32_t tr = c.tri[s];
+            RP Ax, Ay, Az;
+            vertex_rp(s, Ax, Ay, Az);
+            const RP ACx = -Ax, ACy = -Ay, ACz = -Az;  // seed at the origin
+            const Pl<RP> pl[3] = {c.planes[t0(tr)], c.planes[t1(tr)], c.planes[t2(tr)]};
+            RP opx[3], opy[3], opz[3], olx[3], oly[3], olz[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const RP nn = pl[k].x * pl[k].x + pl[k].y * pl[k].y + pl[k].z * pl[k].z;
+                const RP f = (ACx * pl[k].x + ACy * pl[k].y + ACz * pl[k].z) / nn;
+                opx[k] = ACx - f * pl[k].x;
+                opy[k] = ACy - f * pl[k].y;
+                opz[k] = ACz - f * pl[k].z;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const Pl<RP> &p = pl[k], &q = pl[(k + 1) % 3];
+                const RP cx = p.y * q.z - p.z * q.y, cy = p.z * q.x - p.x * q.z, cz = p.x * q.y - p.y * q.x;
+                const RP c2 = cx * cx + cy * cy + cz * cz;
+                const RP pp = p.x * p.x + p.y * p.y + p.z * p.z, qq = q.x * q.x + q.y * q.y + q.z * q.z;
+                if (c2 < vol_eps2 * pp * qq) {
+                    olx[k] = oly[k] = olz[k] = RP(0);
+                } else {
+                    const RP f = (ACx * cx + ACy * cy + ACz * cz) / c2;
+                    olx[k] = f * cx;
+                    oly[k] = f * cy;
+                    olz[k] = f * cz;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                for (int st = 0; st < 2; ++st) {
+                    const int j = (k + 3 - st) % 3;
+                    // det3 of rows AC, onPlane_k, onLine_j (ConvexCell.cl:72-77)
+                    const RP d = ACx * (opy[k] * olz[j] - opz[k] * oly[j]) - opx[k] * (ACy * olz[j] - ACz * oly[j]) +
+                                 olx[j] * (ACy * opz[k] - ACz * opy[k]);
+                    const RP w = st ? d : -d;
+                    sw += w;
+                    sx += w * (Ax + RP(0.25) * (ACx + opx[k] + olx[j]));
+                    sy += w * (Ay + RP(0.25) * (ACy + opy[k] + oly[j]));
+                    sz += w * (Az + RP(0.25) * (ACz + opz[k] + olz[j]));
+                }
+            }
+        }
+        const double f = factor / 6.0;
+        return make_double4(f * wave_sum(double(sx)), f * wave_sum(double(sy)), f * wave_sum(double(sz)),
+                            f * wave_sum(double(sw)));
+    };
+
+    // ---- restriction to the domain grid --------------------------------------
     bool add_volume = true;
     int ntri = 0;
     double P0x = 0, P0y = 0, P0z = 0;  // apex, global frame
@@ -625,7 +794,7 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
         double bmn[3] = {1000.0, 1000.0, 1000.0}, bmx[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if (!((cs.valid[s] >> lane) & 1)) continue;
+            if (!((c.valid[s] >> lane) & 1)) continue;
             RP lx, ly, lz;
             vertex_rp(s, lx, ly, lz);
             const double gx = S0.x + double(lx), gy = S0.y + double(ly), gz = S0.z + double(lz);
@@ -686,7 +855,7 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
             ncol += tot;
         }
         if (overflow) {
-            fail(RPD_TRIANGLE_OVERFLOW, kused);
+            fail(RPD_TRIANGLE_OVERFLOW);
             return;
         }
         wsync();
@@ -694,18 +863,18 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
         // first occurrence of every triangle id
         int nu = 0;
         for (int ib = 0; ib < ncol; ib += 64) {
-            const int i = ib + lane;
-            const int val = i < ncol ? tl[i] : -1;
+            const int ii = ib + lane;
+            const int val = ii < ncol ? tl[ii] : -1;
             bool dup = false;
             const int jend = min(ncol, ib + 64);
-            for (int j = 0; j < jend; ++j) dup |= (j < i) && (tl[j] == val);
-            const bool keep = i < ncol && !dup;
+            for (int j = 0; j < jend; ++j) dup |= (j < ii) && (tl[j] == val);
+            const bool keep = ii < ncol && !dup;
             const uint64_t m = ballot(keep);
             if (keep && nu + mbcnt(m) < a.T) tu[nu + mbcnt(m)] = val;
             nu += __popcll(m);
         }
         if (nu > a.T) {
-            fail(RPD_TRIANGLE_OVERFLOW, kused);
+            fail(RPD_TRIANGLE_OVERFLOW);
             return;
         }
         ntri = nu;
@@ -719,9 +888,9 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
         if (a.apex_mode == 1 && first_c >= 0) {
             int x, y, z;
             vox_xyz(first_c, x, y, z);
-            P0x = (x + 0.5) * vs;
-            P0y = (y + 0.5) * vs;
-            P0z = (z + 0.5) * vs;
+            P0x = (x + kApexFrac[0]) * vs;
+            P0y = (y + kApexFrac[1]) * vs;
+            P0z = (z + kApexFrac[2]) * vs;
             in_domain = c_flag;
         } else if (first_ref >= 0) {
             int x, y, z;
@@ -778,106 +947,170 @@ __global__ __launch_bounds__(256) void clip_kernel(ClipArgs a) {
                 setc(vmx[0] + 1, vmn[1], vmn[2]);
             }
             if (in_domain == 1) {
-                fail(RPD_FIND_ANOTHER_BEGINNING_VERTEX, kused);
+                fail(RPD_FIND_ANOTHER_BEGINNING_VERTEX);
                 return;
             }
         }
         add_volume = in_domain == 2;
     }
-    if (add_volume) {
-        if (!integrate(1.0)) {
-            fail(RPD_NEEDS_EXACT_PREDICATES, kused);
-            return;
-        }
-    }
+    double4 cadd = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (add_volume && sub == 0) cadd = integrate(1.0);
+
+    int fail_k = 0x7fffffff, fail_st = 0;
     if (ntri > 0) {
         // backup (ConvexCell.cl:692-715): the vertex registers and the plane count
-        CellState<R, FILTER, S> bk = cs;
-        const int bk_np = nb_plane;
-        const R bk_r2 = maxr2;
+        const WaveCell<R, RP, FILTER, S> bk = c;
+        const bool may_skip = c.nb_plane + 4 <= P && nvalid + 6 <= a.V;
         const double lx0 = P0x - S0.x, ly0 = P0y - S0.y, lz0 = P0z - S0.z;
-        for (int k = 0; k < ntri; ++k) {
-            const int tid = tu[k];
-            const int i1 = a.tris[3 * tid], i2 = a.tris[3 * tid + 1], i3 = a.tris[3 * tid + 2];
-            const double4 G1 = a.gpts[i1];
-            double4 G2 = a.gpts[i2], G3 = a.gpts[i3];
-            // orientation test in the global frame, as the reference
-            const double ux = G1.x - P0x, uy = G1.y - P0y, uz = G1.z - P0z;
-            const double vx = G2.x - P0x, vy = G2.y - P0y, vz = G2.z - P0z;
-            const double wx = G3.x - P0x, wy = G3.y - P0y, wz = G3.z - P0z;
-            const double det = ux * (vy * wz - vz * wy) - vx * (uy * wz - uz * wy) + wx * (uy * vz - uz * vy);
-            const bool go_out = det > 0.0;
-            if (!go_out) {
-                const double4 tmp = G2;
-                G2 = G3;
-                G3 = tmp;
-            }
-            // cone planes (A,B,C), (A,D,B), (A,C,D), (C,D,B) with A = apex
-            const double Ax = lx0, Ay = ly0, Az = lz0;
-            const double Bx = G1.x - S0.x, By = G1.y - S0.y, Bz = G1.z - S0.z;
-            const double Cx = G2.x - S0.x, Cy = G2.y - S0.y, Cz = G2.z - S0.z;
-            const double Dx = G3.x - S0.x, Dy = G3.y - S0.y, Dz = G3.z - S0.z;
-            const double F[4][9] = {{Ax, Ay, Az, Bx, By, Bz, Cx, Cy, Cz},
-                                    {Ax, Ay, Az, Dx, Dy, Dz, Bx, By, Bz},
-                                    {Ax, Ay, Az, Cx, Cy, Cz, Dx, Dy, Dz},
-                                    {Cx, Cy, Cz, Dx, Dy, Dz, Bx, By, Bz}};
-            int res = kNoClip;
-            for (int f = 0; f < 4; ++f) {
-                if (nb_plane >= P) {
-                    res = RPD_PLANE_OVERFLOW;
-                    break;
+        for (int jb = sub; jb < ntri && fail_k == 0x7fffffff; jb += 16 * W) {
+            // cone planes (A,B,C), (A,D,B), (A,C,D), (C,D,B), A = apex
+            // (CCClipTetFromPointsInfinite): lane = 4 * triangle + plane
+            {
+                const int i = lane >> 2, f = lane & 3;
+                const int k = jb + W * i;
+                Pl<RP> pl{RP(0), RP(0), RP(0), RP(0)};
+                if (k < ntri) {
+                    const int tid = tu[k];
+                    const double4 G1 = a.gpts[a.tris[3 * tid]];
+                    double4 G2 = a.gpts[a.tris[3 * tid + 1]], G3 = a.gpts[a.tris[3 * tid + 2]];
+                    // orientation test in the global frame, as the reference
+                    const double ux = G1.x - P0x, uy = G1.y - P0y, uz = G1.z - P0z;
+                    const double vx = G2.x - P0x, vy = G2.y - P0y, vz = G2.z - P0z;
+                    const double wx = G3.x - P0x, wy = G3.y - P0y, wz = G3.z - P0z;
+                    const double det =
+                        ux * (vy * wz - vz * wy) - vx * (uy * wz - uz * wy) + wx * (uy * vz - uz * vy);
+                    const bool go_out = det > 0.0;
+                    if (!go_out) {
+                        const double4 tmp = G2;
+                        G2 = G3;
+                        G3 = tmp;
+                    }
+                    if (f == 0) goo[i] = go_out ? 1 : 0;
+                    // the three points of plane f in the local frame
+                    const double Bx = G1.x - S0.x, By = G1.y - S0.y, Bz = G1.z - S0.z;
+                    const double Cx = G2.x - S0.x, Cy = G2.y - S0.y, Cz = G2.z - S0.z;
+                    const double Dx = G3.x - S0.x, Dy = G3.y - S0.y, Dz = G3.z - S0.z;
+                    double px, py, pz, qx, qy, qz, rx, ry, rz;
+                    if (f == 3) {
+                        px = Cx, py = Cy, pz = Cz;
+                        qx = Dx, qy = Dy, qz = Dz;
+                        rx = Bx, ry = By, rz = Bz;
+                    } else {
+                        px = lx0, py = ly0, pz = lz0;
+                        if (f == 0) {
+                            qx = Bx, qy = By, qz = Bz;
+                            rx = Cx, ry = Cy, rz = Cz;
+                        } else if (f == 1) {
+                            qx = Dx, qy = Dy, qz = Dz;
+                            rx = Bx, ry = By, rz = Bz;
+                        } else {
+                            qx = Cx, qy = Cy, qz = Cz;
+                            rx = Dx, ry = Dy, rz = Dz;
+                        }
+                    }
+                    const double e1x = qx - px, e1y = qy - py, e1z = qz - pz;
+                    const double e2x = rx - px, e2y = ry - py, e2z = rz - pz;
+                    double nx_ = e1y * e2z - e1z * e2y, ny_ = e1z * e2x - e1x * e2z, nz_ = e1x * e2y - e1y * e2x;
+                    double nd = -(nx_ * px + ny_ * py + nz_ * pz);
+                    const double nl = sqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
+                    if (nl > 0) {
+                        const double il = 1.0 / nl;
+                        nx_ *= il;
+                        ny_ *= il;
+                        nz_ *= il;
+                        nd *= il;
+                    }
+                    pl = Pl<RP>{RP(nx_), RP(ny_), RP(nz_), RP(nd)};
                 }
-                const double e1x = F[f][3] - F[f][0], e1y = F[f][4] - F[f][1], e1z = F[f][5] - F[f][2];
-                const double e2x = F[f][6] - F[f][0], e2y = F[f][7] - F[f][1], e2z = F[f][8] - F[f][2];
-                double nx_ = e1y * e2z - e1z * e2y, ny_ = e1z * e2x - e1x * e2z, nz_ = e1x * e2y - e1y * e2x;
-                double nd = -(nx_ * F[f][0] + ny_ * F[f][1] + nz_ * F[f][2]);
-                const double nl = sqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
-                if (nl > 0) {
-                    nx_ /= nl;
-                    ny_ /= nl;
-                    nz_ /= nl;
-                    nd /= nl;
+                cpl[lane] = pl;
+            }
+            wsync();
+            for (int i = 0; i < 16; ++i) {
+                const int k = jb + W * i;
+                if (k >= ntri) break;
+                const Pl<RP> q0 = cpl[4 * i], q1 = cpl[4 * i + 1], q2 = cpl[4 * i + 2], q3 = cpl[4 * i + 3];
+                double4 ck = make_double4(0.0, 0.0, 0.0, 0.0);
+                const bool skip = may_skip && (c.removes_all(cvt<R>(q0)) || c.removes_all(cvt<R>(q1)) ||
+                                               c.removes_all(cvt<R>(q2)) || c.removes_all(cvt<R>(q3)));
+                if (!skip) {
+                    int res = kNoClip;
+                    const Pl<RP> *qs[4] = {&q0, &q1, &q2, &q3};
+#pragma unroll
+                    for (int f = 0; f < 4; ++f)
+                        if (res < 0) res = c.nb_plane >= P ? int(RPD_PLANE_OVERFLOW) : c.clip(cvt<R>(*qs[f]), -1, *qs[f]);
+                    if (res >= 0 && res != RPD_EMPTY_CELL) {
+                        fail_k = k;
+                        fail_st = res;
+                        break;
+                    }
+                    if (res != RPD_EMPTY_CELL) ck = integrate(goo[i] ? 1.0 : -1.0);
+                    c = bk;
                 }
-                const Pl<RP> cp{RP(nx_), RP(ny_), RP(nz_), RP(nd)};
-                res = clip(cvt<R>(cp), -1, cp);
-                if (res >= 0) break;
+                if (lane == 0) contrib[k] = ck;
             }
-            if (res == RPD_EMPTY_CELL) {
-                cs = bk;
-                nb_plane = bk_np;
-                maxr2 = bk_r2;
-                continue;
-            }
-            if (res >= 0) {
-                fail(res, kused);
-                return;
-            }
-            if (!integrate(go_out ? 1.0 : -1.0)) {
-                fail(RPD_NEEDS_EXACT_PREDICATES, kused);
-                return;
-            }
-            cs = bk;
-            nb_plane = bk_np;
-            maxr2 = bk_r2;
+            wsync();  // the next batch overwrites cpl / goo
         }
     }
+    if (lane == 0) {
+        fails[2 * sub] = fail_k;
+        fails[2 * sub + 1] = fail_st;
+    }
+    if (W > 1)
+        __syncthreads();
+    else
+        wsync();
+    if (sub != 0) return;
+    // the reference stops at the first failing triangle
+    int fk = 0x7fffffff, fs = 0;
+    for (int w = 0; w < W; ++w) {
+        const int k = fails[2 * w];
+        if (k < fk) {
+            fk = k;
+            fs = fails[2 * w + 1];
+        }
+    }
+    if (fk != 0x7fffffff) {
+        fail(fs);
+        return;
+    }
+    // fixed summation order: lane-strided partial sums, one wave reduction,
+    // then the whole-cell term
+    double sm = 0, sx = 0, sy = 0, sz = 0;
+    for (int k = lane; k < ntri; k += 64) {
+        const double4 q = contrib[k];
+        sx += q.x;
+        sy += q.y;
+        sz += q.z;
+        sm += q.w;
+    }
+    const double acc_m = cadd.w + wave_sum(sm), acc_x = cadd.x + wave_sum(sx), acc_y = cadd.y + wave_sum(sy),
+                 acc_z = cadd.z + wave_sum(sz);
     // local first moments -> global frame
-    finish(RPD_SUCCESS, acc_x + acc_m * S0.x, acc_y + acc_m * S0.y, acc_z + acc_m * S0.z, acc_m, kused);
+    finish(RPD_SUCCESS, acc_x + acc_m * S0.x, acc_y + acc_m * S0.y, acc_z + acc_m * S0.z, acc_m);
 }
 
 }  // namespace rpd
 
 using namespace rpd;
 
+size_t rpd_clip_record_bytes(int form, int P, int V) {
+    const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
+    const size_t pl = form == RPD_PREC_FP32 ? 16 : 32;
+    return (sizeof(RecHeader) + size_t(P) * pl + 4 * 64 * size_t(S) + 15) & ~size_t(15);
+}
+
 int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
     if (L.n_ids <= 0) return RPD_OK;
     if (L.P < 7 || L.P > 255 || L.V < 8 || L.K < 1) return RPD_E_ARG;
+    if (L.V > 256) return RPD_E_LIMIT;
+    if (!L.rec || !L.qcount) return RPD_E_ARG;
     ClipArgs a{};
     a.seeds = L.seeds;
     a.perm = L.perm;
     a.ids = L.ids;
     a.knn = L.knn;
     a.n_ids = L.n_ids;
+    a.n_seeds = L.n_seeds;
     a.K = L.K;
     a.P = L.P;
     a.V = L.V;
@@ -899,34 +1132,83 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
     a.out = L.out;
     a.status = L.status;
     a.kused = L.kused;
-    if (L.V > 256) return RPD_E_LIMIT;
+    a.rec = static_cast<char *>(L.rec);
+    a.rec_stride = int(rpd_clip_record_bytes(form, L.P, L.V));
+    a.qcount = L.qcount;
     const int S = L.V <= 64 ? 1 : (L.V <= 128 ? 2 : 4);
     a.tl_cap = 4 * a.T + 64;
     const size_t pl_bytes = size_t(L.P) * (form == RPD_PREC_FP32 ? 16 : 32);
-    size_t stride = ((pl_bytes + 15) & ~size_t(15)) + 2 * 4 * 64 * S + 256 + 4 * size_t(a.tl_cap + a.T);
-    stride = (stride + 15) & ~size_t(15);
-    a.lds_stride = int(stride);
-    int wpb = 4;
-    while (wpb > 1 && stride * wpb > 64 * 1024) wpb >>= 1;
-    if (stride > 64 * 1024) return RPD_E_LIMIT;
-    const unsigned blocks = unsigned((L.n_ids + wpb - 1) / wpb);
-    const dim3 tb(64 * wpb);
-    const size_t shm = stride * wpb;
-#define RPD_CLIP_LAUNCH(R, RP, F)                                                          \
-    do {                                                                                   \
-        if (S == 1)                                                                        \
-            clip_kernel<R, RP, F, 1><<<blocks, tb, shm, st>>>(a);                          \
-        else if (S == 2)                                                                   \
-            clip_kernel<R, RP, F, 2><<<blocks, tb, shm, st>>>(a);                          \
-        else                                                                               \
-            clip_kernel<R, RP, F, 4><<<blocks, tb, shm, st>>>(a);                          \
+    const size_t base_stride = ((pl_bytes + 15) & ~size_t(15)) + 2 * 4 * 64 * S + 256;
+    if (hipMemsetAsync(L.qcount, 0, sizeof(int), st) != hipSuccess) return RPD_E_DEVICE;
+
+    // kernel A: the bisector loop of every cell
+    {
+        const size_t stride = (base_stride + 15) & ~size_t(15);
+        a.lds_stride = int(stride);
+        const int wpb = 4;
+        const unsigned blocks = unsigned((L.n_ids + wpb - 1) / wpb);
+        const size_t shm = stride * wpb;
+#define RPD_CLIP_A(R, RP, F)                                                          \
+    do {                                                                              \
+        if (S == 1)                                                                   \
+            clip_cells_kernel<R, RP, F, 1><<<blocks, dim3(64 * wpb), shm, st>>>(a);   \
+        else if (S == 2)                                                              \
+            clip_cells_kernel<R, RP, F, 2><<<blocks, dim3(64 * wpb), shm, st>>>(a);   \
+        else                                                                          \
+            clip_cells_kernel<R, RP, F, 4><<<blocks, dim3(64 * wpb), shm, st>>>(a);   \
     } while (0)
-    if (form == RPD_PREC_FP64)
-        RPD_CLIP_LAUNCH(double, double, false);
-    else if (form == RPD_PREC_MIXED)
-        RPD_CLIP_LAUNCH(float, double, true);
-    else
-        RPD_CLIP_LAUNCH(float, float, false);
-#undef RPD_CLIP_LAUNCH
+        if (form == RPD_PREC_FP64)
+            RPD_CLIP_A(double, double, false);
+        else if (form == RPD_PREC_MIXED)
+            RPD_CLIP_A(float, double, true);
+        else
+            RPD_CLIP_A(float, float, false);
+#undef RPD_CLIP_A
+        if (hipGetLastError() != hipSuccess) return RPD_E_DEVICE;
+    }
+    if (L.ev_mid && hipEventRecord(L.ev_mid, st) != hipSuccess) return RPD_E_DEVICE;
+    int nq = 0;
+    if (hipMemcpyAsync(&nq, L.qcount, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return RPD_E_DEVICE;
+    if (hipStreamSynchronize(st) != hipSuccess) return RPD_E_DEVICE;
+    if (L.n_queued) *L.n_queued = nq;
+    if (nq <= 0) return RPD_OK;
+
+    // kernel B: restriction + integration of the queued cells.  Short queues
+    // (re-queue passes) spread each cell over W waves.
+    {
+        const size_t pl = form == RPD_PREC_FP32 ? 16 : 32;
+        const size_t wstride = finish_wave_lds(L.P, S, a.tl_cap, a.T, pl);
+        const size_t cstride = (32 * size_t(a.T) + 8 * 8 + 15) & ~size_t(15);  // contrib[T], fails[8][2]
+        constexpr size_t kLdsMax = 64 * 1024;
+        if (cstride + wstride > kLdsMax) return RPD_E_LIMIT;
+        a.lds_stride = int(wstride);
+        a.cell_stride = int(cstride);
+        int W = nq <= 2048 ? 8 : (nq <= 8192 ? 4 : (nq <= 32768 ? 2 : 1));
+        while (W > 1 && cstride + W * wstride > kLdsMax) W >>= 1;
+        int wpb = W;
+        if (W == 1) {
+            wpb = 4;
+            while (wpb > 1 && wpb * (cstride + wstride) > kLdsMax) wpb >>= 1;
+        }
+        const int cpb = wpb / W;
+        const unsigned blocks = unsigned((nq + cpb - 1) / cpb);
+        const size_t shm = size_t(cpb) * cstride + size_t(wpb) * wstride;
+#define RPD_CLIP_B(R, RP, F)                                                                  \
+    do {                                                                                      \
+        if (S == 1)                                                                           \
+            finish_cells_kernel<R, RP, F, 1><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+        else if (S == 2)                                                                      \
+            finish_cells_kernel<R, RP, F, 2><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+        else                                                                                  \
+            finish_cells_kernel<R, RP, F, 4><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+    } while (0)
+        if (form == RPD_PREC_FP64)
+            RPD_CLIP_B(double, double, false);
+        else if (form == RPD_PREC_MIXED)
+            RPD_CLIP_B(float, double, true);
+        else
+            RPD_CLIP_B(float, float, false);
+#undef RPD_CLIP_B
+    }
     return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }

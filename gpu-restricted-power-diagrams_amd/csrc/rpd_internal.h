@@ -69,6 +69,7 @@ struct ClipLaunch {
     const int *ids;        // stored ids to process (list position t)
     const int *knn;        // n_ids * K stored ids
     int64_t n_ids;
+    int64_t n_seeds;       // seeds in the diagram
     int K, P, V, T;
     int apex_mode;
     int security_mode;
@@ -78,7 +79,14 @@ struct ClipLaunch {
     double4 *out;          // original order
     int *status;           // original order
     int *kused;            // original order (nullable)
+    void *rec;             // n_ids * rpd_clip_record_bytes(form, P, V) bytes
+    int *qcount;           // one device int
+    hipEvent_t ev_mid;     // recorded between the two kernels (nullable)
+    int *n_queued;         // host: cells handed to the finish kernel (nullable)
 };
 
-// form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32
+// Bytes of one queued-cell record.
+size_t rpd_clip_record_bytes(int form, int P, int V);
+// form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32.  Synchronizes the
+// stream once, between the bisector-loop kernel and the finish kernel.
 int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L);

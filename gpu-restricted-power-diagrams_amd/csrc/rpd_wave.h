@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+#include <type_traits>
+
 namespace rpd {
 
 constexpr int kWave = 64;
@@ -46,40 +49,73 @@ __device__ __forceinline__ uint64_t rlane(uint64_t v, int l) {
     return (uint64_t(hi) << 32) | lo;
 }
 
+// ---- DPP (data-parallel primitives of the gfx9 VALU): row shifts inside a
+// row of 16 lanes and row broadcasts of lanes 15 / 31 give a 64-lane
+// inclusive scan in 6 steps without an LDS round trip.  Lanes a step does not
+// write keep the identity element passed as `old`.
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROW = 0xf, int BANK = 0xf>
+__device__ __forceinline__ int dpp_i(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW, BANK, false);
+}
+template <int CTRL, int ROW = 0xf, int BANK = 0xf>
+__device__ __forceinline__ float dpp_f(float old, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, ROW, BANK, false));
+}
+template <int CTRL, int ROW = 0xf, int BANK = 0xf>
+__device__ __forceinline__ double dpp_d(double old, double v) {
+    const long long o = __double_as_longlong(old), b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(int(uint32_t(o)), int(uint32_t(b)), CTRL, ROW, BANK, false);
+    const int hi = __builtin_amdgcn_update_dpp(int(uint32_t(uint64_t(o) >> 32)), int(uint32_t(uint64_t(b) >> 32)),
+                                               CTRL, ROW, BANK, false);
+    return __longlong_as_double((long long)((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo)));
+}
+
+// inclusive scan with operator OP over the 64 lanes; ID is OP's identity
+template <typename T, typename OP>
+__device__ __forceinline__ T wave_incl_scan_op(T x, T id, OP op) {
+    if constexpr (sizeof(T) == 8) {
+        x = op(x, dpp_d<kDppRowShr1>(id, x));
+        x = op(x, dpp_d<kDppRowShr2>(id, x));
+        x = op(x, dpp_d<kDppRowShr4>(id, x));
+        x = op(x, dpp_d<kDppRowShr8>(id, x));
+        x = op(x, dpp_d<kDppRowBcast15, 0xa>(id, x));
+        x = op(x, dpp_d<kDppRowBcast31, 0xc>(id, x));
+    } else if constexpr (std::is_same<T, float>::value) {
+        x = op(x, dpp_f<kDppRowShr1>(id, x));
+        x = op(x, dpp_f<kDppRowShr2>(id, x));
+        x = op(x, dpp_f<kDppRowShr4>(id, x));
+        x = op(x, dpp_f<kDppRowShr8>(id, x));
+        x = op(x, dpp_f<kDppRowBcast15, 0xa>(id, x));
+        x = op(x, dpp_f<kDppRowBcast31, 0xc>(id, x));
+    } else {
+        x = op(x, dpp_i<kDppRowShr1>(id, x));
+        x = op(x, dpp_i<kDppRowShr2>(id, x));
+        x = op(x, dpp_i<kDppRowShr4>(id, x));
+        x = op(x, dpp_i<kDppRowShr8>(id, x));
+        x = op(x, dpp_i<kDppRowBcast15, 0xa>(id, x));
+        x = op(x, dpp_i<kDppRowBcast31, 0xc>(id, x));
+    }
+    return x;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    return rlane(wave_incl_scan_op(v, T(0), [](T a, T b) { return a + b; }), 63);
 }
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const T w = __shfl_xor(v, o);
-        v = w > v ? w : v;
-    }
-    return v;
+    return rlane(wave_incl_scan_op(v, T(-INFINITY), [](T a, T b) { return a > b ? a : b; }), 63);
 }
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const T w = __shfl_xor(v, o);
-        v = w < v ? w : v;
-    }
-    return v;
+    return rlane(wave_incl_scan_op(v, T(INFINITY), [](T a, T b) { return a < b ? a : b; }), 63);
 }
 // exclusive prefix sum across the wave
 __device__ __forceinline__ int wave_excl_scan(int v) {
-    const int lane = lane_id();
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const int y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    return x - v;
+    return wave_incl_scan_op(v, 0, [](int a, int b) { return a + b; }) - v;
 }
 
 // index of the lowest set bit (m != 0)

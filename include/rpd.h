@@ -144,15 +144,25 @@ typedef struct {
 } rpd_params;
 
 /* Per-solve statistics (VoroAlgoStat, voronoi.h:60-103). */
+#define RPD_MAX_PASSES 8
 typedef struct {
     double ms_bin, ms_knn, ms_clip, ms_fallback, ms_total; /* device time   */
     double ms_knn_first, ms_clip_first; /* kNN and clip launch of pass 1     */
+    double ms_cells_first, ms_finish_first; /* pass 1 clip split: bisector
+                                       loop kernel / restriction+integration */
     int32_t status_hist[RPD_STATUS_NUM];  /* final status per cell           */
     int32_t passes;                 /* clip passes run                       */
     int32_t fallback_cells;         /* cells recomputed in FP64 (mixed form) */
     int32_t final_K, final_P, final_V, final_T;
     int32_t dim;                    /* kNN grid resolution                   */
+    int32_t queued_first;           /* pass 1 cells that reached the finish
+                                       kernel (not culled, not failed)      */
     int64_t n;                      /* seeds                                 */
+    /* per pass (one VoroAlgoStat per step, voronoi.cpp:1269-1330); pass p >=
+       RPD_MAX_PASSES-1 is accumulated into the last entry                   */
+    int32_t pass_cells[RPD_MAX_PASSES];   /* cells entering the pass        */
+    int32_t pass_caps[RPD_MAX_PASSES][4]; /* K, P, V, T of the pass          */
+    double pass_ms_knn[RPD_MAX_PASSES], pass_ms_clip[RPD_MAX_PASSES];
 } rpd_stats;
 
 /* Replaces OpenCLContext::createContext (openCL.cpp:151-207) + VoroAlgo ctor

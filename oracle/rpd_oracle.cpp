@@ -121,6 +121,13 @@ inline R4<R> plane_from_points(const R4<R> &A, const R4<R> &B, const R4<R> &C) {
     return p;
 }
 
+// apex_mode 1 puts the cone apex at this fraction of a voxel whose interior
+// has one known class (in or out).  Any interior point is valid; a generic
+// one keeps the cone planes off the voxel lattice and off the CUBE_EPSILON
+// box (the voxel centre lies on planes such as y + z = 1000 that contain box
+// edges such as y = 1000.1, z = -0.1).
+constexpr double kApexFrac[3] = {0.530901699437495, 0.520710678118655, 0.536602540378444};
+
 struct Counters {
     long long n_pred = 0, n_radius_vertex = 0, n_planes = 0, n_bbox_vertex = 0, n_integrated_vertex = 0,
               n_clip = 0;
@@ -492,7 +499,7 @@ inline bool merge_union(const std::vector<int> &a, const int *b, int nb, int cap
 template <class R>
 CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, int P, int V, int T, int vhs,
                            const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
-                           R4<R> *out, Counters *ctr) {
+                           R4<R> *out, Counters *ctr, bool all_listed) {
     CellResult<R> res;
     const R vol_eps2 = sizeof(R) == sizeof(float) ? R(1.e-6) : R(1.e-12);  // voronoi.cpp:919-922
     if (double(pts[seed].w) <= kNoCellLimits) {
@@ -527,6 +534,9 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
     }
     res.kused = std::min(i + 1, K);
     if (complete) res.kused = i;
+    // a list that holds every other seed leaves nothing to test: the cell is
+    // exact once it is exhausted (the reference reads past the seeds there)
+    if (!complete && i == K && all_listed) complete = true;
     res.maxP = cc.nb_plane;
     res.maxV = cc.nb_vertex;
     if (!complete && !cc.radius_reached(pts[neigs[K - 1]])) {
@@ -572,7 +582,8 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
                         const int cf = grid->centre ? grid->centre[id] : (o0 == o1 ? f : 1);
                         if (cf != 1) {
                             free_flag = cf;
-                            freePt = mk4<R>((R(x) + R(0.5)) * vs, (R(y) + R(0.5)) * vs, (R(z) + R(0.5)) * vs, R(1));
+                            freePt = mk4<R>((R(x) + R(kApexFrac[0])) * vs, (R(y) + R(kApexFrac[1])) * vs,
+                                            (R(z) + R(kApexFrac[2])) * vs, R(1));
                         }
                     }
                     if (o0 == o1) continue;
@@ -901,7 +912,8 @@ void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, 
         for (long long t = 0; t < nids; ++t) {
             const int s = ids[size_t(t)];
             CellResult<R> r = compute_cell<R>(kg.stored.data(), s, nb.data() + t * K, K, P, V, T, vhs, grid,
-                                              apex_mode, out_domain, security_mode, wmin, &out[size_t(s)], &local);
+                                              apex_mode, out_domain, security_mode, wmin, &out[size_t(s)], &local,
+                                              K >= int(kg.stored.size()) - 1);
             status[size_t(s)] = r.status;
             kused[size_t(s)] = r.kused;
         }
@@ -1087,7 +1099,7 @@ int orc_clip_cell(const double *xyzw, int n, int seed, const int *neigs, int K, 
         R4<R> out = mk4<R>(0, 0, 0, 0);
         CellResult<R> r = compute_cell<R>(pts.data(), seed, neigs, K, P, V, T, sizeof(R) == 8 ? 2 : vhs,
                                           grid ? &gv : nullptr, apex_mode, out_domain != 0, security_mode, wmin,
-                                          &out, &ctr);
+                                          &out, &ctr, K >= n - 1);
         out4[0] = double(out.x);
         out4[1] = double(out.y);
         out4[2] = double(out.z);
