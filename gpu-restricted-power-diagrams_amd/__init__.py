@@ -161,7 +161,7 @@ def lib():
     for name, (res, args) in sig.items():
         fn = getattr(L, name, None)
         if fn is None:
-            continue
+            raise RpdError("librpd.so does not export %s; rebuild it" % name)
         fn.restype = res
         fn.argtypes = args
     _lib = L
@@ -353,6 +353,60 @@ class Context:
         self.check(lib().rpd_compact(self._h, float(keep_volume), ctypes.byref(cnt), _p(out), _p(perm)),
                    "rpd_compact")
         return out, perm
+
+
+def params(K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, apex_mode=1, security_mode=0, cull=1):
+    return Params(int(K), int(P), int(V), int(T), int(precision), int(max_extra_steps), int(apex_mode),
+                  int(security_mode), int(cull))
+
+
+class Server:
+    """Request loop for streams of solves (rpd_server_*): submit requests of
+    any size against resident domains, poll finished ones with their latency."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0, max_inflight=4):
+        h = ctypes.c_void_p()
+        cfg = ServerConfig(int(hbm_budget_bytes), int(max_inflight), 0)
+        _check(lib().rpd_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)), None, "rpd_server_create")
+        self._h = h
+        self._n = {}
+
+    def submit(self, domain, xyzw, **kw):
+        xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
+        p = params(**kw)
+        t = I64(0)
+        _check(lib().rpd_server_submit(self._h, domain._h if domain is not None else None, _p(xyzw), xyzw.shape[0],
+                                       ctypes.byref(p), ctypes.byref(t)), None, "rpd_server_submit")
+        self._n[t.value] = xyzw.shape[0]
+        return t.value
+
+    def poll(self, wait=True):
+        """(ticket, moments[n,4], latency_ms) of a finished request, or None."""
+        cap = max(self._n.values()) if self._n else 0
+        out = np.zeros((max(cap, 1), 4), np.float64)
+        t = I64(-1)
+        lat = ctypes.c_double(0)
+        rc = lib().rpd_server_poll(self._h, 1 if wait else 0, ctypes.byref(t), _p(out), cap, ctypes.byref(lat))
+        if t.value < 0:
+            _check(rc, None, "rpd_server_poll")
+            return None
+        n = self._n.pop(t.value)
+        _check(rc, None, "rpd_server_poll (ticket %d)" % t.value)
+        return t.value, out[:n].copy(), lat.value
+
+    def drain(self):
+        _check(lib().rpd_server_drain(self._h), None, "rpd_server_drain")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rpd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Domain:

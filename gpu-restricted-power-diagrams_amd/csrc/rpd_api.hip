@@ -30,7 +30,7 @@
 struct rpd_domain {
     rpd_ctx *ctx = nullptr;
     DomainDev dev;
-    DevBuf flags, centre, pts, tris, tlist, toffs, dist;
+    DevBuf flags, centre, pts, tris, tlist, toffs, tvoffs, tvox, dist;
     int64_t bytes = 0;
 };
 
@@ -209,6 +209,9 @@ float elapsed_ms(hipEvent_t a, hipEvent_t b) {
 
 }  // namespace
 
+// device of a context (used by the serving loop to open its slots)
+int rpd_ctx_device(const rpd_ctx *c) { return c ? c->device : 0; }
+
 extern "C" {
 
 int rpd_create(int device, rpd_ctx **out) {
@@ -262,6 +265,7 @@ void rpd_default_params(rpd_params *p) {
 
 int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     if (!c || !g || !out || g->size <= 0) return set_err(c, RPD_E_ARG, "rpd_domain_create: bad arguments");
+    if (g->size > 1024) return set_err(c, RPD_E_LIMIT, "rpd_domain_create: at most 1024 voxels per axis");
     hipSetDevice(c->device);
     auto *d = new rpd_domain();
     d->ctx = c;
@@ -283,6 +287,30 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     if (r == RPD_OK) r = up(d->tris, g->triangles, size_t(g->num_triangles) * 12);
     if (r == RPD_OK) r = up(d->tlist, g->tri_list, size_t(g->num_tri_list) * 4);
     if (r == RPD_OK) r = up(d->toffs, g->offsets, (nv + 1) * 4);
+    // per triangle, the voxels whose list holds it, ascending (the finish
+    // kernel keeps a list entry only in the first voxel of a cell's box)
+    std::vector<int> tvoffs(size_t(g->num_triangles) + 1, 0);
+    std::vector<uint32_t> tvox(size_t(std::max(g->num_tri_list, 0)));
+    for (size_t v = 0; v < nv && r == RPD_OK; ++v)
+        for (int q = g->offsets[v]; q < g->offsets[v + 1]; ++q) {
+            const int t = g->tri_list[q];
+            if (t < 0 || t >= g->num_triangles) {
+                r = RPD_E_ARG;
+                break;
+            }
+            tvoffs[size_t(t) + 1]++;
+        }
+    for (size_t t = 0; t < size_t(g->num_triangles); ++t) tvoffs[t + 1] += tvoffs[t];
+    if (r == RPD_OK) {
+        std::vector<int> cur(tvoffs.begin(), tvoffs.end() - 1);
+        const size_t gg = size_t(g->size);
+        for (size_t v = 0; v < nv; ++v)
+            for (int q = g->offsets[v]; q < g->offsets[v + 1]; ++q)
+                tvox[size_t(cur[size_t(g->tri_list[q])]++)] =
+                    (uint32_t(v / (gg * gg)) << 20) | (uint32_t((v / gg) % gg) << 10) | uint32_t(v % gg);
+    }
+    if (r == RPD_OK) r = up(d->tvoffs, tvoffs.data(), tvoffs.size() * 4);
+    if (r == RPD_OK) r = up(d->tvox, tvox.data(), tvox.size() * 4);
     std::vector<uint8_t> dist;
     domain_distance(g, dist);
     if (r == RPD_OK) r = up(d->dist, dist.data(), nv);
@@ -299,6 +327,8 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     d->dev.tris = d->tris.as<int>();
     d->dev.tlist = d->tlist.as<int>();
     d->dev.toffs = d->toffs.as<int>();
+    d->dev.tvoffs = d->tvoffs.as<int>();
+    d->dev.tvox = d->tvox.as<uint32_t>();
     d->dev.dist = d->dist.as<uint8_t>();
     for (size_t i = 0; i < nv; ++i)
         if (g->in_domain[i] == 1) {
@@ -357,7 +387,8 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     if (prm->precision < 0 || prm->precision > 2) return set_err(c, RPD_E_ARG, "rpd_solve: bad precision");
     if (prm->K <= 0 || prm->K > 256 || prm->P < 7 || prm->P > 255 || prm->V < 8 || prm->V > 256 || prm->T < 0)
         return set_err(c, RPD_E_ARG, "rpd_solve: caps out of range (0<K<=256, 7<=P<=255, 8<=V<=256, T>=0)");
-    if (dom && dom->ctx != c) return set_err(c, RPD_E_ARG, "rpd_solve: domain of another context");
+    // a domain is read-only device memory: any context on its device may use it
+    if (dom && dom->ctx->device != c->device) return set_err(c, RPD_E_ARG, "rpd_solve: domain on another device");
     hipSetDevice(c->device);
     const int64_t n = c->n;
     const int form0 = prm->precision;

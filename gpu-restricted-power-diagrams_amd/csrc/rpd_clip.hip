@@ -77,7 +77,8 @@ struct ClipArgs {
     int *kused;
     int lds_stride;   // LDS bytes of one wave
     int cell_stride;  // LDS bytes shared by the waves of one cell (finish kernel)
-    int tl_cap;
+    const int *tvoffs;    // per triangle: offsets into tvox
+    const uint32_t *tvox; // per triangle: sorted voxels whose list holds it, z<<20 | y<<10 | x
     int security_mode;    // 0 exact power stop, 1 reference stop
     double wmin;          // minimum weight over non-border seeds
     const uint8_t *dist;  // Chebyshev voxel distance to the nearest domain voxel (nullable)
@@ -109,10 +110,11 @@ __device__ __forceinline__ Pl<T> cvt(const Pl<U> &p) {
 
 // Homogeneous vertex of planes (p1, p2, p3): the cofactors of
 // CCComputeVertexCoordinates (ConvexCell.cl:344-362), plus (FILTER) the sums
-// of absolute values of their terms.
+// of absolute values of their terms: MXYZ the largest of those of X, Y, Z,
+// MW that of W.
 template <typename R, bool FILTER>
 __device__ __forceinline__ void vertex_from_planes(const Pl<R> &p1, const Pl<R> &p2, const Pl<R> &p3, R &X, R &Y,
-                                                   R &Z, R &W, float &MX, float &MY, float &MZ, float &MW) {
+                                                   R &Z, R &W, float &MXYZ, float &MW) {
     const R m12 = p2.x * p3.y - p2.y * p3.x;
     const R m13 = p2.x * p3.z - p2.z * p3.x;
     const R m14 = p2.x * p3.w - p2.w * p3.x;
@@ -132,16 +134,19 @@ __device__ __forceinline__ void vertex_from_planes(const Pl<R> &p1, const Pl<R> 
         const float a34 = fabsf(float(p2.z * p3.w)) + fabsf(float(p2.w * p3.z));
         const float ax = fabsf(float(p1.x)), ay = fabsf(float(p1.y)), az = fabsf(float(p1.z)),
                     aw = fabsf(float(p1.w));
-        MX = aw * a23 + ay * a34 + az * a24;
-        MY = ax * a34 + aw * a13 + az * a14;
-        MZ = ax * a24 + ay * a14 + aw * a12;
+        const float MX = aw * a23 + ay * a34 + az * a24;
+        const float MY = ax * a34 + aw * a13 + az * a14;
+        const float MZ = ax * a24 + ay * a14 + aw * a12;
+        MXYZ = fmaxf(MX, fmaxf(MY, MZ));
         MW = ax * a23 + ay * a13 + az * a12;
     }
 }
 
 // Error bound factor of the mixed form: 64 units in the last place of FP32
 // (first-order analysis gives 13: 4 from rounding the FP64 planes, 9 from the
-// cofactor and dot-product evaluation).
+// cofactor and dot-product evaluation).  The bound of a test e.v is
+// kappa * (|e.x| MX + |e.y| MY + |e.z| MZ + |e.w| MW), taken here as
+// kappa * ((|e.x| + |e.y| + |e.z|) MXYZ + |e.w| MW), which is not smaller.
 constexpr float kFilterKappa = 0x1.0p-18f;
 // vertices with |W| below this fraction of its term magnitude are
 // ill-conditioned in FP32 and sent to FP64
@@ -172,13 +177,13 @@ struct WaveCell {
     int P, V, nb_plane, lane;
     uint32_t tri[S];
     R X[S], Y[S], Z[S], W[S], r2[S];
-    float MX[S], MY[S], MZ[S], MW[S];
+    float MXYZ[S], MW[S];
     uint64_t valid[S];
     R maxr2;
 
     __device__ void set_vertex(int s, uint32_t tr, const Pl<R> &p1, const Pl<R> &p2, const Pl<R> &p3) {
         tri[s] = tr;
-        vertex_from_planes<R, FILTER>(p1, p2, p3, X[s], Y[s], Z[s], W[s], MX[s], MY[s], MZ[s], MW[s]);
+        vertex_from_planes<R, FILTER>(p1, p2, p3, X[s], Y[s], Z[s], W[s], MXYZ[s], MW[s]);
         const R iw = R(1) / W[s];
         const R ix = X[s] * iw, iy = Y[s] * iw, iz = Z[s] * iw;
         r2[s] = ix * ix + iy * iy + iz * iz;
@@ -201,14 +206,14 @@ struct WaveCell {
     // of |terms| of e.v (FP64 and FP32 forms).
     __device__ bool removes_all(const Pl<R> &e) const {
         bool keep = false;
+        const float exyz = fabsf(float(e.x)) + fabsf(float(e.y)) + fabsf(float(e.z)), ew = fabsf(float(e.w));
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const bool v = (valid[s] >> lane) & 1;
             const R sv = e.x * X[s] + e.y * Y[s] + e.z * Z[s] + e.w * W[s];
             R m;
             if (FILTER)
-                m = R(kFilterKappa * (fabsf(float(e.x)) * MX[s] + fabsf(float(e.y)) * MY[s] +
-                                      fabsf(float(e.z)) * MZ[s] + fabsf(float(e.w)) * MW[s]));
+                m = R(kFilterKappa * (exyz * MXYZ[s] + ew * MW[s]));
             else
                 m = R(sizeof(R) == 8 ? 0x1.0p-33 : 0x1.0p-16) *
                     (fabs(e.x * X[s]) + fabs(e.y * Y[s]) + fabs(e.z * Z[s]) + fabs(e.w * W[s]));
@@ -224,14 +229,14 @@ struct WaveCell {
         uint64_t rem[S];
         int nr = 0, nvalid = 0;
         uint64_t flag = 0;
+        const float exyz = fabsf(float(e.x)) + fabsf(float(e.y)) + fabsf(float(e.z)), ew = fabsf(float(e.w));
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const bool v = (valid[s] >> lane) & 1;
             const R sv = e.x * X[s] + e.y * Y[s] + e.z * Z[s] + e.w * W[s];
             rem[s] = ballot(v && sv > R(0));
             if (FILTER) {
-                const float bound = kFilterKappa * (fabsf(float(e.x)) * MX[s] + fabsf(float(e.y)) * MY[s] +
-                                                    fabsf(float(e.z)) * MZ[s] + fabsf(float(e.w)) * MW[s]);
+                const float bound = kFilterKappa * (exyz * MXYZ[s] + ew * MW[s]);
                 flag |= ballot(v && fabsf(float(sv)) <= bound);
             }
             nr += __popcll(rem[s]);
@@ -367,12 +372,11 @@ struct WaveCell {
     }
 };
 
-// LDS of one wave: planes[P] | rt[64S] | nvs[64S] | nxt[256] | (finish
-// kernel only) tl[tl_cap] | tu[T] | cone planes cpl[64] | go_out flags[16]
+// LDS of one wave: planes[P] | rt[64S] | nvs[64S] | nxt[256] | tu[T]
+// (finish kernel) | chunk / cone planes cpl[64] | go_out flags[16]
 template <typename R, typename RP, bool FILTER, int S>
-__device__ __forceinline__ void bind_lds(char *L, int P, int tl_cap, int T, WaveCell<R, RP, FILTER, S> &c,
-                                         int **tl = nullptr, int **tu = nullptr, Pl<RP> **cpl = nullptr,
-                                         int **goo = nullptr) {
+__device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, FILTER, S> &c, int **tu = nullptr,
+                                         Pl<RP> **cpl = nullptr, int **goo = nullptr) {
     c.planes = reinterpret_cast<Pl<RP> *>(L);
     size_t off = (size_t(P) * sizeof(Pl<RP>) + 15) & ~size_t(15);
     c.rt = reinterpret_cast<uint32_t *>(L + off);
@@ -381,8 +385,6 @@ __device__ __forceinline__ void bind_lds(char *L, int P, int tl_cap, int T, Wave
     off += 4 * 64 * S;
     c.nxt = reinterpret_cast<uint8_t *>(L + off);
     off += 256;
-    if (tl) *tl = reinterpret_cast<int *>(L + off);
-    off += 4 * size_t(tl_cap);
     if (tu) *tu = reinterpret_cast<int *>(L + off);
     off += 4 * size_t(T);
     off = (off + 15) & ~size_t(15);
@@ -392,8 +394,8 @@ __device__ __forceinline__ void bind_lds(char *L, int P, int tl_cap, int T, Wave
 }
 
 // LDS bytes of one wave of the finish kernel
-__host__ __device__ constexpr size_t finish_wave_lds(int P, int S, int tl_cap, int T, size_t pl) {
-    return ((((((P * pl + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) + 256 + 4 * size_t(tl_cap) + 4 * size_t(T)) +
+__host__ __device__ constexpr size_t finish_wave_lds(int P, int S, int T, size_t pl) {
+    return ((((((P * pl + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) + 256 + 4 * size_t(T)) +
               15) & ~size_t(15)) + 64 * pl + 64 + 15) & ~size_t(15);
 }
 
@@ -406,8 +408,16 @@ struct RecHeader {
 // ---------------------------------------------------------------------------
 // Kernel A: the bisector loop (ConvexCell.cl:850-917)
 // ---------------------------------------------------------------------------
+// Minimum waves per SIMD the bisector kernel is compiled for (one vertex
+// slot per lane): its run time is set by latency, so occupancy pays more than
+// the few registers the compiler then spills (measured at 1M seeds: FP64 5,
+// FP32 forms 6).  Wider cells (S > 1, re-queue passes) are left unconstrained.
+template <typename R, int S>
+constexpr int kClipWaves = S > 1 ? 1 : (sizeof(R) == 8 ? 5 : 6);
+
 template <typename R, typename RP, bool FILTER, int S>
-__global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kClipWaves<R, S>)))
+void clip_cells_kernel(ClipArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int lane = lane_id();
     const int wv = threadIdx.x >> 6;
@@ -416,7 +426,8 @@ __global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
     if (t >= a.n_ids) return;  // wave-uniform
 
     WaveCell<R, RP, FILTER, S> c;
-    bind_lds(lds_raw + size_t(wv) * a.lds_stride, a.P, 0, 0, c);
+    Pl<RP> *chunk = nullptr;  // planes of the current 64 neighbours
+    bind_lds(lds_raw + size_t(wv) * a.lds_stride, a.P, 0, c, nullptr, &chunk);
     c.P = a.P;
     c.V = a.V;
     c.lane = lane;
@@ -474,7 +485,7 @@ __global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
         c.X[s] = c.Y[s] = c.Z[s] = R(0);
         c.W[s] = R(1);
         c.r2[s] = R(0);
-        c.MX[s] = c.MY[s] = c.MZ[s] = c.MW[s] = 0.f;
+        c.MXYZ[s] = c.MW[s] = 0.f;
     }
     if (lane < 8) {
         const uint32_t sh = 8 * uint32_t(lane);
@@ -485,9 +496,9 @@ __global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
     c.refresh_max_r2();
 
     // ---- neighbour loop -----------------------------------------------------
-    // The 64 neighbours of a chunk are loaded one per lane with their plane,
-    // distance |b| and plane offset t (signed distance of the bisector from
-    // the seed).  A plane with t >= R, R the distance from the seed to the
+    // The 64 neighbours of a chunk are loaded one per lane: the plane goes to
+    // LDS, the distance |b| and plane offset t (signed distance of the
+    // bisector from the seed) stay in registers.  A plane with t >= R, R the distance from the seed to the
     // farthest vertex, cannot cut the cell, so instead of visiting every
     // neighbour the wave ballots for the next neighbour whose plane may cut
     // (t < R) and for the first one where the stop test holds, and jumps to
@@ -498,9 +509,8 @@ __global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
     const int *nrow = a.knn + t * int64_t(K);
     bool reached = false, complete = false;
     int kused = 0;
-    Pl<RP> cpl;
-    Pl<R> cplr;
-    double cdl = 0, ct = 0;
+    double cdl = 0;
+    float ct = 0;
     int cnb = -1;
     int loaded = -1;
     const double rmarg = sizeof(R) == 8 ? 1.0 + 1e-9 : 1.0 + 2e-4;  // FP32 R may be low by ~1e-4
@@ -518,7 +528,7 @@ __global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
         if (base != loaded) {
             const int j = base + lane;
             cnb = j < K ? nrow[j] : -2;  // -1: list exhausted, -2: past K
-            cpl = Pl<RP>{RP(0), RP(0), RP(0), RP(0)};
+            Pl<RP> cpl{RP(0), RP(0), RP(0), RP(0)};
             cdl = 0;
             ct = 0;
             if (cnb >= 0) {
@@ -534,9 +544,11 @@ __global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
                     d = (b2 + (B.w - S0.w)) * (0.5 * ib);
                 cpl = Pl<RP>{RP(-bx * ib), RP(-by * ib), RP(-bz * ib), RP(d)};
                 cdl = bl;
-                ct = d;
+                ct = float(d);
             }
-            cplr = cvt<R>(cpl);
+            wsync();  // the previous chunk's planes are no longer read
+            chunk[lane] = cpl;
+            wsync();
             loaded = base;
         }
         const int l0 = i & 63;
@@ -555,7 +567,9 @@ __global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
             i = base + 64;
             continue;
         }
-        const double Rskip = Rcur * rmarg;
+        // skip test in FP32: t is rounded to FP32, so the radius gets a margin
+        // well above FP32 rounding (a false candidate only costs a no-op clip)
+        const float Rskip = float(Rcur * rmarg) * (1.0f + 1e-6f);
         const uint64_t m_end = ballot(in && cnb == -1);
         const uint64_t m_clip = ballot(in && cnb >= 0 && ct < Rskip);
         const uint64_t m_stop = ballot(in && cnb >= 0 && stop_at(cdl, Rcur));
@@ -578,7 +592,8 @@ __global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
             continue;
         }
         // clip by the plane of neighbour base + f_clip
-        const int res = c.clip(rlane_pl(cplr, f_clip), f_clip, cpl);
+        const Pl<RP> eP = chunk[f_clip];
+        const int res = c.clip(cvt<R>(eP), -1, eP);
         kused = base + f_clip + 1;
         if (res >= 0) {
             fail(res, kused);
@@ -647,7 +662,8 @@ __global__ __launch_bounds__(256) void clip_cells_kernel(ClipArgs a) {
 // outcome is the reference's.
 // ---------------------------------------------------------------------------
 template <typename R, typename RP, bool FILTER, int S>
-__global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, int W) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == 1 ? 4 : 1)))
+void finish_cells_kernel(ClipArgs a, int nq, int W) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int lane = lane_id();
     const int wv = threadIdx.x >> 6;
@@ -661,9 +677,9 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
     double4 *contrib = reinterpret_cast<double4 *>(cell_l);              // [T]
     int *fails = reinterpret_cast<int *>(cell_l + 32 * size_t(a.T));  // [W][2]
     WaveCell<R, RP, FILTER, S> c;
-    int *tl = nullptr, *tu = nullptr, *goo = nullptr;
+    int *tu = nullptr, *goo = nullptr;
     Pl<RP> *cpl = nullptr;
-    bind_lds(lds_raw + size_t(cpb) * a.cell_stride + size_t(wv) * a.lds_stride, a.P, a.tl_cap, a.T, c, &tl, &tu,
+    bind_lds(lds_raw + size_t(cpb) * a.cell_stride + size_t(wv) * a.lds_stride, a.P, a.T, c, &tu,
              &cpl, &goo);
     c.P = a.P;
     c.V = a.V;
@@ -688,7 +704,7 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
         c.X[s] = c.Y[s] = c.Z[s] = R(0);
         c.W[s] = R(1);
         c.r2[s] = R(0);
-        c.MX[s] = c.MY[s] = c.MZ[s] = c.MW[s] = 0.f;
+        c.MXYZ[s] = c.MW[s] = 0.f;
         if (j < nvalid) c.set_vertex_lds(s, rtr[j]);
         c.valid[s] = ballot(j < nvalid);
     }
@@ -716,9 +732,8 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
         } else {
             const uint32_t tr = c.tri[s];
             RP X, Y, Z, W;
-            float d0, d1, d2, d3;
-            vertex_from_planes<RP, false>(c.planes[t0(tr)], c.planes[t1(tr)], c.planes[t2(tr)], X, Y, Z, W, d0, d1,
-                                          d2, d3);
+            float d0, d1;
+            vertex_from_planes<RP, false>(c.planes[t0(tr)], c.planes[t1(tr)], c.planes[t2(tr)], X, Y, Z, W, d0, d1);
             const RP iw = RP(1) / W;
             Ax = X * iw;
             Ay = Y * iw;
@@ -819,12 +834,18 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
         const int nx = vmx[0] - vmn[0] + 1, ny = vmx[1] - vmn[1] + 1, nz = vmx[2] - vmn[2] + 1;
         const int nvox = nx * ny * nz;
         int first_ref = -1, first_c = -1, ref_flag = 1, c_flag = 1;
-        int ncol = 0;
-        bool overflow = false;
-        for (int vb = 0; vb < nvox && !overflow; vb += 64) {
+        // Union of the voxels' lists (merge, ConvexCell.cl:722-749): a list
+        // entry is kept in the first voxel of the bounding box, in scan order,
+        // whose list holds it -- read from the triangle's own sorted voxel
+        // list -- so each triangle of the union appears once.
+        int nu = 0;
+        const uint32_t pmn = (uint32_t(vmn[2]) << 20) | (uint32_t(vmn[1]) << 10) | uint32_t(vmn[0]);
+        const uint32_t pmx = (uint32_t(vmx[2]) << 20) | (uint32_t(vmx[1]) << 10) | uint32_t(vmx[0]);
+        for (int vb = 0; vb < nvox; vb += 64) {
             const int v = vb + lane;
             const bool in = v < nvox;
             int f = 1, cf = 1, o0 = 0, cnt = 0;
+            uint32_t pv = 0;
             if (in) {
                 const int x = vmn[0] + v % nx, y = vmn[1] + (v / nx) % ny, z = vmn[2] + v / (nx * ny);
                 const int id = x + y * g + z * g * g;
@@ -832,6 +853,7 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
                 o0 = a.toffs[id];
                 cnt = a.toffs[id + 1] - o0;
                 cf = a.centre ? a.centre[id] : (cnt == 0 ? f : 1);
+                pv = (uint32_t(z) << 20) | (uint32_t(y) << 10) | uint32_t(x);
             }
             const uint64_t mr = ballot(in && f != 1);
             if (first_ref < 0 && mr) {
@@ -845,38 +867,34 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
                 first_c = vb + l;
                 c_flag = rlane(cf, l);
             }
-            const int o = wave_excl_scan(cnt);
-            const int tot = uni(rlane(o + cnt, 63));
-            if (ncol + tot > a.tl_cap) {
-                overflow = true;
-                break;
+            const int maxc = uni(rlane(wave_incl_scan_op(cnt, 0, [](int x, int y) { return x > y ? x : y; }), 63));
+            for (int k = 0; k < maxc; ++k) {
+                bool keep = false;
+                int t = -1;
+                if (k < cnt) {
+                    t = a.tlist[o0 + k];
+                    keep = true;
+                    for (int q = a.tvoffs[t], qe = a.tvoffs[t + 1]; q < qe; ++q) {
+                        const uint32_t u = a.tvox[q];
+                        if (u >= pv) break;
+                        const uint32_t ux = u & 1023u, uy = (u >> 10) & 1023u, uz = u >> 20;
+                        if (ux >= (pmn & 1023u) && ux <= (pmx & 1023u) && uy >= ((pmn >> 10) & 1023u) &&
+                            uy <= ((pmx >> 10) & 1023u) && uz >= (pmn >> 20) && uz <= (pmx >> 20)) {
+                            keep = false;  // an earlier voxel of the box holds it
+                            break;
+                        }
+                    }
+                }
+                const uint64_t m = ballot(keep);
+                if (keep && nu + mbcnt(m) < a.T) tu[nu + mbcnt(m)] = t;
+                nu += __popcll(m);
             }
-            for (int k = 0; k < cnt; ++k) tl[ncol + o + k] = a.tlist[o0 + k];
-            ncol += tot;
-        }
-        if (overflow) {
-            fail(RPD_TRIANGLE_OVERFLOW);
-            return;
-        }
-        wsync();
-        // sorted-union semantics of merge (ConvexCell.cl:722-749): keep the
-        // first occurrence of every triangle id
-        int nu = 0;
-        for (int ib = 0; ib < ncol; ib += 64) {
-            const int ii = ib + lane;
-            const int val = ii < ncol ? tl[ii] : -1;
-            bool dup = false;
-            const int jend = min(ncol, ib + 64);
-            for (int j = 0; j < jend; ++j) dup |= (j < ii) && (tl[j] == val);
-            const bool keep = ii < ncol && !dup;
-            const uint64_t m = ballot(keep);
-            if (keep && nu + mbcnt(m) < a.T) tu[nu + mbcnt(m)] = val;
-            nu += __popcll(m);
         }
         if (nu > a.T) {
             fail(RPD_TRIANGLE_OVERFLOW);
             return;
         }
+        wsync();
         ntri = nu;
         // apex
         int in_domain = 1;
@@ -956,13 +974,17 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
     double4 cadd = make_double4(0.0, 0.0, 0.0, 0.0);
     if (add_volume && sub == 0) cadd = integrate(1.0);
 
-    int fail_k = 0x7fffffff, fail_st = 0;
+    // The reference clips the triangles in increasing id order and stops at
+    // the first failure: the cell's status is that of the smallest failing
+    // id.  Triangles are visited here in list order, so after a failure only
+    // triangles of smaller id are still clipped.
+    int fail_t = 0x7fffffff, fail_st = 0;
     if (ntri > 0) {
         // backup (ConvexCell.cl:692-715): the vertex registers and the plane count
         const WaveCell<R, RP, FILTER, S> bk = c;
         const bool may_skip = c.nb_plane + 4 <= P && nvalid + 6 <= a.V;
         const double lx0 = P0x - S0.x, ly0 = P0y - S0.y, lz0 = P0z - S0.z;
-        for (int jb = sub; jb < ntri && fail_k == 0x7fffffff; jb += 16 * W) {
+        for (int jb = sub; jb < ntri; jb += 16 * W) {
             // cone planes (A,B,C), (A,D,B), (A,C,D), (C,D,B), A = apex
             // (CCClipTetFromPointsInfinite): lane = 4 * triangle + plane
             {
@@ -1028,10 +1050,12 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
             for (int i = 0; i < 16; ++i) {
                 const int k = jb + W * i;
                 if (k >= ntri) break;
+                const int tid = uni(tu[k]);
                 const Pl<RP> q0 = cpl[4 * i], q1 = cpl[4 * i + 1], q2 = cpl[4 * i + 2], q3 = cpl[4 * i + 3];
                 double4 ck = make_double4(0.0, 0.0, 0.0, 0.0);
-                const bool skip = may_skip && (c.removes_all(cvt<R>(q0)) || c.removes_all(cvt<R>(q1)) ||
-                                               c.removes_all(cvt<R>(q2)) || c.removes_all(cvt<R>(q3)));
+                const bool skip = tid > fail_t ||
+                                  (may_skip && (c.removes_all(cvt<R>(q0)) || c.removes_all(cvt<R>(q1)) ||
+                                                c.removes_all(cvt<R>(q2)) || c.removes_all(cvt<R>(q3))));
                 if (!skip) {
                     int res = kNoClip;
                     const Pl<RP> *qs[4] = {&q0, &q1, &q2, &q3};
@@ -1039,11 +1063,11 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
                     for (int f = 0; f < 4; ++f)
                         if (res < 0) res = c.nb_plane >= P ? int(RPD_PLANE_OVERFLOW) : c.clip(cvt<R>(*qs[f]), -1, *qs[f]);
                     if (res >= 0 && res != RPD_EMPTY_CELL) {
-                        fail_k = k;
+                        fail_t = tid;
                         fail_st = res;
-                        break;
+                    } else if (res != RPD_EMPTY_CELL) {
+                        ck = integrate(goo[i] ? 1.0 : -1.0);
                     }
-                    if (res != RPD_EMPTY_CELL) ck = integrate(goo[i] ? 1.0 : -1.0);
                     c = bk;
                 }
                 if (lane == 0) contrib[k] = ck;
@@ -1052,7 +1076,7 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
         }
     }
     if (lane == 0) {
-        fails[2 * sub] = fail_k;
+        fails[2 * sub] = fail_t;
         fails[2 * sub + 1] = fail_st;
     }
     if (W > 1)
@@ -1060,7 +1084,7 @@ __global__ __launch_bounds__(512) void finish_cells_kernel(ClipArgs a, int nq, i
     else
         wsync();
     if (sub != 0) return;
-    // the reference stops at the first failing triangle
+    // smallest failing triangle id over the waves
     int fk = 0x7fffffff, fs = 0;
     for (int w = 0; w < W; ++w) {
         const int k = fails[2 * w];
@@ -1125,6 +1149,8 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
         a.tris = L.dom->tris;
         a.tlist = L.dom->tlist;
         a.toffs = L.dom->toffs;
+        a.tvoffs = L.dom->tvoffs;
+        a.tvox = L.dom->tvox;
         a.dist = L.cull ? L.dom->dist : nullptr;
     }
     a.security_mode = L.security_mode;
@@ -1136,14 +1162,11 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
     a.rec_stride = int(rpd_clip_record_bytes(form, L.P, L.V));
     a.qcount = L.qcount;
     const int S = L.V <= 64 ? 1 : (L.V <= 128 ? 2 : 4);
-    a.tl_cap = 4 * a.T + 64;
-    const size_t pl_bytes = size_t(L.P) * (form == RPD_PREC_FP32 ? 16 : 32);
-    const size_t base_stride = ((pl_bytes + 15) & ~size_t(15)) + 2 * 4 * 64 * S + 256;
     if (hipMemsetAsync(L.qcount, 0, sizeof(int), st) != hipSuccess) return RPD_E_DEVICE;
 
     // kernel A: the bisector loop of every cell
     {
-        const size_t stride = (base_stride + 15) & ~size_t(15);
+        const size_t stride = finish_wave_lds(L.P, S, 0, form == RPD_PREC_FP32 ? 16 : 32);
         a.lds_stride = int(stride);
         const int wpb = 4;
         const unsigned blocks = unsigned((L.n_ids + wpb - 1) / wpb);
@@ -1177,7 +1200,7 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
     // (re-queue passes) spread each cell over W waves.
     {
         const size_t pl = form == RPD_PREC_FP32 ? 16 : 32;
-        const size_t wstride = finish_wave_lds(L.P, S, a.tl_cap, a.T, pl);
+        const size_t wstride = finish_wave_lds(L.P, S, a.T, pl);
         const size_t cstride = (32 * size_t(a.T) + 8 * 8 + 15) & ~size_t(15);  // contrib[T], fails[8][2]
         constexpr size_t kLdsMax = 64 * 1024;
         if (cstride + wstride > kLdsMax) return RPD_E_LIMIT;

@@ -59,7 +59,9 @@ struct DomainDev {
     const int *tris = nullptr;
     const int *tlist = nullptr;
     const int *toffs = nullptr;
-    const uint8_t *dist = nullptr;  // Chebyshev voxel distance to domain voxels
+    const int *tvoffs = nullptr;     // per triangle: offsets into tvox
+    const uint32_t *tvox = nullptr;  // per triangle: sorted voxels (z<<20 | y<<10 | x) whose list holds it
+    const uint8_t *dist = nullptr;   // Chebyshev voxel distance to domain voxels
     bool has_border_vertices = false;
 };
 

@@ -1,0 +1,171 @@
+"""Host-side parts of librpd (CPU only): grid writer, grid file round trip,
+Grid::valid / getVolume restatements, synthetic seed generators, and the
+C-ABI symbol table.  Nothing here launches a kernel."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, oracle_grid
+
+SHAPES = [("sphere", 0, (0.5, 0.5, 0.5, 0.45)), ("torus", 1, (0.5, 0.5, 0.5, 0.3, 0.12)),
+          ("cube", 2, ()), ("cube_minus_sphere", 3, (0.5, 0.5, 0.5, 0.3))]
+
+
+def _mask_closed_form(code, g, prm):
+    """Voxel kept iff its centre is inside the shape (unit frame)."""
+    c = (np.arange(g) + 0.5) / g
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    if code == 0:
+        return (x - prm[0]) ** 2 + (y - prm[1]) ** 2 + (z - prm[2]) ** 2 < prm[3] ** 2
+    if code == 1:
+        rho = np.hypot(x - prm[0], y - prm[1])
+        return (rho - prm[3]) ** 2 + (z - prm[2]) ** 2 < prm[4] ** 2
+    if code == 2:
+        return np.ones_like(x, dtype=bool)
+    return (x - prm[0]) ** 2 + (y - prm[1]) ** 2 + (z - prm[2]) ** 2 >= prm[3] ** 2
+
+
+@pytest.mark.parametrize("name,code,prm", SHAPES)
+def test_voxel_grid_closed_forms(rpd, orc, name, code, prm):
+    g = 20
+    mask = rpd.voxel_mask(code, g, prm).reshape(g, g, g).astype(bool)
+    ref = _mask_closed_form(code, g, prm)
+    # centres within rounding of the surface may go either way
+    assert (mask != ref).sum() <= 2
+    grid = rpd.Grid.shape(code, g, prm)
+    assert grid.valid()
+    vs = 1000.0 / g
+    vol = mask.sum() * vs ** 3
+    assert abs(grid.volume() / vol - 1) < 1e-12
+    assert abs(orc.grid_volume(oracle_grid(orc, grid)) / vol - 1) < 1e-12
+    assert orc.grid_valid(oracle_grid(orc, grid))
+    idx = np.nonzero(mask.reshape(-1))[0]
+    cen = np.stack([idx % g, (idx // g) % g, idx // (g * g)], 1) * vs + vs / 2
+    mom = grid.moments()
+    assert np.allclose(mom[:3], cen.sum(0) * vs ** 3, rtol=1e-12)
+    # corner flags: 2 / 0 / 1 when all / none / some of the 8 incident voxels
+    # (grid exterior counts as not kept) are kept; corner (i,j,k) of voxel (i,j,k)
+    pad = np.zeros((g + 2, g + 2, g + 2), dtype=int)
+    pad[1:-1, 1:-1, 1:-1] = mask
+    cnt = sum(pad[1 - dz:g + 1 - dz, 1 - dy:g + 1 - dy, 1 - dx:g + 1 - dx]
+              for dz in (0, 1) for dy in (0, 1) for dx in (0, 1))
+    flags = np.where(cnt == 8, 2, np.where(cnt == 0, 0, 1))
+    assert np.array_equal(grid.in_domain.reshape(g, g, g), flags)
+    # voxel centre classes follow the mask
+    assert np.array_equal(grid.centre.reshape(g, g, g), np.where(mask, 2, 0))
+
+
+def _tri_box_overlap(t, blo, bhi, eps):
+    """Closed triangle / closed box overlap by the separating axis theorem
+    (box axes, triangle normal, the 9 edge cross products)."""
+    c = (blo + bhi) / 2
+    h = (bhi - blo) / 2
+    v = t - c
+    e = [v[1] - v[0], v[2] - v[1], v[0] - v[2]]
+    axes = [np.eye(3)[i] for i in range(3)] + [np.cross(e[0], e[1])]
+    axes += [np.cross(np.eye(3)[i], ed) for i in range(3) for ed in e]
+    for a in axes:
+        if not np.any(a):
+            continue
+        p = v @ a
+        r = h @ np.abs(a)
+        if p.min() > r + eps or p.max() < -r - eps:
+            return False
+    return True
+
+
+def test_triangle_lists_closed_box_overlap(rpd):
+    """Each triangle is listed in exactly the voxels whose closed box meets
+    it (closed triangle / box overlap), ids ascending."""
+    g = 10
+    grid = rpd.Grid.shape(0, g, (0.5, 0.5, 0.5, 0.4))
+    vs = 1000.0 / g
+    tri = grid.points.reshape(-1, 4)[:, :3][grid.triangles.reshape(-1, 3)]
+    offs = grid.offsets
+    lo, hi = tri.min(1), tri.max(1)
+    for v in range(0, g ** 3, 3):
+        x, y, z = v % g, (v // g) % g, v // (g * g)
+        blo = np.array([x, y, z], dtype=np.float64) * vs
+        bhi = blo + vs
+        cand = np.nonzero(np.all(lo <= bhi + 1e-6, 1) & np.all(hi >= blo - 1e-6, 1))[0]
+        want = [t for t in cand if _tri_box_overlap(tri[t], blo, bhi, 1e-6)]
+        got = grid.tri_list[offs[v]:offs[v + 1]]
+        assert np.array_equal(got, want)
+
+
+def test_grid_save_load_roundtrip(rpd, tmp_path):
+    grid = rpd.Grid.shape(1, 16, (0.5, 0.5, 0.5, 0.3, 0.12))
+    path = str(tmp_path / "torus.grid")
+    grid.save(path)
+    back = rpd.Grid.load(path)
+    assert back.size == grid.size
+    assert np.array_equal(back.in_domain, grid.in_domain)
+    assert np.array_equal(back.triangles, grid.triangles)
+    assert np.array_equal(back.tri_list, grid.tri_list)
+    assert np.array_equal(back.offsets, grid.offsets)
+    assert np.allclose(back.points[:, :3], grid.points[:, :3], rtol=1e-6)
+    assert back.valid()
+    assert abs(back.volume() / grid.volume() - 1) < 1e-5
+    # centre classes of a loaded file come from its corner flags: equal to the
+    # writer's wherever a voxel holds no triangle
+    free = grid.offsets[1:] == grid.offsets[:-1]
+    assert np.array_equal(back.centre[free], grid.centre[free])
+
+
+def _splitmix(seed, count):
+    mask = (1 << 64) - 1
+    s = seed
+    out = []
+    for _ in range(count):
+        s = (s + 0x9E3779B97F4A7C15) & mask
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def test_seed_generators(rpd):
+    n = 50
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=42, wmax=3.5)
+    u = np.array([(v >> 11) * 2.0 ** -53 for v in _splitmix(42, 4 * n)])
+    assert np.array_equal(x[:, :3].reshape(-1), 1000.0 * u[: 3 * n])
+    assert np.array_equal(x[:, 3], 3.5 * u[3 * n:])
+    m = 5
+    lat = rpd.generate_seeds(rpd.SEEDS_LATTICE, 100, seed=0, p0=m, p1=0.0)
+    k, j, i = np.unravel_index(np.arange(100), (m, m, m))
+    assert np.array_equal(lat[:, 0], 1000.0 * ((i + 0.5) / m))
+    assert np.array_equal(lat[:, 2], 1000.0 * ((k + 0.5) / m))
+    bn = rpd.generate_seeds(rpd.SEEDS_BLUE_NOISE, 400, seed=1, p0=0.08)
+    d = np.linalg.norm(bn[:, None, :3] - bn[None, :, :3], axis=2) / 1000.0
+    d[np.arange(400), np.arange(400)] = 1.0
+    assert d.min() >= 0.08
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "rpd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(rpd_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+def test_abi_exports_every_declared_symbol(rpd):
+    names = _declared_functions()
+    assert len(names) >= 30
+    lib = ctypes.CDLL(rpd.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_product_library_does_not_link_the_oracle(rpd):
+    data = open(rpd.LIB_PATH, "rb").read()
+    assert b"librpd_oracle" not in data and b"orc_solve" not in data
+
+
+def test_missing_extension_fails_loudly(rpd, monkeypatch):
+    monkeypatch.setattr(rpd, "_lib", None)
+    monkeypatch.setattr(rpd, "LIB_PATH", os.path.join(ROOT, "no_such_dir", "librpd.so"))
+    with pytest.raises(Exception):
+        rpd.lib()
