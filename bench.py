@@ -14,6 +14,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -62,6 +63,82 @@ def rel_errors(got, ref, n):
             "bary_over_cell_size": rb}
 
 
+def serving(rpd, ctx, n_req, log):
+    """Serving stream: Poisson arrivals, seed counts log-uniform in [1e3, 1e6],
+    uniform seeds with w ~ U[0, 0.25 h^2], one of 4 resident domains at 96^3,
+    successive requests on one seed set with weights perturbed by
+    N(0, 0.01 h^2); mixed form, K50 P32 V48.  The arrival rate is set to 70%
+    of the throughput measured by a back-to-back pass of the same requests."""
+    rng = np.random.default_rng(1)
+    doms = [ctx.domain(rpd.Grid.shape(code, 96, prm)) for code, prm in
+            ((rpd.SHAPE_SPHERE, (0.5, 0.5, 0.5, 0.45)), (rpd.SHAPE_TORUS, (0.5, 0.5, 0.5, 0.3, 0.12)),
+             (rpd.SHAPE_CUBE, ()), (rpd.SHAPE_CUBE_MINUS_SPHERE, (0.5, 0.5, 0.5, 0.3)))]
+    n_sets = 24
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(1e6), n_sets)).astype(np.int64)
+    sets = []
+    for i, n in enumerate(sizes):
+        h = (1.0 / n) ** (1.0 / 3.0)
+        sets.append((rpd.generate_seeds(rpd.SEEDS_UNIFORM, int(n), seed=1000 + i, wmax=0.25 * h * h * 1e6), h))
+    reqs = []
+    for r in range(n_req):
+        k = int(rng.integers(n_sets))
+        x, h = sets[k]
+        x = x.copy()
+        x[:, 3] = np.maximum(0.0, x[:, 3] + rng.normal(0.0, 0.01 * h * h * 1e6, x.shape[0]))
+        sets[k] = (x, h)
+        reqs.append((int(rng.integers(len(doms))), x))
+    caps = dict(K=50, P=32, V=48, T=100, precision=rpd.PREC_MIXED)
+    cells = sum(x.shape[0] for _, x in reqs)
+
+    def run(rate):
+        srv = rpd.Server(ctx, max_inflight=4)
+        lat = []
+        done = threading.Event()
+
+        def poller():
+            while len(lat) < len(reqs):
+                r = srv.poll(wait=True)
+                if r is None:
+                    time.sleep(0.0005)
+                    continue
+                lat.append(r[2])
+            done.set()
+
+        th = threading.Thread(target=poller)
+        th.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+        for (d, x), gap in zip(reqs, gaps):
+            t_next += gap
+            while True:
+                now = time.perf_counter()
+                if now >= t_next:
+                    break
+                time.sleep(min(0.001, t_next - now))
+            srv.submit(doms[d], x, **caps)
+        done.wait()
+        t1 = time.perf_counter()
+        th.join()
+        srv.close()
+        return t1 - t0, np.array(lat)
+
+    run(0)  # warm the slots
+    t_b2b, _ = run(0)
+    rate = 0.7 * len(reqs) / t_b2b
+    t_str, lat = run(rate)
+    for d in doms:
+        d.close()
+    res = {"requests": len(reqs), "cells": int(cells), "arrival_rate_per_s": rate,
+           "backtoback_cells_per_s": cells / t_b2b, "stream_cells_per_s": cells / t_str,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)), "slots": 4,
+           "workload": "Poisson arrivals at 70% of back-to-back throughput; n log-uniform [1e3,1e6]; 4 domains "
+                       "@96^3; weights perturbed N(0,0.01h^2) per reuse; mixed K50 P32 V48"}
+    log("bench: serving", res)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +146,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--grid", type=int, default=128)
-    ap.add_argument("--cpu-sample", type=int, default=100_000, help="cells timed on the CPU oracle")
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000, help="cells timed on the CPU oracle")
+    ap.add_argument("--serve-requests", type=int, default=240, help="requests in the serving stream")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
@@ -129,14 +207,15 @@ def main():
                       grid.centre)
         stride = max(1, n // args.cpu_sample)
         ids = np.arange(0, n, stride, dtype=np.int32)[: args.cpu_sample]
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = max(1, min(avail, int(os.environ.get("OMP_NUM_THREADS", "16"))))
         t0 = time.perf_counter()
         ref, ref_st, _, ost = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads)
         t1 = time.perf_counter()
         cpu = {"value": float(ids.size / (t1 - t0)), "unit": "cells/s", "cores": int(threads), "kind": "port",
-               "sample": "cells of every %d-th seed (%d cells) of the 1M-seed torus workload, all 1M seeds as "
-                         "neighbours; FP64 (USE_DOUBLE) oracle with the reference pass policy; %.1f s" %
-                         (stride, ids.size, t1 - t0)}
+               "sample": "%d cells (every %d-th seed) of the 1M-seed torus workload, all seeds as neighbours: "
+                         "binning + kNN + clip + every re-queue pass of the FP64 (USE_DOUBLE) oracle on %d threads; "
+                         "%.1f s" % (ids.size, stride, threads, t1 - t0)}
         log("bench: cpu oracle", cpu)
         _, _, _, ost1 = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads, single_pass=1)
         flop = {"mixed": flops_per_cell(ost1, ids.size, True), "fp64": flops_per_cell(ost1, ids.size, False)}
@@ -160,6 +239,10 @@ def main():
     roofline_knn = {"bound": "hbm", "achieved": knn_bytes / t_knn / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": knn_bytes / t_knn / 1e9 / PEAK_HBM_GBS, "bytes_per_seed": 60 + 4 * K,
                     "launch_ms": t_knn * 1e3, "kernel": "knn_kernel<float,1>"}
+
+    serve = None
+    if not args.no_serving and args.serve_requests > 0:
+        serve = serving(rpd, ctx, args.serve_requests, log)
 
     ms_mixed = res["mixed"]["ms"]
     ms_fp64 = res["fp64"]["ms"]
@@ -189,7 +272,7 @@ def main():
         "roofline": roofline,
         "roofline_knn": roofline_knn,
         "cpu_baseline": cpu,
-        "serving": None,
+        "serving": serve,
     }
     dom.close()
     ctx.close()

@@ -169,3 +169,12 @@ def test_missing_extension_fails_loudly(rpd, monkeypatch):
     monkeypatch.setattr(rpd, "LIB_PATH", os.path.join(ROOT, "no_such_dir", "librpd.so"))
     with pytest.raises(Exception):
         rpd.lib()
+
+
+def test_cli_usage_and_errors(tmp_path):
+    import subprocess
+    cli = os.path.join(ROOT, "gpu-restricted-power-diagrams_amd", "rpd_voronoi")
+    r = subprocess.run([cli, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "--K=k" in r.stderr
+    r = subprocess.run([cli, str(tmp_path / "missing.xyz")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "could not load" in r.stderr
