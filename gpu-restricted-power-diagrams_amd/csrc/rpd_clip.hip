@@ -290,7 +290,8 @@ struct WaveCell {
 #pragma unroll
         for (int s = 0; s < S; ++s) nkeep += __popcll(valid[s] & ~rem[s]);
         if (nkeep + nb > V) return RPD_VERTEX_OVERFLOW;
-        // stage new triples (plane, a, b) and the boundary successor map
+        // stage new triples (plane, a, b) and (unfiltered forms) the
+        // boundary successor map
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             if (bm[s]) {
@@ -299,51 +300,60 @@ struct WaveCell {
                 int o = obase[s];
                 if (bm[s] & 1u) {
                     nvs[o++] = pack3(uint32_t(pid), v0, v1);
-                    nxt[v0] = uint8_t(v1);
+                    if (!FILTER) nxt[v0] = uint8_t(v1);
                 }
                 if (bm[s] & 2u) {
                     nvs[o++] = pack3(uint32_t(pid), v1, v2);
-                    nxt[v1] = uint8_t(v2);
+                    if (!FILTER) nxt[v1] = uint8_t(v2);
                 }
                 if (bm[s] & 4u) {
                     nvs[o++] = pack3(uint32_t(pid), v2, v0);
-                    nxt[v2] = uint8_t(v0);
+                    if (!FILTER) nxt[v2] = uint8_t(v0);
                 }
             }
         }
         wsync();
-        // one simple cycle: every start plane written once, and the walk
-        // from one start closes after exactly nb steps
-        bool bad = false;
+        // The cavity must be a disk: its boundary one simple cycle.  With
+        // certified signs (FILTER) it always is: the removed vertices are
+        // those of a convex polytope strictly on one side of a plane, which,
+        // like the kept ones, form a connected set, and every face meets the
+        // plane's side in one contiguous arc, so no face is pinched.  The
+        // unfiltered forms check it, as CCComputeBoundary does: every start
+        // plane written once, and the walk from one start closes after
+        // exactly nb steps.
+        bool inconsistent = false;
+        if (!FILTER) {
+            bool bad = false;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (bm[s]) {
-                const uint32_t tr = tri[s];
-                const uint32_t v0 = t0(tr), v1 = t1(tr), v2 = t2(tr);
-                if (bm[s] & 1u) bad |= nxt[v0] != v1;
-                if (bm[s] & 2u) bad |= nxt[v1] != v2;
-                if (bm[s] & 4u) bad |= nxt[v2] != v0;
+            for (int s = 0; s < S; ++s) {
+                if (bm[s]) {
+                    const uint32_t tr = tri[s];
+                    const uint32_t v0 = t0(tr), v1 = t1(tr), v2 = t2(tr);
+                    if (bm[s] & 1u) bad |= nxt[v0] != v1;
+                    if (bm[s] & 2u) bad |= nxt[v1] != v2;
+                    if (bm[s] & 4u) bad |= nxt[v2] != v0;
+                }
             }
-        }
-        bool inconsistent = ballot(bad) != 0;
-        if (!inconsistent) {
-            const uint32_t start = t1(uni(nvs[0]));
-            uint32_t cur = start;
-            int steps = 0;
-            do {
-                cur = uni(uint32_t(nxt[cur]));
-                ++steps;
-            } while (cur != start && cur != 0xffu && steps <= nb);
-            inconsistent = !(cur == start && steps == nb);
-        }
-        wsync();
+            inconsistent = ballot(bad) != 0;
+            if (!inconsistent) {
+                const uint32_t start = t1(uni(nvs[0]));
+                uint32_t cur = start;
+                int steps = 0;
+                do {
+                    cur = uni(uint32_t(nxt[cur]));
+                    ++steps;
+                } while (cur != start && cur != 0xffu && steps <= nb);
+                inconsistent = !(cur == start && steps == nb);
+            }
+            wsync();
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (bm[s]) {
-                const uint32_t tr = tri[s];
-                if (bm[s] & 1u) nxt[t0(tr)] = 0xff;
-                if (bm[s] & 2u) nxt[t1(tr)] = 0xff;
-                if (bm[s] & 4u) nxt[t2(tr)] = 0xff;
+            for (int s = 0; s < S; ++s) {
+                if (bm[s]) {
+                    const uint32_t tr = tri[s];
+                    if (bm[s] & 1u) nxt[t0(tr)] = 0xff;
+                    if (bm[s] & 2u) nxt[t1(tr)] = 0xff;
+                    if (bm[s] & 4u) nxt[t2(tr)] = 0xff;
+                }
             }
         }
         if (inconsistent) return RPD_INCONSISTENT_BOUNDARY;
@@ -1117,9 +1127,12 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
 
 using namespace rpd;
 
+// Bytes of one stored plane: the planes of the FP64 and mixed forms are FP64.
+static size_t plane_bytes(int form) { return form == RPD_PREC_FP32 ? 16 : 32; }
+
 size_t rpd_clip_record_bytes(int form, int P, int V) {
     const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
-    const size_t pl = form == RPD_PREC_FP32 ? 16 : 32;
+    const size_t pl = plane_bytes(form);
     return (sizeof(RecHeader) + size_t(P) * pl + 4 * 64 * size_t(S) + 15) & ~size_t(15);
 }
 
@@ -1166,7 +1179,7 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
 
     // kernel A: the bisector loop of every cell
     {
-        const size_t stride = finish_wave_lds(L.P, S, 0, form == RPD_PREC_FP32 ? 16 : 32);
+        const size_t stride = finish_wave_lds(L.P, S, 0, plane_bytes(form));
         a.lds_stride = int(stride);
         const int wpb = 4;
         const unsigned blocks = unsigned((L.n_ids + wpb - 1) / wpb);
@@ -1199,7 +1212,7 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
     // kernel B: restriction + integration of the queued cells.  Short queues
     // (re-queue passes) spread each cell over W waves.
     {
-        const size_t pl = form == RPD_PREC_FP32 ? 16 : 32;
+        const size_t pl = plane_bytes(form);
         const size_t wstride = finish_wave_lds(L.P, S, a.T, pl);
         const size_t cstride = (32 * size_t(a.T) + 8 * 8 + 15) & ~size_t(15);  // contrib[T], fails[8][2]
         constexpr size_t kLdsMax = 64 * 1024;

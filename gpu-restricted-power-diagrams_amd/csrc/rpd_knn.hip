@@ -303,10 +303,25 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
         refresh_kth();
     };
 
+    // Cells whose box lies farther from the query than the current K-th
+    // distance cannot hold a closer candidate, so from the second shell on,
+    // rows and the x range of a row are cut to the ball of that radius (the
+    // K-th key only decreases).  The bound is widened by a relative 1e-5 and
+    // the boxes by kCullPad, far above the rounding of d^2 and of the cell
+    // index, so no candidate at or below the K-th key is cut.
+    constexpr double kCullPad = 0.01;
+    const double px = double(p.x), py = double(p.y), pz = double(p.z);
+    auto box_gap = [&](int c, int cq, double v) -> double {
+        if (c < cq) return fmax(0.0, v - double(c + 1) * cs - kCullPad);
+        if (c > cq) return fmax(0.0, double(c) * cs - v - kCullPad);
+        return 0.0;
+    };
     for (int r = 1;; ++r) {
         const int side = 2 * r + 1;
         const int nrows = side * side;
         const int nseg = 2 * nrows;
+        const bool cull = r >= 2 && !kth.is_inf();
+        const double k2 = cull ? kth.dist2() * (1.0 + 1e-5) : 0.0;
         for (int chunk = 0; chunk < nseg; chunk += 64) {
             const int u = chunk + lane;
             int sa = 0, sb = 0;
@@ -323,6 +338,18 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
                         xhi = part == 0 ? min(dim - 1, ci + r) : 0;
                     } else {
                         xlo = xhi = part == 0 ? ci - r : ci + r;
+                    }
+                    if (cull) {
+                        const double gy = box_gap(y, cj, py), gz = box_gap(z, ck, pz);
+                        const double rem = k2 - gy * gy - gz * gz;
+                        if (rem < 0.0) {
+                            xlo = 1;
+                            xhi = 0;
+                        } else {
+                            const double w = sqrt(rem) + kCullPad;
+                            xlo = max(xlo, int(floor((px - w) / cs)));
+                            xhi = min(xhi, int(floor((px + w) / cs)));
+                        }
                     }
                     if (xlo >= 0 && xhi < dim && xlo <= xhi) {
                         const int row = y * dim + z * dim * dim;
