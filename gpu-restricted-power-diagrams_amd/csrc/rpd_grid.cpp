@@ -31,14 +31,65 @@ struct rpd_grid {
 };
 
 namespace {
-// Centre classes a file grid implies: a voxel that no triangle touches lies on
-// one side of the surface, so its centre has the class of its first corner
-// when that corner is not on the border.
+// Fractions of a voxel at which the restriction puts its cone apex (the
+// kernels' apex_mode 1); the voxel classes below are those of these points.
+constexpr double kApexFrac[3] = {0.530901699437495, 0.520710678118655, 0.536602540378444};
+
+// Classes of the voxels' apex points for a grid read from a file: parity of
+// the surface crossings of the ray from the point towards +x, row by row from
+// the voxels' triangle lists (a triangle the ray meets lies in the list of a
+// voxel of the row).  A row whose ray passes within rounding of a triangle
+// edge, and a point within rounding of the surface, keep class 1 (unknown).
 void derive_centre_classes(rpd_grid *gr) {
+    const int g = gr->size;
     const size_t nv = gr->in_domain.size();
+    const double vs = 1000.0 / double(g);
     gr->centre.assign(nv, 1);
-    for (size_t i = 0; i < nv; ++i)
-        if (gr->offsets[i] == gr->offsets[i + 1] && gr->in_domain[i] != 1) gr->centre[i] = gr->in_domain[i];
+    std::vector<int> tris;
+    std::vector<double> xs;
+    for (int z = 0; z < g; ++z)
+        for (int y = 0; y < g; ++y) {
+            const double py = (y + kApexFrac[1]) * vs, pz = (z + kApexFrac[2]) * vs;
+            tris.clear();
+            for (int x = 0; x < g; ++x) {
+                const size_t id = size_t(x) + size_t(y) * g + size_t(z) * g * g;
+                for (int q = gr->offsets[id]; q < gr->offsets[id + 1]; ++q) tris.push_back(gr->tri_list[q]);
+            }
+            std::sort(tris.begin(), tris.end());
+            tris.erase(std::unique(tris.begin(), tris.end()), tris.end());
+            xs.clear();
+            bool degenerate = false;
+            for (int t : tris) {
+                const double *A = &gr->points[4 * size_t(gr->triangles[3 * size_t(t)])];
+                const double *B = &gr->points[4 * size_t(gr->triangles[3 * size_t(t) + 1])];
+                const double *C = &gr->points[4 * size_t(gr->triangles[3 * size_t(t) + 2])];
+                const double d = (B[1] - A[1]) * (C[2] - A[2]) - (B[2] - A[2]) * (C[1] - A[1]);
+                const double scale = std::fabs(B[1] - A[1]) + std::fabs(C[1] - A[1]) + std::fabs(B[2] - A[2]) +
+                                     std::fabs(C[2] - A[2]);
+                if (std::fabs(d) <= 1e-14 * scale * scale) continue;  // parallel to the ray
+                const double l1 = ((B[1] - py) * (C[2] - pz) - (B[2] - pz) * (C[1] - py)) / d;
+                const double l2 = ((C[1] - py) * (A[2] - pz) - (C[2] - pz) * (A[1] - py)) / d;
+                const double l3 = 1.0 - l1 - l2;
+                const double eps = 1e-12;
+                if (l1 < -eps || l2 < -eps || l3 < -eps) continue;
+                if (l1 < eps || l2 < eps || l3 < eps) {
+                    degenerate = true;
+                    break;
+                }
+                xs.push_back(l1 * A[0] + l2 * B[0] + l3 * C[0]);
+            }
+            if (degenerate) continue;
+            std::sort(xs.begin(), xs.end());
+            for (int x = 0; x < g; ++x) {
+                const double px = (x + kApexFrac[0]) * vs;
+                const auto it = std::upper_bound(xs.begin(), xs.end(), px);
+                const bool near = (it != xs.end() && *it - px < 1e-9 * vs) ||
+                                  (it != xs.begin() && px - *(it - 1) < 1e-9 * vs);
+                if (near) continue;
+                const size_t above = size_t(xs.end() - it);
+                gr->centre[size_t(x) + size_t(y) * g + size_t(z) * g * g] = (above & 1) ? 2 : 0;
+            }
+        }
 }
 }  // namespace
 

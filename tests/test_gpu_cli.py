@@ -40,6 +40,8 @@ def test_cli_mg_output_matches_binding(rpd, ctx, tmp_path):
     ref = ctx.moments()
     dom.close()
     ref[ref[:, 3] <= 0.001] = 0.0
+    d = np.abs(mg - ref)
+    print("cli vs binding: max abs diff", d.max(axis=0), "cells differing", int((d.max(axis=1) > 0).sum()))
     assert np.allclose(mg, ref, rtol=1e-15, atol=1e-9)
     xyz = np.loadtxt(tmp_path / "gridGPU.xyz", skiprows=1)
     keep = ref[:, 3] >= 0.001
