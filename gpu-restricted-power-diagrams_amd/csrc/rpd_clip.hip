@@ -994,6 +994,38 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         const WaveCell<R, RP, FILTER, S> bk = c;
         const bool may_skip = c.nb_plane + 4 <= P && nvalid + 6 <= a.V;
         const double lx0 = P0x - S0.x, ly0 = P0y - S0.y, lz0 = P0z - S0.z;
+        // The cone region of a triangle (apex A, beyond the triangle) lies
+        // strictly outside one of the cell's planes when the triangle's three
+        // corners do (with a margin) and the rays from A through them do not
+        // turn back towards the plane.  The cell is inside every plane of its
+        // list (each one cut it when added), so the two are disjoint: the
+        // reference's clip ends empty_cell (same caps guard as above).  The
+        // planes are tested one per lane.
+        const double face_margin = sizeof(RP) == 8 ? 1e-7 : 1e-3;
+        auto cone_beyond_a_face = [&](int tid) -> bool {
+            double px[3], py[3], pz[3];
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const double4 G = a.gpts[a.tris[3 * tid + v]];
+                px[v] = G.x - S0.x;
+                py[v] = G.y - S0.y;
+                pz[v] = G.z - S0.z;
+            }
+            bool sep = false;
+            for (int j = lane; j < c.nb_plane; j += 64) {
+                const Pl<RP> q = c.planes[j];
+                const double nx = double(q.x), ny = double(q.y), nz = double(q.z), nw = double(q.w);
+                const double m = face_margin * (1.0 + fabs(nw));
+                bool out = true;
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    out = out && nx * px[v] + ny * py[v] + nz * pz[v] + nw < -m;
+                    out = out && nx * (px[v] - lx0) + ny * (py[v] - ly0) + nz * (pz[v] - lz0) <= 0.0;
+                }
+                sep = sep || out;
+            }
+            return ballot(sep) != 0;
+        };
         for (int jb = sub; jb < ntri; jb += 16 * W) {
             // cone planes (A,B,C), (A,D,B), (A,C,D), (C,D,B), A = apex
             // (CCClipTetFromPointsInfinite): lane = 4 * triangle + plane
@@ -1065,7 +1097,8 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                 double4 ck = make_double4(0.0, 0.0, 0.0, 0.0);
                 const bool skip = tid > fail_t ||
                                   (may_skip && (c.removes_all(cvt<R>(q0)) || c.removes_all(cvt<R>(q1)) ||
-                                                c.removes_all(cvt<R>(q2)) || c.removes_all(cvt<R>(q3))));
+                                                c.removes_all(cvt<R>(q2)) || c.removes_all(cvt<R>(q3)) ||
+                                                cone_beyond_a_face(tid)));
                 if (!skip) {
                     int res = kNoClip;
                     const Pl<RP> *qs[4] = {&q0, &q1, &q2, &q3};
