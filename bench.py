@@ -95,13 +95,18 @@ def serving(rpd, ctx, n_req, log):
         lat = []
         done = threading.Event()
 
+        err = []
+
         def poller():
-            while len(lat) < len(reqs):
-                r = srv.poll(wait=True)
-                if r is None:
-                    time.sleep(0.0005)
-                    continue
-                lat.append(r[2])
+            try:
+                while len(lat) < len(reqs):
+                    r = srv.poll(wait=True)
+                    if r is None:
+                        time.sleep(0.0005)
+                        continue
+                    lat.append(r[2])
+            except Exception as e:  # surfaced by the main thread
+                err.append(e)
             done.set()
 
         th = threading.Thread(target=poller)
@@ -117,10 +122,13 @@ def serving(rpd, ctx, n_req, log):
                     break
                 time.sleep(min(0.001, t_next - now))
             srv.submit(doms[d], x, **caps)
-        done.wait()
+        if not done.wait(timeout=300):
+            raise RuntimeError("serving stream did not finish")
         t1 = time.perf_counter()
         th.join()
         srv.close()
+        if err:
+            raise err[0]
         return t1 - t0, np.array(lat)
 
     run(0)  # warm the slots

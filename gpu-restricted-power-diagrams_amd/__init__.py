@@ -9,6 +9,7 @@ library or a GPU is missing the calls raise -- there is no CPU fallback.
 """
 import ctypes
 import os
+import threading
 
 import numpy as np
 
@@ -370,19 +371,28 @@ class Server:
         _check(lib().rpd_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)), None, "rpd_server_create")
         self._h = h
         self._n = {}
+        self._max_n = 0
+        # submit() and poll() may run on different threads: a request's size
+        # is recorded (and the output capacity raised) under this lock before
+        # the request can finish
+        self._lock = threading.Lock()
 
     def submit(self, domain, xyzw, **kw):
         xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
         p = params(**kw)
         t = I64(0)
-        _check(lib().rpd_server_submit(self._h, domain._h if domain is not None else None, _p(xyzw), xyzw.shape[0],
-                                       ctypes.byref(p), ctypes.byref(t)), None, "rpd_server_submit")
-        self._n[t.value] = xyzw.shape[0]
+        with self._lock:
+            self._max_n = max(self._max_n, xyzw.shape[0])
+            _check(lib().rpd_server_submit(self._h, domain._h if domain is not None else None, _p(xyzw),
+                                           xyzw.shape[0], ctypes.byref(p), ctypes.byref(t)), None,
+                   "rpd_server_submit")
+            self._n[t.value] = xyzw.shape[0]
         return t.value
 
     def poll(self, wait=True):
         """(ticket, moments[n,4], latency_ms) of a finished request, or None."""
-        cap = max(self._n.values()) if self._n else 0
+        with self._lock:
+            cap = self._max_n
         out = np.zeros((max(cap, 1), 4), np.float64)
         t = I64(-1)
         lat = ctypes.c_double(0)
@@ -390,7 +400,8 @@ class Server:
         if t.value < 0:
             _check(rc, None, "rpd_server_poll")
             return None
-        n = self._n.pop(t.value)
+        with self._lock:
+            n = self._n.pop(t.value)
         _check(rc, None, "rpd_server_poll (ticket %d)" % t.value)
         return t.value, out[:n].copy(), lat.value
 
