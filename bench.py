@@ -32,19 +32,48 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def flops_per_cell(st, n_cells, mixed):
-    """Algorithmic flops per cell from the oracle's event counters:
-    F = c_pred*n_pred + 50*n_radius_vertex + 25*n_planes + 41*n_bbox_vertex
-        + 420*n_integrated_vertex   (op counts of ConvexCell.cl:79-97,
-    416-448, 344-373, 314-328, 930-935, 577-619).  Returns (fp32, fp64)."""
+def kernel_flops(st, n_cells, mixed):
+    """Algorithmic flops per cell of the two clip kernels, from the oracle's
+    event counters on the same inputs (single pass):
+        F = c_pred*n_pred + 50*n_radius_vertex + 25*n_planes + 41*n_bbox_vertex
+            + 420*n_integrated_vertex
+    (op counts of ConvexCell.cl:79-97, 416-448, 344-373, 314-328, 930-935,
+    577-619; c_pred = 109 for the filtered FP32 test, 45 for FP64).  The
+    bisector kernel does the predicates, radius checks and bisector planes of
+    the neighbour loop; the finish kernel the cone-plane predicates and
+    planes, the bbox and the integration.  Returns {kernel: (fp32, fp64)}."""
+    pred_a = st["n_pred"] - st["n_pred_restrict"]
+    pred_b = st["n_pred_restrict"]
+    planes_a = st["n_planes"] - st["n_planes_restrict"]
+    planes_b = st["n_planes_restrict"]
     if mixed:
-        f32 = 109.0 * st["n_pred"] + 50.0 * st["n_radius_vertex"]
-        f64 = 25.0 * st["n_planes"] + 41.0 * st["n_bbox_vertex"] + 420.0 * st["n_integrated_vertex"]
+        a = (109.0 * pred_a + 50.0 * st["n_radius_vertex"], 25.0 * planes_a)
+        b = (109.0 * pred_b, 25.0 * planes_b + 41.0 * st["n_bbox_vertex"] + 420.0 * st["n_integrated_vertex"])
     else:
-        f32 = 0.0
-        f64 = (45.0 * st["n_pred"] + 50.0 * st["n_radius_vertex"] + 25.0 * st["n_planes"] +
-               41.0 * st["n_bbox_vertex"] + 420.0 * st["n_integrated_vertex"])
-    return f32 / n_cells, f64 / n_cells
+        a = (0.0, 45.0 * pred_a + 50.0 * st["n_radius_vertex"] + 25.0 * planes_a)
+        b = (0.0, 45.0 * pred_b + 25.0 * planes_b + 41.0 * st["n_bbox_vertex"] + 420.0 * st["n_integrated_vertex"])
+    return {"cells": (a[0] / n_cells, a[1] / n_cells), "finish": (b[0] / n_cells, b[1] / n_cells)}
+
+
+def valu_roofline(f32, f64, n, seconds, kernel, traffic):
+    achieved = (f32 + f64) * n / seconds / 1e12
+    peak = (f32 + f64) / (f32 / PEAK_FP32_TFLOPS + f64 / PEAK_FP64_TFLOPS)
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": traffic, "kernel": kernel, "flops_per_cell": {"fp32": f32, "fp64": f64},
+            "launch_ms": seconds * 1e3}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel on this workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes kept in profiles/ (scaled by the counters'
+    reading for a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_1M_torus.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["bytes_per_launch"]
 
 
 def rel_errors(got, ref, n):
@@ -238,27 +267,28 @@ def main():
                          "%.1f s" % (ids.size, stride, threads, t1 - t0)}
         log("bench: cpu oracle", cpu)
         _, _, _, ost1 = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads, single_pass=1)
-        flop = {"mixed": flops_per_cell(ost1, ids.size, True), "fp64": flops_per_cell(ost1, ids.size, False)}
+        flop = {"mixed": kernel_flops(ost1, ids.size, True), "fp64": kernel_flops(ost1, ids.size, False)}
         err_vs_oracle = {k: rel_errors(res[k]["out"][ids], ref, n) for k in ("mixed", "fp64")}
         log("bench: vs oracle", err_vs_oracle)
 
-    # ---- roofline of the dominant kernel: pass-1 clip launch (mixed) ---------
+    # ---- roofline of the dominant kernel: the pass-1 bisector kernel (mixed);
+    # launch times from HIP events recorded on the solve's stream ------------
     sm = res["mixed"]["stats"]
-    t_clip = float(np.mean([s["ms_clip_first"] for s in sm])) * 1e-3
-    roofline = None
+    t_cells = float(np.mean([s["ms_cells_first"] for s in sm])) * 1e-3
+    t_finish = float(np.mean([s["ms_finish_first"] for s in sm])) * 1e-3
+    roofline = roofline_finish = None
     if flop is not None:
-        f32, f64 = flop["mixed"]
-        achieved = (f32 + f64) * n / t_clip / 1e12
-        peak = (f32 + f64) / (f32 / PEAK_FP32_TFLOPS + f64 / PEAK_FP64_TFLOPS)
-        roofline = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                    "frac": achieved / peak, "traffic": None,
-                    "kernel": "clip_kernel<float,double,true,1> (mixed form, pass 1)",
-                    "flops_per_cell": {"fp32": f32, "fp64": f64}, "launch_ms": t_clip * 1e3}
+        ka, kb = "clip_cells_kernel<float, double, true, 1>", "finish_cells_kernel<float, double, true, 1>"
+        roofline = valu_roofline(*flop["mixed"]["cells"], n, t_cells, ka + " (mixed form, pass 1)",
+                                 measured_traffic(ka))
+        roofline_finish = valu_roofline(*flop["mixed"]["finish"], n, t_finish, kb + " (mixed form, pass 1)",
+                                        measured_traffic(kb))
     t_knn = float(np.mean([s["ms_knn_first"] for s in sm])) * 1e-3
     knn_bytes = (60 + 4 * K) * n
     roofline_knn = {"bound": "hbm", "achieved": knn_bytes / t_knn / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": knn_bytes / t_knn / 1e9 / PEAK_HBM_GBS, "bytes_per_seed": 60 + 4 * K,
-                    "launch_ms": t_knn * 1e3, "kernel": "knn_kernel<float,1>"}
+                    "launch_ms": t_knn * 1e3, "kernel": "knn_kernel<float, 1>",
+                    "traffic": measured_traffic("knn_kernel<float, 1>")}
 
     serve = None
     if not args.no_serving and args.serve_requests > 0:
@@ -290,6 +320,7 @@ def main():
         "fallback_fraction": res["mixed"]["stats"][-1]["fallback_cells"] / n,
         "status_hist_mixed": res["mixed"]["stats"][-1]["status_hist"],
         "roofline": roofline,
+        "roofline_finish": roofline_finish,
         "roofline_knn": roofline_knn,
         "cpu_baseline": cpu,
         "serving": serve,

@@ -61,6 +61,8 @@ class OrcStats(ctypes.Structure):
         ("seconds_knn", ctypes.c_double),
         ("seconds_clip", ctypes.c_double),
         ("seconds_total", ctypes.c_double),
+        ("n_pred_restrict", ctypes.c_longlong),
+        ("n_planes_restrict", ctypes.c_longlong),
     ]
 
     def as_dict(self):

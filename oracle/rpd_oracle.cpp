@@ -130,8 +130,11 @@ constexpr double kApexFrac[3] = {0.530901699437495, 0.520710678118655, 0.5366025
 
 struct Counters {
     long long n_pred = 0, n_radius_vertex = 0, n_planes = 0, n_bbox_vertex = 0, n_integrated_vertex = 0,
-              n_clip = 0;
+              n_clip = 0, n_pred_restrict = 0, n_planes_restrict = 0;
+    bool restricting = false;  // the cell is in its restriction phase
     void add(const Counters &o) {
+        n_pred_restrict += o.n_pred_restrict;
+        n_planes_restrict += o.n_planes_restrict;
         n_pred += o.n_pred;
         n_radius_vertex += o.n_radius_vertex;
         n_planes += o.n_planes;
@@ -287,6 +290,7 @@ struct Cell {
     // CCVertexIsInHalfSpace :400-461
     bool in_half_space(const V3 &t, const R4<R> &e) {
         ctr->n_pred++;
+        if (ctr->restricting) ctr->n_pred_restrict++;
         const R4<R> a = planes[t[0]], b = planes[t[1]], c = planes[t[2]];
         if (vhs == 2) {
             const double d = det4<double>(double(a.x), double(b.x), double(c.x), double(e.x), double(a.y),
@@ -419,6 +423,7 @@ struct Cell {
                 return;
             }
             ctr->n_planes++;
+            ctr->n_planes_restrict++;
             planes[nb_plane++] = plane_from_points(*tri[f][0], *tri[f][1], *tri[f][2]);
             clip_by_plane(nb_plane - 1);
             if (status != kSuccess) return;
@@ -671,6 +676,11 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
         add_volume = in_domain == 2;
     }
     if (add_volume) cc.export_bary_volume(out, R(1), vol_eps2);
+    ctr->restricting = true;
+    struct PhaseEnd {
+        Counters *c;
+        ~PhaseEnd() { c->restricting = false; }
+    } phase_end{ctr};
     if (!tr.empty()) {
         // backup / restore ConvexCell.cl:692-715
         const int bk_nv = cc.nb_vertex, bk_np = cc.nb_plane, bk_st = cc.status;
@@ -1032,6 +1042,8 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
         st->n_bbox_vertex = ctr.n_bbox_vertex;
         st->n_integrated_vertex = ctr.n_integrated_vertex;
         st->n_clip = ctr.n_clip;
+        st->n_pred_restrict = ctr.n_pred_restrict;
+        st->n_planes_restrict = ctr.n_planes_restrict;
         for (int s : all) st->status_hist[status[size_t(s)]]++;
         st->passes = passes;
         st->final_K = K;

@@ -60,6 +60,8 @@ typedef struct {
     int passes;                  /* number of clip passes run                  */
     int final_K, final_P, final_V, final_T; /* caps of the last pass           */
     double seconds_knn, seconds_clip, seconds_total;
+    long long n_pred_restrict;   /* of n_pred: during the restriction          */
+    long long n_planes_restrict; /* of n_planes: cone planes                   */
 } orc_stats;
 
 /* Grid kNN.  perm[stored] = original index; knn[stored*K + i] = stored index
