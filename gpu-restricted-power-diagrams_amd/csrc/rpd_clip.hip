@@ -168,9 +168,12 @@ constexpr uint64_t kCube2 = 0x0101030003020000ull;  // 0 0 2 3 0 3 1 1
 // ---------------------------------------------------------------------------
 // The cell of one wavefront: vertices in lanes, planes in LDS.
 // ---------------------------------------------------------------------------
-template <typename R, typename RP, bool FILTER, int S>
+// The cell of one wavefront.  Planes are stored in LDS in precision PS: the
+// bisector kernel keeps them in the test precision R, the finish kernel in
+// the geometry precision RP.
+template <typename R, typename RP, bool FILTER, int S, typename PS = RP>
 struct WaveCell {
-    Pl<RP> *planes;
+    Pl<PS> *planes;
     uint32_t *rt;
     uint32_t *nvs;
     uint8_t *nxt;
@@ -225,7 +228,7 @@ struct WaveCell {
     // One clip by plane e (wave-uniform, test precision).  Lane wl holds the
     // same plane in the storage precision in eP (wl < 0: every lane does).
     // Returns kNoClip, kClipped or a failure status.
-    __device__ int clip(const Pl<R> &e, int wl, const Pl<RP> &eP) {
+    __device__ int clip(const Pl<R> &e, int wl, const Pl<PS> &eP) {
         uint64_t rem[S];
         int nr = 0, nvalid = 0;
         uint64_t flag = 0;
@@ -382,13 +385,17 @@ struct WaveCell {
     }
 };
 
-// LDS of one wave: planes[P] | rt[64S] | nvs[64S] | nxt[256] | tu[T]
-// (finish kernel) | chunk / cone planes cpl[64] | go_out flags[16]
-template <typename R, typename RP, bool FILTER, int S>
-__device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, FILTER, S> &c, int **tu = nullptr,
-                                         Pl<RP> **cpl = nullptr, int **goo = nullptr) {
-    c.planes = reinterpret_cast<Pl<RP> *>(L);
-    size_t off = (size_t(P) * sizeof(Pl<RP>) + 15) & ~size_t(15);
+// LDS of one wave: planes[P] | plane sources src[P] (int16) | rt[64S] |
+// nvs[64S] | nxt[256] | tu[T] (finish kernel) | chunk / cone planes cpl[64] |
+// go_out flags[16]
+template <typename R, typename RP, bool FILTER, int S, typename PS>
+__device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, FILTER, S, PS> &c,
+                                         int16_t **src = nullptr, int **tu = nullptr, Pl<PS> **cpl = nullptr,
+                                         int **goo = nullptr) {
+    c.planes = reinterpret_cast<Pl<PS> *>(L);
+    size_t off = (size_t(P) * sizeof(Pl<PS>) + 15) & ~size_t(15);
+    if (src) *src = reinterpret_cast<int16_t *>(L + off);
+    off += (2 * size_t(P) + 15) & ~size_t(15);
     c.rt = reinterpret_cast<uint32_t *>(L + off);
     off += 4 * 64 * S;
     c.nvs = reinterpret_cast<uint32_t *>(L + off);
@@ -398,15 +405,41 @@ __device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, 
     if (tu) *tu = reinterpret_cast<int *>(L + off);
     off += 4 * size_t(T);
     off = (off + 15) & ~size_t(15);
-    if (cpl) *cpl = reinterpret_cast<Pl<RP> *>(L + off);
-    off += 64 * sizeof(Pl<RP>);
+    if (cpl) *cpl = reinterpret_cast<Pl<PS> *>(L + off);
+    off += 64 * sizeof(Pl<PS>);
     if (goo) *goo = reinterpret_cast<int *>(L + off);
 }
 
-// LDS bytes of one wave of the finish kernel
+// LDS bytes of one wave (pl: bytes of one stored plane)
 __host__ __device__ constexpr size_t finish_wave_lds(int P, int S, int T, size_t pl) {
-    return ((((((P * pl + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) + 256 + 4 * size_t(T)) +
+    return ((((((P * pl + 15) & ~size_t(15)) + ((2 * size_t(P) + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) +
+               256 + 4 * size_t(T)) +
               15) & ~size_t(15)) + 64 * pl + 64 + 15) & ~size_t(15);
+}
+
+// Power bisector of the seed S0 and neighbour B in the seed-local frame
+// (CCNewPlane, ConvexCell.cl:314-333): unit normal -b/|b|, offset
+// d = (|b|^2 + w_B - w_s) / (2|b|), b = B - S0; a border neighbour
+// (w <= NO_CELL_LIMITS) gives the plane through B.  Evaluated without
+// contraction so that every evaluation of one plane has the same bits.
+__device__ __forceinline__ Pl<double> bisector_plane(const double4 &S0, const double4 &B, double &bl) {
+#pragma clang fp contract(off)
+    const double bx = B.x - S0.x, by = B.y - S0.y, bz = B.z - S0.z;
+    const double b2 = bx * bx + by * by + bz * bz;
+    bl = sqrt(b2);
+    const double ib = 1.0 / bl;
+    const double d = B.w <= RPD_NO_CELL_LIMITS ? bl : (b2 + (B.w - S0.w)) * (0.5 * ib);
+    return Pl<double>{-bx * ib, -by * ib, -bz * ib, d};
+}
+
+// Face f of the bounding cube [-eps, 1000 + eps]^3 in the seed-local frame
+// (CCInit, ConvexCell.cl:278-297): x >= -eps, x <= 1000 + eps, then y, z.
+__device__ __forceinline__ Pl<double> cube_plane(int f, const double4 &S0, double eps) {
+    const int ax = f >> 1;
+    const double sc = ax == 0 ? S0.x : (ax == 1 ? S0.y : S0.z);
+    const double sgn = (f & 1) ? -1.0 : 1.0;
+    const double d = (f & 1) ? (1000.0 + eps) - sc : sc + eps;
+    return Pl<double>{ax == 0 ? sgn : 0.0, ax == 1 ? sgn : 0.0, ax == 2 ? sgn : 0.0, d};
 }
 
 // Record of a finished cell in the queue: header (sid, nb_plane, nvalid,
@@ -421,9 +454,9 @@ struct RecHeader {
 // Minimum waves per SIMD the bisector kernel is compiled for (one vertex
 // slot per lane): its run time is set by latency, so occupancy pays more than
 // the few registers the compiler then spills (measured at 1M seeds: FP64 5,
-// FP32 forms 6).  Wider cells (S > 1, re-queue passes) are left unconstrained.
+// FP32 forms 7).  Wider cells (S > 1, re-queue passes) are left unconstrained.
 template <typename R, int S>
-constexpr int kClipWaves = S > 1 ? 1 : (sizeof(R) == 8 ? 5 : 6);
+constexpr int kClipWaves = S > 1 ? 1 : (sizeof(R) == 8 ? 5 : 7);
 
 template <typename R, typename RP, bool FILTER, int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kClipWaves<R, S>)))
@@ -435,9 +468,13 @@ void clip_cells_kernel(ClipArgs a) {
     const int64_t t = int64_t(blockIdx.x) * wpb + wv;
     if (t >= a.n_ids) return;  // wave-uniform
 
-    WaveCell<R, RP, FILTER, S> c;
-    Pl<RP> *chunk = nullptr;  // planes of the current 64 neighbours
-    bind_lds(lds_raw + size_t(wv) * a.lds_stride, a.P, 0, c, nullptr, &chunk);
+    // planes in the test precision; src[p] records where plane p came from
+    // (neighbour index in the list, or -1 - f for cube face f) so that the
+    // record of a queued cell carries the planes in the geometry precision
+    WaveCell<R, RP, FILTER, S, R> c;
+    Pl<R> *chunk = nullptr;  // planes of the current 64 neighbours
+    int16_t *src = nullptr;
+    bind_lds(lds_raw + size_t(wv) * a.lds_stride, a.P, 0, c, &src, nullptr, &chunk);
     c.P = a.P;
     c.V = a.V;
     c.lane = lane;
@@ -479,12 +516,8 @@ void clip_cells_kernel(ClipArgs a) {
     // ---- CCInit: bounding cube in the local frame ---------------------------
     const double eps = a.g > 0 ? 0.1 : 0.0;  // CUBE_EPSILON, voronoi.cpp:929
     if (lane < 6) {
-        const int ax = lane >> 1;
-        const double sc = ax == 0 ? S0.x : (ax == 1 ? S0.y : S0.z);
-        const double sgn = (lane & 1) ? -1.0 : 1.0;
-        const double d = (lane & 1) ? (1000.0 + eps) - sc : sc + eps;
-        c.planes[lane] = Pl<RP>{ax == 0 ? RP(sgn) : RP(0), ax == 1 ? RP(sgn) : RP(0), ax == 2 ? RP(sgn) : RP(0),
-                                RP(d)};
+        c.planes[lane] = cvt<R>(cube_plane(lane, S0, eps));
+        src[lane] = int16_t(-1 - lane);
     }
     wsync();
     c.nb_plane = 6;
@@ -538,23 +571,15 @@ void clip_cells_kernel(ClipArgs a) {
         if (base != loaded) {
             const int j = base + lane;
             cnb = j < K ? nrow[j] : -2;  // -1: list exhausted, -2: past K
-            Pl<RP> cpl{RP(0), RP(0), RP(0), RP(0)};
+            Pl<R> cpl{R(0), R(0), R(0), R(0)};
             cdl = 0;
             ct = 0;
             if (cnb >= 0) {
-                const double4 B = a.seeds[cnb];
-                const double bx = B.x - S0.x, by = B.y - S0.y, bz = B.z - S0.z;
-                const double b2 = bx * bx + by * by + bz * bz;
-                const double bl = sqrt(b2);
-                const double ib = 1.0 / bl;
-                double d;
-                if (B.w <= RPD_NO_CELL_LIMITS)
-                    d = bl;  // plane through B (ConvexCell.cl:325)
-                else
-                    d = (b2 + (B.w - S0.w)) * (0.5 * ib);
-                cpl = Pl<RP>{RP(-bx * ib), RP(-by * ib), RP(-bz * ib), RP(d)};
+                double bl;
+                const Pl<double> q = bisector_plane(S0, a.seeds[cnb], bl);
+                cpl = cvt<R>(q);
                 cdl = bl;
-                ct = float(d);
+                ct = float(q.w);
             }
             wsync();  // the previous chunk's planes are no longer read
             chunk[lane] = cpl;
@@ -602,8 +627,9 @@ void clip_cells_kernel(ClipArgs a) {
             continue;
         }
         // clip by the plane of neighbour base + f_clip
-        const Pl<RP> eP = chunk[f_clip];
-        const int res = c.clip(cvt<R>(eP), -1, eP);
+        const Pl<R> e = chunk[f_clip];
+        if (lane == 0) src[c.nb_plane] = int16_t(base + f_clip);
+        const int res = c.clip(e, -1, e);
         kused = base + f_clip + 1;
         if (res >= 0) {
             fail(res, kused);
@@ -641,7 +667,16 @@ void clip_cells_kernel(ClipArgs a) {
 #pragma unroll
     for (int s = 0; s < S; ++s) nv += __popcll(c.valid[s]);
     if (lane == 0) *reinterpret_cast<RecHeader *>(rec) = RecHeader{sid, c.nb_plane, nv, kused};
-    for (int p = lane; p < c.nb_plane; p += 64) rpl[p] = c.planes[p];
+    wsync();
+    for (int p = lane; p < c.nb_plane; p += 64) {
+        if (sizeof(R) == sizeof(RP)) {
+            rpl[p] = cvt<RP>(c.planes[p]);
+        } else {
+            const int sj = src[p];
+            double bl;
+            rpl[p] = cvt<RP>(sj < 0 ? cube_plane(-1 - sj, S0, eps) : bisector_plane(S0, a.seeds[nrow[sj]], bl));
+        }
+    }
     int carry = 0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -689,7 +724,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     WaveCell<R, RP, FILTER, S> c;
     int *tu = nullptr, *goo = nullptr;
     Pl<RP> *cpl = nullptr;
-    bind_lds(lds_raw + size_t(cpb) * a.cell_stride + size_t(wv) * a.lds_stride, a.P, a.T, c, &tu,
+    bind_lds(lds_raw + size_t(cpb) * a.cell_stride + size_t(wv) * a.lds_stride, a.P, a.T, c, nullptr, &tu,
              &cpl, &goo);
     c.P = a.P;
     c.V = a.V;
@@ -1212,7 +1247,7 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
 
     // kernel A: the bisector loop of every cell
     {
-        const size_t stride = finish_wave_lds(L.P, S, 0, plane_bytes(form));
+        const size_t stride = finish_wave_lds(L.P, S, 0, form == RPD_PREC_FP64 ? 32 : 16);  // planes in R
         a.lds_stride = int(stride);
         const int wpb = 4;
         const unsigned blocks = unsigned((L.n_ids + wpb - 1) / wpb);
