@@ -94,18 +94,9 @@ struct Key<float> {
     __device__ bool operator<(const Key &o) const { return k < o.k; }
     __device__ bool is_inf() const { return k == ~0ull; }
     __device__ double dist2() const { return double(__uint_as_float(uint32_t(k >> 32))); }
-    __device__ Key xor_shfl(int m) const {
-        Key r;
-        r.k = __shfl_xor(k, m);
-        r.s = __shfl_xor(s, m);
-        return r;
-    }
-    __device__ Key up_shfl() const {
-        Key r;
-        r.k = __shfl_up(k, 1);
-        r.s = __shfl_up(s, 1);
-        return r;
-    }
+    __device__ Key xor_shfl(int m) const { return Key{xor_lane(k, m), xor_lane(s, m)}; }
+    // key of lane - 1 (lane 0: inf)
+    __device__ Key up_shfl() const { return Key{shr_lane1(k, ~0ull), shr_lane1(s, 0xffffffffu)}; }
     __device__ Key lane(int l) const { return Key{rlane(k, l), rlane(s, l)}; }
 };
 
@@ -121,19 +112,10 @@ struct Key<double> {
     __device__ bool operator<(const Key &b) const { return d < b.d || (d == b.d && o < b.o); }
     __device__ bool is_inf() const { return d == ~0ull && o == 0xffffffffu; }
     __device__ double dist2() const { return __longlong_as_double((long long)d); }
-    __device__ Key xor_shfl(int m) const {
-        Key r;
-        r.d = __shfl_xor(d, m);
-        r.o = __shfl_xor(o, m);
-        r.s = __shfl_xor(s, m);
-        return r;
-    }
+    __device__ Key xor_shfl(int m) const { return Key{xor_lane(d, m), xor_lane(o, m), xor_lane(s, m)}; }
+    // key of lane - 1 (lane 0: inf)
     __device__ Key up_shfl() const {
-        Key r;
-        r.d = __shfl_up(d, 1);
-        r.o = __shfl_up(o, 1);
-        r.s = __shfl_up(s, 1);
-        return r;
+        return Key{shr_lane1(d, ~0ull), shr_lane1(o, 0xffffffffu), shr_lane1(s, 0xffffffffu)};
     }
     __device__ Key lane(int l) const { return Key{rlane(d, l), rlane(o, l), rlane(s, l)}; }
 };

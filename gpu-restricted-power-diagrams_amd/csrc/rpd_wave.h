@@ -118,6 +118,53 @@ __device__ __forceinline__ int wave_excl_scan(int v) {
     return wave_incl_scan_op(v, 0, [](int a, int b) { return a + b; }) - v;
 }
 
+// Value of v in lane (lane ^ J), J a power of two below 64, on the VALU
+// rather than the LDS crossbar: DPP quad permutes (J = 1, 2), row rotations
+// by 4/12 and 8 within 16-lane rows (J = 4, 8), and the gfx950 cross-row
+// swaps v_permlane16_swap / v_permlane32_swap (J = 16, 32).
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+    static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "xor_lane");
+    if constexpr (J == 1) {
+        return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0xB1, 0xf, 0xf, false));  // quad_perm 1,0,3,2
+    } else if constexpr (J == 2) {
+        return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x4E, 0xf, 0xf, false));  // quad_perm 2,3,0,1
+    } else if constexpr (J == 4) {
+        const uint32_t up = uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x12C, 0xf, 0xf, false));  // row_ror 12
+        const uint32_t dn = uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x124, 0xf, 0xf, false));  // row_ror 4
+        return (lane_id() & 4) ? dn : up;
+    } else if constexpr (J == 8) {
+        return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x128, 0xf, 0xf, false));  // row_ror 8
+    } else if constexpr (J == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane_id() & 16) ? r[0] : r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (lane_id() & 32) ? r[0] : r[1];
+    }
+}
+// same, J known only after loop unrolling
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int j) {
+    switch (j) {
+        case 1: return xor_lane<1>(v);
+        case 2: return xor_lane<2>(v);
+        case 4: return xor_lane<4>(v);
+        case 8: return xor_lane<8>(v);
+        case 16: return xor_lane<16>(v);
+        default: return xor_lane<32>(v);
+    }
+}
+__device__ __forceinline__ uint64_t xor_lane(uint64_t v, int j) {
+    return (uint64_t(xor_lane(uint32_t(v >> 32), j)) << 32) | xor_lane(uint32_t(v), j);
+}
+// Value of v in lane - 1 (DPP wave_shr:1); lane 0 gets `first`.
+__device__ __forceinline__ uint32_t shr_lane1(uint32_t v, uint32_t first) {
+    return uint32_t(__builtin_amdgcn_update_dpp(int(first), int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ uint64_t shr_lane1(uint64_t v, uint64_t first) {
+    return (uint64_t(shr_lane1(uint32_t(v >> 32), uint32_t(first >> 32))) << 32) | shr_lane1(uint32_t(v), uint32_t(first));
+}
+
 // index of the lowest set bit (m != 0)
 __device__ __forceinline__ int lowest_bit(uint64_t m) { return __ffsll((long long)m) - 1; }
 
