@@ -4,8 +4,11 @@
 // Pass policy (replaces VoroAlgo::launch, voronoi.cpp:1244-1376):
 //   pass 1: every cell, caps K/P/V/T of rpd_params, in the requested form;
 //   then up to max_extra_steps re-queue passes over the failed cells only:
-//     * vertex/plane/K/triangle overflow -> the matching cap x1.5 (as the
-//       reference);
+//     * vertex/plane overflow -> that cap x1.5 (as the reference);
+//     * security radius not reached -> K x2 (the reference: x1.5);
+//     * triangle overflow -> T = the largest triangle union the finish
+//       kernel counted, at least T x1.5 (the reference: x1.5), within the
+//       LDS of one finish-kernel cell;
 //     * needs_exact_predicates, inconsistent_boundary and
 //       find_another_beginning_vertex from an FP32 form -> recomputed (the
 //       reference drops cells flagged only for exact predicates,
@@ -434,7 +437,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         if (r != RPD_OK) return r;
         r = c->rec.ensure(size_t(n_ids) * rpd_clip_record_bytes(form, P_, V_));
         if (r != RPD_OK) return r;
-        r = c->qcount.ensure(sizeof(int));
+        r = c->qcount.ensure(2 * sizeof(int));
         if (r != RPD_OK) return r;
         hipEventRecord(c->ev[0], c->stream);
         r = rpd_launch_knn(c->stream, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids,
@@ -504,16 +507,22 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         const unsigned nb = unsigned((cur_n + 255) / 256);
         collect_failed_kernel<<<nb, 256, 0, c->stream>>>(cur_ids, cur_n, c->perm.as<int>(), c->status.as<int>(),
                                                          exact_ids, over_ids, c->cnt.as<int>());
-        int cnt[2 + RPD_STATUS_NUM];
+        int cnt[2 + RPD_STATUS_NUM], tneed = 0;
         HIPCHK(c, hipMemcpyAsync(cnt, c->cnt.ptr, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&tneed, c->qcount.as<int>() + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         const int n_exact = cnt[0], n_over = cnt[1];
         const int *h = cnt + 2;
         int K2 = K, P2 = P, V2 = V, T2 = T;
         if (h[RPD_VERTEX_OVERFLOW]) V2 = std::min(256, int(V * 1.5));
         if (h[RPD_PLANE_OVERFLOW]) P2 = std::min(255, int(P * 1.5));
-        if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) K2 = int(std::min<int64_t>(n - 1, std::min(256, int(K * 1.5))));
-        if (h[RPD_TRIANGLE_OVERFLOW]) T2 = int(T * 1.5) + 1;
+        // x2: a cell's result does not depend on K once its security radius
+        // is reached, and one wider kNN costs less than another pass
+        if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) K2 = int(std::min<int64_t>(n - 1, std::min(256, 2 * K)));
+        // the largest triangle union the finish kernel met, within what one
+        // cell of that kernel holds in LDS
+        if (h[RPD_TRIANGLE_OVERFLOW]) T2 = std::max(int(T * 1.5) + 1, tneed);
+        T2 = std::min(T2, rpd_clip_max_T(RPD_PREC_FP64, P2, V2));
         const bool caps_changed = K2 != K || P2 != P || V2 != V || T2 != T;
         // cells an FP32 form could not certify always go to FP64; cells that
         // failed in FP64 are retried only when a cap grew

@@ -84,7 +84,7 @@ struct ClipArgs {
     const uint8_t *dist;  // Chebyshev voxel distance to the nearest domain voxel (nullable)
     char *rec;            // queue of finished cells (records of rec_stride bytes)
     int rec_stride;
-    int *qcount;          // number of queued records
+    int *qcount;          // [0] number of queued records, [1] largest triangle union of a cell over T
 };
 
 __device__ __forceinline__ uint32_t pack3(uint32_t a, uint32_t b, uint32_t c) { return a | (b << 8) | (c << 16); }
@@ -936,6 +936,8 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
             }
         }
         if (nu > a.T) {
+            // the cap a re-queue pass needs for this cell
+            if (sub == 0 && lane == 0) atomicMax(a.qcount + 1, nu);
             fail(RPD_TRIANGLE_OVERFLOW);
             return;
         }
@@ -1198,6 +1200,37 @@ using namespace rpd;
 // Bytes of one stored plane: the planes of the FP64 and mixed forms are FP64.
 static size_t plane_bytes(int form) { return form == RPD_PREC_FP32 ? 16 : 32; }
 
+// LDS bytes a workgroup may use on the current device.
+static size_t lds_limit() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+        return 64 * 1024;
+    return size_t(v);
+}
+
+// LDS of one finish-kernel cell handled by a single wave: the cell's
+// contributions (T) and fail slots, then the wave's cell copy.
+static size_t finish_cell_lds(int form, int P, int V, int T) {
+    const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
+    const size_t cstride = (32 * size_t(T) + 8 * 8 + 15) & ~size_t(15);
+    return cstride + finish_wave_lds(P, S, T, plane_bytes(form));
+}
+
+int rpd_clip_max_T(int form, int P, int V) {
+    const size_t lim = lds_limit();
+    int lo = 0, hi = 1 << 20;
+    if (finish_cell_lds(form, P, V, 0) > lim) return -1;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo + 1) / 2;
+        if (finish_cell_lds(form, P, V, mid) <= lim)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 size_t rpd_clip_record_bytes(int form, int P, int V) {
     const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
     const size_t pl = plane_bytes(form);
@@ -1243,7 +1276,7 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
     a.rec_stride = int(rpd_clip_record_bytes(form, L.P, L.V));
     a.qcount = L.qcount;
     const int S = L.V <= 64 ? 1 : (L.V <= 128 ? 2 : 4);
-    if (hipMemsetAsync(L.qcount, 0, sizeof(int), st) != hipSuccess) return RPD_E_DEVICE;
+    if (hipMemsetAsync(L.qcount, 0, 2 * sizeof(int), st) != hipSuccess) return RPD_E_DEVICE;
 
     // kernel A: the bisector loop of every cell
     {
@@ -1283,7 +1316,7 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
         const size_t pl = plane_bytes(form);
         const size_t wstride = finish_wave_lds(L.P, S, a.T, pl);
         const size_t cstride = (32 * size_t(a.T) + 8 * 8 + 15) & ~size_t(15);  // contrib[T], fails[8][2]
-        constexpr size_t kLdsMax = 64 * 1024;
+        const size_t kLdsMax = lds_limit();
         if (cstride + wstride > kLdsMax) return RPD_E_LIMIT;
         a.lds_stride = int(wstride);
         a.cell_stride = int(cstride);

@@ -82,13 +82,16 @@ struct ClipLaunch {
     int *status;           // original order
     int *kused;            // original order (nullable)
     void *rec;             // n_ids * rpd_clip_record_bytes(form, P, V) bytes
-    int *qcount;           // one device int
+    int *qcount;           // two device ints: queued cells, largest triangle union over T
     hipEvent_t ev_mid;     // recorded between the two kernels (nullable)
     int *n_queued;         // host: cells handed to the finish kernel (nullable)
 };
 
 // Bytes of one queued-cell record.
 size_t rpd_clip_record_bytes(int form, int P, int V);
+// Largest T whose finish-kernel cell fits the current device's workgroup LDS
+// (-1: none).
+int rpd_clip_max_T(int form, int P, int V);
 // form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32.  Synchronizes the
 // stream once, between the bisector-loop kernel and the finish kernel.
 int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L);

@@ -69,3 +69,29 @@ def test_power_torus(rpd, orc, ctx, prec, tol, sm, cull):
     else:
         assert np.array_equal(status, ref_st)
     assert ev <= tol and em <= tol
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-6)])
+def test_coarse_seeds_fine_grid(rpd, orc, ctx, prec, tol):
+    """1000 power-weighted seeds in a sphere voxelized at 96^3: boundary cells
+    meet up to a few thousand triangles, far beyond T = 100.  The re-queue pass
+    takes T from the largest triangle union the finish kernel counted, so the
+    solve ends in at most three passes; the oracle runs with T = 4000."""
+    n = 1000
+    h = (1.0 / n) ** (1 / 3)
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=5, wmax=0.25 * h * h * 1e6)
+    grid = rpd.Grid.shape(rpd.SHAPE_SPHERE, 96, (0.5, 0.5, 0.5, 0.45))
+    ref, ref_st, _, _ = orc.solve(x, oracle_grid(orc, grid), K=50, P=32, V=48, T=4000)
+    dom = ctx.domain(grid)
+    ctx.set_seeds(x)
+    st = ctx.solve(dom, K=50, P=32, V=48, T=100, precision=prec)
+    got, status = ctx.moments(), ctx.status()
+    dom.close()
+    ev, em = moment_errors(got, ref, n)
+    print("coarse seeds", prec, st["passes"], st["pass_cells"], st["pass_caps"], ev, em, flush=True)
+    assert set(np.unique(ref_st)) <= {4, 6}
+    diff = status != ref_st
+    assert (got[diff] == 0).all() and (ref[diff] == 0).all()
+    assert set(zip(status[diff], ref_st[diff])) <= {(4, 6)}
+    assert st["passes"] <= 3
+    assert ev <= tol and em <= tol
