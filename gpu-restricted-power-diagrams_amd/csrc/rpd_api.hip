@@ -33,7 +33,7 @@
 struct rpd_domain {
     rpd_ctx *ctx = nullptr;
     DomainDev dev;
-    DevBuf flags, centre, pts, tris, tlist, toffs, tvoffs, tvox, dist;
+    DevBuf flags, centre, pts, tris, tlist, toffs, tvoffs, tvox, tinfo, dist;
     int64_t bytes = 0;
 };
 
@@ -312,8 +312,35 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
                 tvox[size_t(cur[size_t(g->tri_list[q])]++)] =
                     (uint32_t(v / (gg * gg)) << 20) | (uint32_t((v / gg) % gg) << 10) | uint32_t(v % gg);
     }
+    // per list entry of voxel v: v, and which neighbours of v that precede
+    // it in scan order also list the triangle (bit (dz+1)*9 + (dy+1)*3 +
+    // (dx+1)); bit 27 when a preceding voxel listing it is farther away
+    std::vector<uint32_t> tinfo(2 * size_t(std::max(g->num_tri_list, 0)));
+    if (r == RPD_OK) {
+        const size_t gg = size_t(g->size);
+        for (size_t v = 0; v < nv; ++v) {
+            const int vx = int(v % gg), vy = int((v / gg) % gg), vz = int(v / (gg * gg));
+            const uint32_t pv = (uint32_t(vz) << 20) | (uint32_t(vy) << 10) | uint32_t(vx);
+            for (int q = g->offsets[v]; q < g->offsets[v + 1]; ++q) {
+                const int t = g->tri_list[q];
+                uint32_t m = 0;
+                for (int u = tvoffs[size_t(t)]; u < tvoffs[size_t(t) + 1]; ++u) {
+                    const uint32_t w = tvox[size_t(u)];
+                    if (w >= pv) break;
+                    const int dx = int(w & 1023u) - vx, dy = int((w >> 10) & 1023u) - vy, dz = int(w >> 20) - vz;
+                    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1)
+                        m |= 1u << 27;
+                    else
+                        m |= 1u << ((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+                }
+                tinfo[2 * size_t(q)] = pv;
+                tinfo[2 * size_t(q) + 1] = m;
+            }
+        }
+    }
     if (r == RPD_OK) r = up(d->tvoffs, tvoffs.data(), tvoffs.size() * 4);
     if (r == RPD_OK) r = up(d->tvox, tvox.data(), tvox.size() * 4);
+    if (r == RPD_OK) r = up(d->tinfo, tinfo.data(), tinfo.size() * 4);
     std::vector<uint8_t> dist;
     domain_distance(g, dist);
     if (r == RPD_OK) r = up(d->dist, dist.data(), nv);
@@ -332,6 +359,7 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     d->dev.toffs = d->toffs.as<int>();
     d->dev.tvoffs = d->tvoffs.as<int>();
     d->dev.tvox = d->tvox.as<uint32_t>();
+    d->dev.tinfo = d->tinfo.as<uint2>();
     d->dev.dist = d->dist.as<uint8_t>();
     for (size_t i = 0; i < nv; ++i)
         if (g->in_domain[i] == 1) {

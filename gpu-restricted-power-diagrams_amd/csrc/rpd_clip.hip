@@ -79,6 +79,7 @@ struct ClipArgs {
     int cell_stride;  // LDS bytes shared by the waves of one cell (finish kernel)
     const int *tvoffs;    // per triangle: offsets into tvox
     const uint32_t *tvox; // per triangle: sorted voxels whose list holds it, z<<20 | y<<10 | x
+    const uint2 *tinfo;   // per triangle-list entry: its voxel, earlier neighbours holding it
     int security_mode;    // 0 exact power stop, 1 reference stop
     double wmin;          // minimum weight over non-border seeds
     const uint8_t *dist;  // Chebyshev voxel distance to the nearest domain voxel (nullable)
@@ -387,11 +388,11 @@ struct WaveCell {
 
 // LDS of one wave: planes[P] | plane sources src[P] (int16) | rt[64S] |
 // nvs[64S] | nxt[256] | tu[T] (finish kernel) | chunk / cone planes cpl[64] |
-// go_out flags[16]
+// go_out flags[16] | triangle corners tcor[16][9] (finish kernel)
 template <typename R, typename RP, bool FILTER, int S, typename PS>
 __device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, FILTER, S, PS> &c,
                                          int16_t **src = nullptr, int **tu = nullptr, Pl<PS> **cpl = nullptr,
-                                         int **goo = nullptr) {
+                                         int **goo = nullptr, double **tcor = nullptr) {
     c.planes = reinterpret_cast<Pl<PS> *>(L);
     size_t off = (size_t(P) * sizeof(Pl<PS>) + 15) & ~size_t(15);
     if (src) *src = reinterpret_cast<int16_t *>(L + off);
@@ -408,13 +409,16 @@ __device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, 
     if (cpl) *cpl = reinterpret_cast<Pl<PS> *>(L + off);
     off += 64 * sizeof(Pl<PS>);
     if (goo) *goo = reinterpret_cast<int *>(L + off);
+    off += 64;
+    if (tcor) *tcor = reinterpret_cast<double *>(L + off);
 }
 
-// LDS bytes of one wave (pl: bytes of one stored plane)
-__host__ __device__ constexpr size_t finish_wave_lds(int P, int S, int T, size_t pl) {
+// LDS bytes of one wave (pl: bytes of one stored plane; corners: the finish
+// kernel's triangle corners)
+__host__ __device__ constexpr size_t finish_wave_lds(int P, int S, int T, size_t pl, bool corners = false) {
     return ((((((P * pl + 15) & ~size_t(15)) + ((2 * size_t(P) + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) +
                256 + 4 * size_t(T)) +
-              15) & ~size_t(15)) + 64 * pl + 64 + 15) & ~size_t(15);
+              15) & ~size_t(15)) + 64 * pl + 64 + (corners ? 16 * 9 * 8 : 0) + 15) & ~size_t(15);
 }
 
 // Power bisector of the seed S0 and neighbour B in the seed-local frame
@@ -685,6 +689,21 @@ void clip_cells_kernel(ClipArgs a) {
     }
 }
 
+// tinfo[q].y of a triangle-list entry (voxel v): bit (dz+1)*9 + (dy+1)*3 +
+// (dx+1) set when the voxel v + (dx, dy, dz) precedes v in scan order and
+// its list holds the same triangle; kTinfoFar when such a voxel is not a
+// neighbour of v.  tinfo[q].x is v packed as z<<20 | y<<10 | x.
+constexpr uint32_t kTinfoFar = 1u << 27;
+
+// Mask of the offsets (dx, dy, dz) in {-1,0,1}^3 that keep a voxel inside the
+// box, given whether it lies above the box minimum / below its maximum per axis.
+__device__ __forceinline__ uint32_t box_neighbours(bool xl, bool xh, bool yl, bool yh, bool zl, bool zh) {
+    const uint32_t ax = (xl ? 1u : 0u) | 2u | (xh ? 4u : 0u);
+    const uint32_t y9 = (yl ? 0x7u : 0u) | 0x38u | (yh ? 0x1C0u : 0u);
+    const uint32_t z27 = (zl ? 0x1FFu : 0u) | 0x3FE00u | (zh ? 0x7FC0000u : 0u);
+    return (ax * 0x1249249u) & (y9 * 0x40201u) & z27;
+}
+
 // ---------------------------------------------------------------------------
 // Kernel B: restriction to the domain (ConvexCell.cl:918-1116) and the
 // integration (CCExportBaryVolume, ConvexCell.cl:577-619) of queued cells.
@@ -724,8 +743,9 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     WaveCell<R, RP, FILTER, S> c;
     int *tu = nullptr, *goo = nullptr;
     Pl<RP> *cpl = nullptr;
+    double *tcor = nullptr;
     bind_lds(lds_raw + size_t(cpb) * a.cell_stride + size_t(wv) * a.lds_stride, a.P, a.T, c, nullptr, &tu,
-             &cpl, &goo);
+             &cpl, &goo, &tcor);
     c.P = a.P;
     c.V = a.V;
     c.lane = lane;
@@ -878,27 +898,19 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         }
         const int nx = vmx[0] - vmn[0] + 1, ny = vmx[1] - vmn[1] + 1, nz = vmx[2] - vmn[2] + 1;
         const int nvox = nx * ny * nz;
+        // First voxel of the box, in scan order (x fastest, then y, z),
+        // whose flag is not 1 (the reference corner rule) or whose class is
+        // known (apex_mode 1); usually the first voxel already.
         int first_ref = -1, first_c = -1, ref_flag = 1, c_flag = 1;
-        // Union of the voxels' lists (merge, ConvexCell.cl:722-749): a list
-        // entry is kept in the first voxel of the bounding box, in scan order,
-        // whose list holds it -- read from the triangle's own sorted voxel
-        // list -- so each triangle of the union appears once.
-        int nu = 0;
-        const uint32_t pmn = (uint32_t(vmn[2]) << 20) | (uint32_t(vmn[1]) << 10) | uint32_t(vmn[0]);
-        const uint32_t pmx = (uint32_t(vmx[2]) << 20) | (uint32_t(vmx[1]) << 10) | uint32_t(vmx[0]);
-        for (int vb = 0; vb < nvox; vb += 64) {
+        for (int vb = 0; vb < nvox && (a.apex_mode == 1 ? first_c < 0 : first_ref < 0); vb += 64) {
             const int v = vb + lane;
             const bool in = v < nvox;
-            int f = 1, cf = 1, o0 = 0, cnt = 0;
-            uint32_t pv = 0;
+            int f = 1, cf = 1;
             if (in) {
                 const int x = vmn[0] + v % nx, y = vmn[1] + (v / nx) % ny, z = vmn[2] + v / (nx * ny);
                 const int id = x + y * g + z * g * g;
                 f = a.flags[id];
-                o0 = a.toffs[id];
-                cnt = a.toffs[id + 1] - o0;
-                cf = a.centre ? a.centre[id] : (cnt == 0 ? f : 1);
-                pv = (uint32_t(z) << 20) | (uint32_t(y) << 10) | uint32_t(x);
+                cf = a.centre ? a.centre[id] : (a.toffs[id + 1] == a.toffs[id] ? f : 1);
             }
             const uint64_t mr = ballot(in && f != 1);
             if (first_ref < 0 && mr) {
@@ -912,28 +924,75 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                 first_c = vb + l;
                 c_flag = rlane(cf, l);
             }
-            const int maxc = uni(rlane(wave_incl_scan_op(cnt, 0, [](int x, int y) { return x > y ? x : y; }), 63));
-            for (int k = 0; k < maxc; ++k) {
+        }
+        // Union of the voxels' lists (merge, ConvexCell.cl:722-749).  The
+        // list entries of one row of the box are one range of tlist; the
+        // rows' ranges are spread over the lanes.  An entry is kept in the
+        // first voxel of the box, in scan order, whose list holds its
+        // triangle: tinfo holds the entry's voxel and a mask of the
+        // neighbours of that voxel that precede it and hold the triangle
+        // too (or kTinfoFar when one of those is not a neighbour: then the
+        // triangle's own voxel list is read).  Entries are kept in scan order.
+        int nu = 0;
+        const uint32_t pmn = (uint32_t(vmn[2]) << 20) | (uint32_t(vmn[1]) << 10) | uint32_t(vmn[0]);
+        const uint32_t pmx = (uint32_t(vmx[2]) << 20) | (uint32_t(vmx[1]) << 10) | uint32_t(vmx[0]);
+        int *seg_off = reinterpret_cast<int *>(cpl), *seg_a = seg_off + 64;
+        const int nrows = ny * nz;
+        for (int rb = 0; rb < nrows; rb += 64) {
+            const int r = rb + lane;
+            int sa = 0, len = 0;
+            if (r < nrows) {
+                const int id0 = vmn[0] + (vmn[1] + r % ny) * g + (vmn[2] + r / ny) * g * g;
+                sa = a.toffs[id0];
+                len = a.toffs[id0 + nx] - sa;
+            }
+            const int off = wave_excl_scan(len);
+            const int total = uni(rlane(off + len, 63));
+            if (total == 0) continue;
+            seg_off[lane] = off;
+            seg_a[lane] = sa;
+            wsync();
+            for (int base = 0; base < total; base += 64) {
+                const int e = base + lane;
                 bool keep = false;
                 int t = -1;
-                if (k < cnt) {
-                    t = a.tlist[o0 + k];
-                    keep = true;
-                    for (int q = a.tvoffs[t], qe = a.tvoffs[t + 1]; q < qe; ++q) {
-                        const uint32_t u = a.tvox[q];
-                        if (u >= pv) break;
-                        const uint32_t ux = u & 1023u, uy = (u >> 10) & 1023u, uz = u >> 20;
-                        if (ux >= (pmn & 1023u) && ux <= (pmx & 1023u) && uy >= ((pmn >> 10) & 1023u) &&
-                            uy <= ((pmx >> 10) & 1023u) && uz >= (pmn >> 20) && uz <= (pmx >> 20)) {
-                            keep = false;  // an earlier voxel of the box holds it
-                            break;
+                if (e < total) {
+                    // last row whose offset is <= e (empty rows never win)
+                    int lo = 0, hi = 63;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (seg_off[mid] <= e)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    const int q = seg_a[lo] + (e - seg_off[lo]);
+                    t = a.tlist[q];
+                    const uint2 inf = a.tinfo[q];
+                    const uint32_t pv = inf.x;
+                    if (inf.y & kTinfoFar) {
+                        keep = true;
+                        for (int u = a.tvoffs[t], ue = a.tvoffs[t + 1]; u < ue; ++u) {
+                            const uint32_t w = a.tvox[u];
+                            if (w >= pv) break;
+                            const uint32_t ux = w & 1023u, uy = (w >> 10) & 1023u, uz = w >> 20;
+                            if (ux >= (pmn & 1023u) && ux <= (pmx & 1023u) && uy >= ((pmn >> 10) & 1023u) &&
+                                uy <= ((pmx >> 10) & 1023u) && uz >= (pmn >> 20) && uz <= (pmx >> 20)) {
+                                keep = false;  // an earlier voxel of the box holds it
+                                break;
+                            }
                         }
+                    } else {
+                        const int x = int(pv & 1023u), y = int((pv >> 10) & 1023u), z = int(pv >> 20);
+                        keep = (inf.y & box_neighbours(x > vmn[0], x < vmx[0], y > vmn[1], y < vmx[1], z > vmn[2],
+                                                       z < vmx[2])) == 0;
                     }
                 }
                 const uint64_t m = ballot(keep);
                 if (keep && nu + mbcnt(m) < a.T) tu[nu + mbcnt(m)] = t;
                 nu += __popcll(m);
             }
+            wsync();  // the next rows overwrite seg_off / seg_a
         }
         if (nu > a.T) {
             // the cap a re-queue pass needs for this cell
@@ -1039,14 +1098,13 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         // reference's clip ends empty_cell (same caps guard as above).  The
         // planes are tested one per lane.
         const double face_margin = sizeof(RP) == 8 ? 1e-7 : 1e-3;
-        auto cone_beyond_a_face = [&](int tid) -> bool {
+        auto cone_beyond_a_face = [&](int i) -> bool {
             double px[3], py[3], pz[3];
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
-                const double4 G = a.gpts[a.tris[3 * tid + v]];
-                px[v] = G.x - S0.x;
-                py[v] = G.y - S0.y;
-                pz[v] = G.z - S0.z;
+                px[v] = tcor[9 * i + 3 * v];
+                py[v] = tcor[9 * i + 3 * v + 1];
+                pz[v] = tcor[9 * i + 3 * v + 2];
             }
             bool sep = false;
             for (int j = lane; j < c.nb_plane; j += 64) {
@@ -1091,6 +1149,14 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                     const double Bx = G1.x - S0.x, By = G1.y - S0.y, Bz = G1.z - S0.z;
                     const double Cx = G2.x - S0.x, Cy = G2.y - S0.y, Cz = G2.z - S0.z;
                     const double Dx = G3.x - S0.x, Dy = G3.y - S0.y, Dz = G3.z - S0.z;
+                    if (f < 3) {  // the corners in the local frame, for cone_beyond_a_face
+                        const double cx = f == 0 ? Bx : (f == 1 ? Cx : Dx);
+                        const double cy = f == 0 ? By : (f == 1 ? Cy : Dy);
+                        const double cz = f == 0 ? Bz : (f == 1 ? Cz : Dz);
+                        tcor[9 * i + 3 * f] = cx;
+                        tcor[9 * i + 3 * f + 1] = cy;
+                        tcor[9 * i + 3 * f + 2] = cz;
+                    }
                     double px, py, pz, qx, qy, qz, rx, ry, rz;
                     if (f == 3) {
                         px = Cx, py = Cy, pz = Cz;
@@ -1135,7 +1201,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                 const bool skip = tid > fail_t ||
                                   (may_skip && (c.removes_all(cvt<R>(q0)) || c.removes_all(cvt<R>(q1)) ||
                                                 c.removes_all(cvt<R>(q2)) || c.removes_all(cvt<R>(q3)) ||
-                                                cone_beyond_a_face(tid)));
+                                                cone_beyond_a_face(i)));
                 if (!skip) {
                     int res = kNoClip;
                     const Pl<RP> *qs[4] = {&q0, &q1, &q2, &q3};
@@ -1214,7 +1280,7 @@ static size_t lds_limit() {
 static size_t finish_cell_lds(int form, int P, int V, int T) {
     const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
     const size_t cstride = (32 * size_t(T) + 8 * 8 + 15) & ~size_t(15);
-    return cstride + finish_wave_lds(P, S, T, plane_bytes(form));
+    return cstride + finish_wave_lds(P, S, T, plane_bytes(form), true);
 }
 
 int rpd_clip_max_T(int form, int P, int V) {
@@ -1265,6 +1331,7 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
         a.toffs = L.dom->toffs;
         a.tvoffs = L.dom->tvoffs;
         a.tvox = L.dom->tvox;
+        a.tinfo = L.dom->tinfo;
         a.dist = L.cull ? L.dom->dist : nullptr;
     }
     a.security_mode = L.security_mode;
@@ -1314,7 +1381,7 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
     // (re-queue passes) spread each cell over W waves.
     {
         const size_t pl = plane_bytes(form);
-        const size_t wstride = finish_wave_lds(L.P, S, a.T, pl);
+        const size_t wstride = finish_wave_lds(L.P, S, a.T, pl, true);
         const size_t cstride = (32 * size_t(a.T) + 8 * 8 + 15) & ~size_t(15);  // contrib[T], fails[8][2]
         const size_t kLdsMax = lds_limit();
         if (cstride + wstride > kLdsMax) return RPD_E_LIMIT;

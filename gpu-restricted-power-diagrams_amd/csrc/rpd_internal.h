@@ -61,6 +61,7 @@ struct DomainDev {
     const int *toffs = nullptr;
     const int *tvoffs = nullptr;     // per triangle: offsets into tvox
     const uint32_t *tvox = nullptr;  // per triangle: sorted voxels (z<<20 | y<<10 | x) whose list holds it
+    const uint2 *tinfo = nullptr;    // per triangle-list entry: its voxel, earlier neighbours holding it
     const uint8_t *dist = nullptr;   // Chebyshev voxel distance to domain voxels
     bool has_border_vertices = false;
 };
