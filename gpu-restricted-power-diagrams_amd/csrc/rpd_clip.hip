@@ -467,7 +467,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kClipWaves<
 void clip_cells_kernel(ClipArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int lane = lane_id();
-    const int wv = threadIdx.x >> 6;
+    const int wv = uni(int(threadIdx.x >> 6));  // wave-uniform: lets ids, seeds and pointers live in SGPRs
     const int wpb = blockDim.x >> 6;
     const int64_t t = int64_t(blockIdx.x) * wpb + wv;
     if (t >= a.n_ids) return;  // wave-uniform
@@ -730,7 +730,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == 1 ? 4 
 void finish_cells_kernel(ClipArgs a, int nq, int W) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int lane = lane_id();
-    const int wv = threadIdx.x >> 6;
+    const int wv = uni(int(threadIdx.x >> 6));  // wave-uniform: lets ids, seeds and pointers live in SGPRs
     const int wpb = blockDim.x >> 6;
     const int cpb = wpb / W;  // cells per block; W > 1 implies cpb == 1
     const int ci = wv / W, sub = wv - ci * W;

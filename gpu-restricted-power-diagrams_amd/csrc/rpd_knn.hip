@@ -218,7 +218,7 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
     __shared__ int seg_off_all[kKnnWavesPerBlock][64];
     __shared__ int seg_a_all[kKnnWavesPerBlock][64];
     const int lane = lane_id();
-    const int wv = threadIdx.x >> 6;
+    const int wv = uni(int(threadIdx.x >> 6));  // wave-uniform: lets ids, seeds and pointers live in SGPRs
     const int64_t qi = int64_t(blockIdx.x) * kKnnWavesPerBlock + wv;
     if (qi >= a.n_ids) return;  // wave-uniform
     int *seg_off = seg_off_all[wv];
