@@ -37,9 +37,19 @@ struct rpd_domain {
     int64_t bytes = 0;
 };
 
+// work buffers of one pass: neighbour lists, queued-cell records, queue length
+struct PassBufs {
+    DevBuf knn, rec, qcount;
+};
+// failed cells handed to the next pass: counters, then stored ids
+struct FailBufs {
+    DevBuf cnt, exact, over;
+};
+
 struct rpd_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // == streams[0]
+    hipStream_t streams[2] = {nullptr, nullptr};
     std::string err;
     int64_t n = 0;
     DevBuf seeds;  // original order, FP64 xyzw
@@ -49,10 +59,12 @@ struct rpd_ctx {
     DevBuf perm, counters, stored, stored64, bin_scratch;
     // outputs, original order
     DevBuf out, status, kused;
-    // pass work buffers
-    DevBuf knn, ids_a, ids_b, ids_c, cnt, wred, rec, qcount;
+    // pass work buffers: two passes may run at once (streams[0], streams[1])
+    PassBufs pb[2];
+    FailBufs fl[2];
+    DevBuf knn, ids, cnt, wred;
     bool solved = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[2][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
 };
 
 namespace {
@@ -84,42 +96,6 @@ int set_err(rpd_ctx *c, int code, const char *fmt, ...) {
 // failed cells of a list: compacts stored ids whose status (by original
 // index) is not success into two lists, with one atomic per wave and list,
 // and counts statuses in LDS with one atomic per block and status.
-__global__ __launch_bounds__(256) void collect_failed_kernel(const int *__restrict__ ids, int64_t n_ids,
-                                                             const int *__restrict__ perm,
-                                                             const int *__restrict__ status,
-                                                             int *__restrict__ exact_ids, int *__restrict__ over_ids,
-                                                             int *__restrict__ cnt) {
-    __shared__ int hist[RPD_STATUS_NUM];
-    if (threadIdx.x < RPD_STATUS_NUM) hist[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    int sid = -1, st = RPD_SUCCESS;
-    if (i < n_ids) {
-        sid = ids ? ids[i] : int(i);
-        st = status[perm[sid]];
-        if (st != RPD_SUCCESS) atomicAdd(&hist[st], 1);
-    }
-    const bool ex = st == RPD_NEEDS_EXACT_PREDICATES || st == RPD_INCONSISTENT_BOUNDARY ||
-                    st == RPD_FIND_ANOTHER_BEGINNING_VERTEX;
-    const bool ov = st != RPD_SUCCESS && st != RPD_EMPTY_CELL && !ex;
-    const uint64_t me = __ballot(ex), mo = __ballot(ov);
-    const int lane = threadIdx.x & 63;
-    auto rank = [&](uint64_t m) {
-        return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-    };
-    int be = 0, bo = 0;
-    if (lane == 0) {
-        if (me) be = atomicAdd(&cnt[0], __popcll(me));
-        if (mo) bo = atomicAdd(&cnt[1], __popcll(mo));
-    }
-    be = __shfl(be, 0);
-    bo = __shfl(bo, 0);
-    if (ex) exact_ids[be + rank(me)] = sid;
-    if (ov) over_ids[bo + rank(mo)] = sid;
-    __syncthreads();
-    if (threadIdx.x < RPD_STATUS_NUM && hist[threadIdx.x]) atomicAdd(&cnt[2 + threadIdx.x], hist[threadIdx.x]);
-}
-
 // per-block minimum of the weights of non-border seeds
 __global__ __launch_bounds__(256) void wmin_kernel(const double4 *__restrict__ seeds, int64_t n,
                                                    double *__restrict__ out) {
@@ -226,15 +202,18 @@ int rpd_create(int device, rpd_ctx **out) {
     if (hipSetDevice(device) != hipSuccess) return RPD_E_DEVICE;
     auto *c = new rpd_ctx();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return RPD_E_DEVICE;
-    }
-    for (auto &e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            delete c;
+    for (auto &s : c->streams)
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+            rpd_destroy(c);
             return RPD_E_DEVICE;
         }
+    c->stream = c->streams[0];
+    for (auto &es : c->ev)
+        for (auto &e : es)
+            if (hipEventCreate(&e) != hipSuccess) {
+                rpd_destroy(c);
+                return RPD_E_DEVICE;
+            }
     *out = c;
     return RPD_OK;
 }
@@ -242,10 +221,13 @@ int rpd_create(int device, rpd_ctx **out) {
 void rpd_destroy(rpd_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (auto &e : c->ev)
-        if (e) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+    for (auto &s : c->streams)
+        if (s) hipStreamSynchronize(s);
+    for (auto &es : c->ev)
+        for (auto &e : es)
+            if (e) hipEventDestroy(e);
+    for (auto &s : c->streams)
+        if (s) hipStreamDestroy(s);
     delete c;
 }
 
@@ -429,18 +411,21 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     st.n = n;
     auto t_start = std::chrono::steady_clock::now();
 
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[0][0], c->stream));
     RPDCHK(c, ensure_binning(c, kprec), "binning");
-    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[0][1], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    st.ms_bin = elapsed_ms(c->ev[0], c->ev[1]);
+    st.ms_bin = elapsed_ms(c->ev[0][0], c->ev[0][1]);
 
     RPDCHK(c, c->out.ensure(size_t(n) * 32), "alloc out");
     RPDCHK(c, c->status.ensure(size_t(n) * 4), "alloc status");
     RPDCHK(c, c->kused.ensure(size_t(n) * 4), "alloc kused");
-    RPDCHK(c, c->ids_a.ensure(size_t(n) * 4), "alloc ids");
-    RPDCHK(c, c->ids_b.ensure(size_t(n) * 4), "alloc ids");
-    RPDCHK(c, c->ids_c.ensure(size_t(n) * 4), "alloc ids");
+    RPDCHK(c, c->ids.ensure(size_t(n) * 4), "alloc ids");
+    for (auto &l : c->fl) {
+        RPDCHK(c, l.cnt.ensure(kFailCounters * sizeof(int)), "alloc failure list");
+        RPDCHK(c, l.exact.ensure(size_t(n) * 4), "alloc failure list");
+        RPDCHK(c, l.over.ensure(size_t(n) * 4), "alloc failure list");
+    }
     RPDCHK(c, c->cnt.ensure(64), "alloc cnt");
 
     const DomainDev *dd = dom ? &dom->dev : nullptr;
@@ -456,28 +441,38 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         for (double v : red) wmin = std::min(wmin, v);
         if (!(wmin < INFINITY)) wmin = 0;
     }
-    double ms_cells = 0, ms_finish = 0;
-    int n_queued = 0;
-    auto run_pass = [&](int form, const int *ids, int64_t n_ids, int K_, int P_, int V_, int T_, double &ms_knn,
-                        double &ms_clip) -> int {
-        if (n_ids <= 0) return RPD_OK;
-        int r = c->knn.ensure(size_t(n_ids) * K_ * 4);
+
+    // One pass = kNN + bisector kernel (which waits for the queue length)
+    // on a stream, then the finish kernel queued behind them.  Failed cells
+    // go to failure list fl_out; fl_cnt_host receives the list's counters
+    // as they stand once the bisector kernel is done.
+    struct Pass {
+        int slot = 0;  // stream, events and buffers
+        int form = 0;
+        int64_t n_ids = 0;
+        int caps[4] = {0, 0, 0, 0};
+        int nq = 0;
+    };
+    auto launch_pass = [&](Pass &ps, const int *ids, int fl_out, int *fl_cnt_host) -> int {
+        const int sl = ps.slot;
+        hipStream_t s = c->streams[sl];
+        PassBufs &b = c->pb[sl];
+        const int K_ = ps.caps[0], P_ = ps.caps[1], V_ = ps.caps[2], T_ = ps.caps[3];
+        int r = b.knn.ensure(size_t(ps.n_ids) * K_ * 4);
+        if (r == RPD_OK) r = b.rec.ensure(size_t(ps.n_ids) * rpd_clip_record_bytes(ps.form, P_, V_));
+        if (r == RPD_OK) r = b.qcount.ensure(sizeof(int));
         if (r != RPD_OK) return r;
-        r = c->rec.ensure(size_t(n_ids) * rpd_clip_record_bytes(form, P_, V_));
+        if (hipEventRecord(c->ev[sl][0], s) != hipSuccess) return RPD_E_DEVICE;
+        r = rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids, ps.n_ids,
+                           K_, b.knn.as<int>());
         if (r != RPD_OK) return r;
-        r = c->qcount.ensure(2 * sizeof(int));
-        if (r != RPD_OK) return r;
-        hipEventRecord(c->ev[0], c->stream);
-        r = rpd_launch_knn(c->stream, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids,
-                           n_ids, K_, c->knn.as<int>());
-        if (r != RPD_OK) return r;
-        hipEventRecord(c->ev[1], c->stream);
+        if (hipEventRecord(c->ev[sl][1], s) != hipSuccess) return RPD_E_DEVICE;
         ClipLaunch L{};
         L.seeds = c->stored64.as<double4>();
         L.perm = c->perm.as<int>();
         L.ids = ids;
-        L.knn = c->knn.as<int>();
-        L.n_ids = n_ids;
+        L.knn = b.knn.as<int>();
+        L.n_ids = ps.n_ids;
         L.n_seeds = n;
         L.K = K_;
         L.P = P_;
@@ -491,55 +486,45 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.out = c->out.as<double4>();
         L.status = c->status.as<int>();
         L.kused = c->kused.as<int>();
-        L.rec = c->rec.ptr;
-        L.qcount = c->qcount.as<int>();
-        L.ev_mid = c->ev[3];
-        L.n_queued = &n_queued;
-        r = rpd_launch_clip(c->stream, form, L);
+        L.rec = b.rec.ptr;
+        L.qcount = b.qcount.as<int>();
+        L.ev_mid = c->ev[sl][2];
+        L.fl_cnt = c->fl[fl_out].cnt.as<int>();
+        L.fl_exact = c->fl[fl_out].exact.as<int>();
+        L.fl_over = c->fl[fl_out].over.as<int>();
+        L.fl_cnt_host = fl_cnt_host;
+        r = rpd_launch_cells(s, ps.form, L, &ps.nq);
         if (r != RPD_OK) return r;
-        hipEventRecord(c->ev[2], c->stream);
-        if (hipStreamSynchronize(c->stream) != hipSuccess) return RPD_E_DEVICE;
-        const double mk = elapsed_ms(c->ev[0], c->ev[1]), mc = elapsed_ms(c->ev[1], c->ev[2]);
-        ms_knn += mk;
-        ms_clip += mc;
-        ms_cells = elapsed_ms(c->ev[1], c->ev[3]);
-        ms_finish = elapsed_ms(c->ev[3], c->ev[2]);
-        const int pi = std::min(st.passes, RPD_MAX_PASSES - 1);
-        st.pass_cells[pi] += int32_t(n_ids);
-        st.pass_caps[pi][0] = K_;
-        st.pass_caps[pi][1] = P_;
-        st.pass_caps[pi][2] = V_;
-        st.pass_caps[pi][3] = T_;
+        r = rpd_launch_finish(s, ps.form, L, ps.nq);
+        if (r != RPD_OK) return r;
+        return hipEventRecord(c->ev[sl][3], s) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+    };
+    double ms_knn = 0, ms_fb_knn = 0, ms_fb = 0;
+    auto account = [&](const Pass &ps, int index) {
+        const int sl = ps.slot;
+        const double mk = elapsed_ms(c->ev[sl][0], c->ev[sl][1]), mc = elapsed_ms(c->ev[sl][1], c->ev[sl][3]);
+        const int pi = std::min(index, RPD_MAX_PASSES - 1);
+        st.pass_cells[pi] += int32_t(ps.n_ids);
+        for (int k = 0; k < 4; ++k) st.pass_caps[pi][k] = ps.caps[k];
         st.pass_ms_knn[pi] += mk;
         st.pass_ms_clip[pi] += mc;
-        return RPD_OK;
+        if (index == 0) {
+            ms_knn = mk;
+            st.ms_knn_first = mk;
+            st.ms_clip_first = mc;
+            st.ms_cells_first = elapsed_ms(c->ev[sl][1], c->ev[sl][2]);
+            st.ms_finish_first = elapsed_ms(c->ev[sl][2], c->ev[sl][3]);
+            st.queued_first = ps.nq;
+        } else {
+            ms_fb_knn += mk;
+            ms_fb += mc;
+        }
     };
-
-    double ms_knn = 0, ms_clip = 0, ms_fb_knn = 0, ms_fb = 0;
-    RPDCHK(c, run_pass(form0, nullptr, n, K, P, V, T, ms_knn, ms_clip), "pass 1");
-    st.passes = 1;
-    st.ms_knn_first = ms_knn;
-    st.ms_clip_first = ms_clip;
-    st.ms_cells_first = ms_cells;
-    st.ms_finish_first = ms_finish;
-    st.queued_first = n_queued;
-
-    // re-queue passes: the failed cells of the last pass, in one FP64 launch
-    // per step, with the caps grown x1.5 for each overflow kind seen
-    const int *cur_ids = nullptr;
-    int64_t cur_n = n;
-    int cur_form = form0;
-    for (int step = 0; step < std::max(0, prm->max_extra_steps); ++step) {
-        int *exact_ids = c->ids_b.as<int>(), *over_ids = c->ids_c.as<int>();
-        HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));
-        const unsigned nb = unsigned((cur_n + 255) / 256);
-        collect_failed_kernel<<<nb, 256, 0, c->stream>>>(cur_ids, cur_n, c->perm.as<int>(), c->status.as<int>(),
-                                                         exact_ids, over_ids, c->cnt.as<int>());
-        int cnt[2 + RPD_STATUS_NUM], tneed = 0;
-        HIPCHK(c, hipMemcpyAsync(cnt, c->cnt.ptr, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(&tneed, c->qcount.as<int>() + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        const int n_exact = cnt[0], n_over = cnt[1];
+    // Caps of the next pass from the counters of a failure list, and the
+    // cells it takes (copied to c->ids on stream s).  Returns the cell count
+    // (0: no further pass).
+    auto next_pass = [&](const int *cnt, int fl, hipStream_t s, int caps[4]) -> int64_t {
+        const int n_exact = cnt[0], n_over = cnt[1], tneed = cnt[13];
         const int *h = cnt + 2;
         int K2 = K, P2 = P, V2 = V, T2 = T;
         if (h[RPD_VERTEX_OVERFLOW]) V2 = std::min(256, int(V * 1.5));
@@ -552,31 +537,142 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         if (h[RPD_TRIANGLE_OVERFLOW]) T2 = std::max(int(T * 1.5) + 1, tneed);
         T2 = std::min(T2, rpd_clip_max_T(RPD_PREC_FP64, P2, V2));
         const bool caps_changed = K2 != K || P2 != P || V2 != V || T2 != T;
-        // cells an FP32 form could not certify always go to FP64; cells that
-        // failed in FP64 are retried only when a cap grew
-        const bool take_exact = n_exact > 0 && (cur_form != RPD_PREC_FP64 || caps_changed);
+        // exact-predicate failures (only listed from the FP32 forms) always
+        // go to FP64; overflows only when a cap grew
         const bool take_over = n_over > 0 && caps_changed;
-        if (!take_exact && !take_over) break;
+        if (n_exact == 0 && !take_over) return 0;
         K = K2;
         P = P2;
         V = V2;
         T = T2;
-        int *dst = c->ids_a.as<int>();
-        int64_t next_n = 0;
-        if (take_exact) {
-            HIPCHK(c, hipMemcpyAsync(dst, exact_ids, size_t(n_exact) * 4, hipMemcpyDeviceToDevice, c->stream));
-            next_n += n_exact;
-            if (cur_form != RPD_PREC_FP64) st.fallback_cells += n_exact;
+        caps[0] = K;
+        caps[1] = P;
+        caps[2] = V;
+        caps[3] = T;
+        int64_t m = 0;
+        if (n_exact > 0) {
+            if (hipMemcpyAsync(c->ids.ptr, c->fl[fl].exact.ptr, size_t(n_exact) * 4, hipMemcpyDeviceToDevice, s) !=
+                hipSuccess)
+                return -1;
+            m += n_exact;
+            st.fallback_cells += n_exact;
         }
         if (take_over) {
-            HIPCHK(c, hipMemcpyAsync(dst + next_n, over_ids, size_t(n_over) * 4, hipMemcpyDeviceToDevice, c->stream));
-            next_n += n_over;
+            if (hipMemcpyAsync(c->ids.as<int>() + m, c->fl[fl].over.ptr, size_t(n_over) * 4,
+                               hipMemcpyDeviceToDevice, s) != hipSuccess)
+                return -1;
+            m += n_over;
         }
-        RPDCHK(c, run_pass(RPD_PREC_FP64, dst, next_n, K, P, V, T, ms_fb_knn, ms_fb), "re-queue pass");
+        return m;
+    };
+    auto clear_list = [&](int fl, hipStream_t s) {
+        return hipMemsetAsync(c->fl[fl].cnt.ptr, 0, kFailCounters * sizeof(int), s) == hipSuccess;
+    };
+
+    // Pass 1 on stream 0: the bisector kernel's failures go to list 0, the
+    // finish kernel's to list 1.  The first re-queue pass (the bisector
+    // failures, known as soon as that kernel is done) runs on stream 1 while
+    // pass 1's finish kernel still runs; its failures go to list 1 too.
+    // Later passes run one after another, alternating the lists.
+    if (!clear_list(0, c->stream) || !clear_list(1, c->stream)) return set_err(c, RPD_E_DEVICE, "clear lists");
+    Pass p1;
+    p1.slot = 0;
+    p1.form = form0;
+    p1.n_ids = n;
+    p1.caps[0] = K;
+    p1.caps[1] = P;
+    p1.caps[2] = V;
+    p1.caps[3] = T;
+    int cnt_a[kFailCounters] = {0}, cnt_b[kFailCounters] = {0};
+    {
+        // the bisector kernel appends to list 0 and the finish kernel to
+        // list 1: launch the two halves separately
+        hipStream_t s = c->streams[0];
+        PassBufs &b = c->pb[0];
+        RPDCHK(c, b.knn.ensure(size_t(n) * K * 4), "alloc knn");
+        RPDCHK(c, b.rec.ensure(size_t(n) * rpd_clip_record_bytes(form0, P, V)), "alloc records");
+        RPDCHK(c, b.qcount.ensure(sizeof(int)), "alloc qcount");
+        HIPCHK(c, hipEventRecord(c->ev[0][0], s));
+        RPDCHK(c,
+               rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, nullptr, n, K,
+                              b.knn.as<int>()),
+               "pass 1 knn");
+        HIPCHK(c, hipEventRecord(c->ev[0][1], s));
+        ClipLaunch L{};
+        L.seeds = c->stored64.as<double4>();
+        L.perm = c->perm.as<int>();
+        L.ids = nullptr;
+        L.knn = b.knn.as<int>();
+        L.n_ids = n;
+        L.n_seeds = n;
+        L.K = K;
+        L.P = P;
+        L.V = V;
+        L.T = T;
+        L.apex_mode = prm->apex_mode;
+        L.security_mode = prm->security_mode;
+        L.wmin = wmin;
+        L.cull = prm->cull != 0;
+        L.dom = dd;
+        L.out = c->out.as<double4>();
+        L.status = c->status.as<int>();
+        L.kused = c->kused.as<int>();
+        L.rec = b.rec.ptr;
+        L.qcount = b.qcount.as<int>();
+        L.ev_mid = c->ev[0][2];
+        L.fl_cnt = c->fl[0].cnt.as<int>();
+        L.fl_exact = c->fl[0].exact.as<int>();
+        L.fl_over = c->fl[0].over.as<int>();
+        L.fl_cnt_host = cnt_a;
+        RPDCHK(c, rpd_launch_cells(s, form0, L, &p1.nq), "pass 1 cells");
+        L.fl_cnt = c->fl[1].cnt.as<int>();
+        L.fl_exact = c->fl[1].exact.as<int>();
+        L.fl_over = c->fl[1].over.as<int>();
+        RPDCHK(c, rpd_launch_finish(s, form0, L, p1.nq), "pass 1 finish");
+        HIPCHK(c, hipEventRecord(c->ev[0][3], s));
+    }
+    st.passes = 1;
+    int steps = 0;
+    Pass p2;
+    bool have_p2 = false;
+    if (prm->max_extra_steps > 0) {
+        p2.slot = 1;
+        p2.form = RPD_PREC_FP64;
+        p2.n_ids = next_pass(cnt_a, 0, c->streams[1], p2.caps);
+        if (p2.n_ids < 0) return set_err(c, RPD_E_DEVICE, "re-queue copy");
+        if (p2.n_ids > 0) {
+            RPDCHK(c, launch_pass(p2, c->ids.as<int>(), 1, nullptr), "re-queue pass");
+            have_p2 = true;
+            st.passes++;
+            steps = 1;
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->streams[0]));
+    HIPCHK(c, hipStreamSynchronize(c->streams[1]));
+    account(p1, 0);
+    if (have_p2) account(p2, 1);
+    // counters of list 1 (pass 1 finish + first re-queue pass)
+    HIPCHK(c, hipMemcpyAsync(cnt_b, c->fl[1].cnt.ptr, sizeof(cnt_b), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int cur = 1;
+    int *cur_cnt = cnt_b, *nxt_cnt = cnt_a;
+    for (; steps < prm->max_extra_steps; ++steps) {
+        Pass ps;
+        ps.slot = 0;
+        ps.form = RPD_PREC_FP64;
+        ps.n_ids = next_pass(cur_cnt, cur, c->stream, ps.caps);
+        if (ps.n_ids < 0) return set_err(c, RPD_E_DEVICE, "re-queue copy");
+        if (ps.n_ids == 0) break;
+        const int nxt = 1 - cur;
+        if (!clear_list(nxt, c->stream)) return set_err(c, RPD_E_DEVICE, "clear list");
+        RPDCHK(c, launch_pass(ps, c->ids.as<int>(), nxt, nullptr), "re-queue pass");
+        HIPCHK(c, hipMemcpyAsync(nxt_cnt, c->fl[nxt].cnt.ptr, kFailCounters * sizeof(int), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        account(ps, st.passes);
         st.passes++;
-        cur_ids = dst;
-        cur_n = next_n;
-        cur_form = RPD_PREC_FP64;
+        cur = nxt;
+        std::swap(cur_cnt, nxt_cnt);
     }
     // final status histogram
     HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));
@@ -585,7 +681,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     HIPCHK(c, hipMemcpyAsync(st.status_hist, c->cnt.ptr, sizeof(st.status_hist), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     st.ms_knn = ms_knn + ms_fb_knn;
-    st.ms_clip = ms_clip;
+    st.ms_clip = st.ms_clip_first;
     st.ms_fallback = ms_fb;
     st.final_K = K;
     st.final_P = P;

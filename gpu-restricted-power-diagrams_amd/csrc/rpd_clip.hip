@@ -85,8 +85,27 @@ struct ClipArgs {
     const uint8_t *dist;  // Chebyshev voxel distance to the nearest domain voxel (nullable)
     char *rec;            // queue of finished cells (records of rec_stride bytes)
     int rec_stride;
-    int *qcount;          // [0] number of queued records, [1] largest triangle union of a cell over T
+    int *qcount;          // number of queued records
+    // failed cells for the next pass (nullable): fl_cnt[0] / [1] entries of
+    // fl_exact / fl_over, [2 + s] cells that ended with status s, [13] the
+    // largest triangle union of a cell that overflowed T
+    int *fl_cnt, *fl_exact, *fl_over;
+    int exact_final;      // FP64 form: exact-predicate failures are final
 };
+
+// Appends a failed cell (stored id sid) to the failure list: overflows
+// (vertex, plane, K, triangle) always; needs_exact_predicates,
+// inconsistent_boundary and find_another_beginning_vertex when the form was
+// not FP64 (in FP64 a recomputation would fail the same way).  One lane.
+__device__ __forceinline__ void note_failure(const ClipArgs &a, int sid, int st) {
+    if (!a.fl_cnt || st == RPD_SUCCESS || st == RPD_EMPTY_CELL) return;
+    const bool exact = st == RPD_NEEDS_EXACT_PREDICATES || st == RPD_INCONSISTENT_BOUNDARY ||
+                       st == RPD_FIND_ANOTHER_BEGINNING_VERTEX;
+    if (exact && a.exact_final) return;
+    atomicAdd(a.fl_cnt + 2 + st, 1);
+    const int k = atomicAdd(a.fl_cnt + (exact ? 0 : 1), 1);
+    (exact ? a.fl_exact : a.fl_over)[k] = sid;
+}
 
 __device__ __forceinline__ uint32_t pack3(uint32_t a, uint32_t b, uint32_t c) { return a | (b << 8) | (c << 16); }
 __device__ __forceinline__ uint32_t t0(uint32_t t) { return t & 0xff; }
@@ -493,6 +512,7 @@ void clip_cells_kernel(ClipArgs a) {
             a.out[orig] = make_double4(m0, m1, m2, m3);
             a.status[orig] = st;
             if (a.kused) a.kused[orig] = ku;
+            note_failure(a, sid, st);
         }
     };
     auto fail = [&](int st, int ku) { finish(st, 0.0, 0.0, 0.0, 0.0, ku); };
@@ -782,6 +802,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
             a.out[orig] = make_double4(m0, m1, m2, m3);
             a.status[orig] = st;
             if (a.kused) a.kused[orig] = kused;
+            note_failure(a, sid, st);
         }
     };
     auto fail = [&](int st) { finish(st, 0.0, 0.0, 0.0, 0.0); };
@@ -996,7 +1017,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         }
         if (nu > a.T) {
             // the cap a re-queue pass needs for this cell
-            if (sub == 0 && lane == 0) atomicMax(a.qcount + 1, nu);
+            if (sub == 0 && lane == 0 && a.fl_cnt) atomicMax(a.fl_cnt + 13, nu);
             fail(RPD_TRIANGLE_OVERFLOW);
             return;
         }
@@ -1303,12 +1324,11 @@ size_t rpd_clip_record_bytes(int form, int P, int V) {
     return (sizeof(RecHeader) + size_t(P) * pl + 4 * 64 * size_t(S) + 15) & ~size_t(15);
 }
 
-int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
-    if (L.n_ids <= 0) return RPD_OK;
+static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     if (L.P < 7 || L.P > 255 || L.V < 8 || L.K < 1) return RPD_E_ARG;
     if (L.V > 256) return RPD_E_LIMIT;
     if (!L.rec || !L.qcount) return RPD_E_ARG;
-    ClipArgs a{};
+    a = ClipArgs{};
     a.seeds = L.seeds;
     a.perm = L.perm;
     a.ids = L.ids;
@@ -1342,8 +1362,21 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
     a.rec = static_cast<char *>(L.rec);
     a.rec_stride = int(rpd_clip_record_bytes(form, L.P, L.V));
     a.qcount = L.qcount;
+    a.fl_cnt = L.fl_cnt;
+    a.fl_exact = L.fl_exact;
+    a.fl_over = L.fl_over;
+    a.exact_final = form == RPD_PREC_FP64 ? 1 : 0;
+    return RPD_OK;
+}
+
+int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out) {
+    *nq_out = 0;
+    if (L.n_ids <= 0) return RPD_OK;
+    ClipArgs a;
+    const int r = make_args(form, L, a);
+    if (r != RPD_OK) return r;
     const int S = L.V <= 64 ? 1 : (L.V <= 128 ? 2 : 4);
-    if (hipMemsetAsync(L.qcount, 0, 2 * sizeof(int), st) != hipSuccess) return RPD_E_DEVICE;
+    if (hipMemsetAsync(L.qcount, 0, sizeof(int), st) != hipSuccess) return RPD_E_DEVICE;
 
     // kernel A: the bisector loop of every cell
     {
@@ -1371,12 +1404,19 @@ int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L) {
         if (hipGetLastError() != hipSuccess) return RPD_E_DEVICE;
     }
     if (L.ev_mid && hipEventRecord(L.ev_mid, st) != hipSuccess) return RPD_E_DEVICE;
-    int nq = 0;
-    if (hipMemcpyAsync(&nq, L.qcount, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return RPD_E_DEVICE;
-    if (hipStreamSynchronize(st) != hipSuccess) return RPD_E_DEVICE;
-    if (L.n_queued) *L.n_queued = nq;
-    if (nq <= 0) return RPD_OK;
+    if (hipMemcpyAsync(nq_out, L.qcount, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return RPD_E_DEVICE;
+    if (L.fl_cnt && L.fl_cnt_host &&
+        hipMemcpyAsync(L.fl_cnt_host, L.fl_cnt, kFailCounters * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return RPD_E_DEVICE;
+    return hipStreamSynchronize(st) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+}
 
+int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq) {
+    if (nq <= 0) return RPD_OK;
+    ClipArgs a;
+    const int r = make_args(form, L, a);
+    if (r != RPD_OK) return r;
+    const int S = L.V <= 64 ? 1 : (L.V <= 128 ? 2 : 4);
     // kernel B: restriction + integration of the queued cells.  Short queues
     // (re-queue passes) spread each cell over W waves.
     {

@@ -83,16 +83,25 @@ struct ClipLaunch {
     int *status;           // original order
     int *kused;            // original order (nullable)
     void *rec;             // n_ids * rpd_clip_record_bytes(form, P, V) bytes
-    int *qcount;           // two device ints: queued cells, largest triangle union over T
-    hipEvent_t ev_mid;     // recorded between the two kernels (nullable)
-    int *n_queued;         // host: cells handed to the finish kernel (nullable)
+    int *qcount;           // one device int: queued cells
+    hipEvent_t ev_mid;     // recorded after the bisector kernel (nullable)
+    // failure list the kernels append to (nullable): kFailCounters counters
+    // ([0] exact entries, [1] overflow entries, [2 + s] cells per status s,
+    // [13] largest triangle union over T), then the stored ids
+    int *fl_cnt, *fl_exact, *fl_over;
+    int *fl_cnt_host;      // host copy of the counters, filled by rpd_launch_cells (nullable)
 };
+
+constexpr int kFailCounters = 16;
 
 // Bytes of one queued-cell record.
 size_t rpd_clip_record_bytes(int form, int P, int V);
 // Largest T whose finish-kernel cell fits the current device's workgroup LDS
 // (-1: none).
 int rpd_clip_max_T(int form, int P, int V);
-// form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32.  Synchronizes the
-// stream once, between the bisector-loop kernel and the finish kernel.
-int rpd_launch_clip(hipStream_t st, int form, const ClipLaunch &L);
+// form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32.  The bisector-loop
+// kernel of L.n_ids cells; waits for it and returns the number of cells it
+// queued (and the failure counters when L.fl_cnt_host is set).
+int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq);
+// The finish kernel over the nq records the bisector kernel queued (async).
+int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq);
