@@ -3,7 +3,8 @@
 //
 // Pass policy (replaces VoroAlgo::launch, voronoi.cpp:1244-1376):
 //   pass 1: every cell, caps K/P/V/T of rpd_params, in the requested form;
-//   then up to max_extra_steps re-queue passes over the failed cells only:
+//   then up to max_extra_steps re-queue passes over the failed cells only,
+//   which the kernels append to device lists:
 //     * vertex/plane overflow -> that cap x1.5 (as the reference);
 //     * security radius not reached -> K x2 (the reference: x1.5);
 //     * triangle overflow -> T = the largest triangle union the finish
@@ -12,9 +13,13 @@
 //     * needs_exact_predicates, inconsistent_boundary and
 //       find_another_beginning_vertex from an FP32 form -> recomputed (the
 //       reference drops cells flagged only for exact predicates,
-//       voronoi.cpp:1349-1356; here they are always recomputed);
+//       voronoi.cpp:1349-1356; here they are always recomputed); from the
+//       FP64 form they are final (a recomputation fails the same way);
 //     * empty_cell is final (volume 0);
-//   every re-queue pass runs in the FP64 form over all re-queued cells.
+//   every re-queue pass runs in the FP64 form.  The first one takes the
+//   failures of pass 1's bisector kernel and runs on a second stream while
+//   pass 1's finish kernel still runs; the failures of both go to the next
+//   pass.  A cell's result does not depend on the caps it succeeds with.
 // Boundary cells are processed in the same pass as interior cells (the
 // reference defers them with a T = 0 first pass and recomputes them).
 

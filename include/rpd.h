@@ -129,7 +129,8 @@ typedef struct rpd_domain rpd_domain;
 typedef struct {
     int32_t K, P, V, T;           /* caps of the first pass                   */
     int32_t precision;            /* enum rpd_precision                       */
-    int32_t max_extra_steps;      /* re-queue passes with caps x1.5 (ref: 5)  */
+    int32_t max_extra_steps;      /* re-queue passes over failed cells with
+                                     grown caps (ref: 5)                      */
     int32_t apex_mode;            /* 0 reference corner apex, 1 (default)
                                      centre of the first voxel of the cell's
                                      bbox whose centre class is known         */
