@@ -92,6 +92,8 @@ class Stats(ctypes.Structure):
         ("pass_caps", (ctypes.c_int32 * 4) * MAX_PASSES),
         ("pass_ms_knn", ctypes.c_double * MAX_PASSES),
         ("pass_ms_clip", ctypes.c_double * MAX_PASSES),
+        ("skipped_cells", ctypes.c_int64),
+        ("ms_cover", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -38,7 +38,7 @@
 struct rpd_domain {
     rpd_ctx *ctx = nullptr;
     DomainDev dev;
-    DevBuf flags, centre, pts, tris, tlist, toffs, tvoffs, tvox, tinfo, dist;
+    DevBuf flags, centre, pts, tris, tlist, toffs, tvoffs, tvox, tinfo, dist, dvox;
     int64_t bytes = 0;
 };
 
@@ -67,7 +67,7 @@ struct rpd_ctx {
     // pass work buffers: two passes may run at once (streams[0], streams[1])
     PassBufs pb[2];
     FailBufs fl[2];
-    DevBuf knn, ids, cnt, wred;
+    DevBuf knn, ids, ids1, cnt, wred, cull_scratch;
     bool solved = false;
     hipEvent_t ev[2][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
 };
@@ -102,21 +102,31 @@ int set_err(rpd_ctx *c, int code, const char *fmt, ...) {
 // index) is not success into two lists, with one atomic per wave and list,
 // and counts statuses in LDS with one atomic per block and status.
 // per-block minimum of the weights of non-border seeds
+// per-block minimum of the weights of non-border seeds (out[b]) and of all
+// weights (out[gridDim.x + b])
 __global__ __launch_bounds__(256) void wmin_kernel(const double4 *__restrict__ seeds, int64_t n,
                                                    double *__restrict__ out) {
-    __shared__ double red[256];
-    double m = INFINITY;
+    __shared__ double red[2][256];
+    double m = INFINITY, ma = INFINITY;
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
         const double w = seeds[i].w;
         if (w > RPD_NO_CELL_LIMITS) m = fmin(m, w);
+        ma = fmin(ma, w);
     }
-    red[threadIdx.x] = m;
+    red[0][threadIdx.x] = m;
+    red[1][threadIdx.x] = ma;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + o]);
+        if (threadIdx.x < o) {
+            red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+            red[1][threadIdx.x] = fmin(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) {
+        out[blockIdx.x] = red[0][0];
+        out[gridDim.x + blockIdx.x] = red[1][0];
+    }
 }
 
 // Chebyshev (26-neighbour BFS) voxel distance to the voxels that hold
@@ -331,6 +341,10 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     std::vector<uint8_t> dist;
     domain_distance(g, dist);
     if (r == RPD_OK) r = up(d->dist, dist.data(), nv);
+    std::vector<int> dvox;
+    for (size_t v = 0; v < nv; ++v)
+        if (dist[v] == 0) dvox.push_back(int(v));
+    if (r == RPD_OK) r = up(d->dvox, dvox.data(), dvox.size() * 4);
     if (r == RPD_OK && hipStreamSynchronize(c->stream) != hipSuccess) r = RPD_E_DEVICE;
     if (r != RPD_OK) {
         delete d;
@@ -348,6 +362,8 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     d->dev.tvox = d->tvox.as<uint32_t>();
     d->dev.tinfo = d->tinfo.as<uint2>();
     d->dev.dist = d->dist.as<uint8_t>();
+    d->dev.dvox = d->dvox.as<int>();
+    d->dev.n_dvox = int(dvox.size());
     for (size_t i = 0; i < nv; ++i)
         if (g->in_domain[i] == 1) {
             d->dev.has_border_vertices = true;
@@ -435,16 +451,52 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
 
     const DomainDev *dd = dom ? &dom->dev : nullptr;
     double wmin = 0;
+    bool border_seeds = true;
     if (prm->security_mode == 0) {
         const int nbk = 256;
-        RPDCHK(c, c->wred.ensure(nbk * sizeof(double)), "alloc wred");
+        RPDCHK(c, c->wred.ensure(2 * nbk * sizeof(double)), "alloc wred");
         wmin_kernel<<<nbk, 256, 0, c->stream>>>(c->seeds.as<double4>(), n, c->wred.as<double>());
-        double red[nbk];
+        double red[2 * nbk];
         HIPCHK(c, hipMemcpyAsync(red, c->wred.ptr, sizeof(red), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         wmin = INFINITY;
-        for (double v : red) wmin = std::min(wmin, v);
+        double wall = INFINITY;
+        for (int b = 0; b < nbk; ++b) {
+            wmin = std::min(wmin, red[b]);
+            wall = std::min(wall, red[nbk + b]);
+        }
         if (!(wmin < INFINITY)) wmin = 0;
+        border_seeds = wall <= RPD_NO_CELL_LIMITS;
+    }
+    // cells provably outside the domain skip pass 1 (exact stop, no walls)
+    const int *ids1 = nullptr;
+    int64_t n1 = n;
+    if (dd && prm->cull && prm->security_mode == 0 && !border_seeds) {
+        RPDCHK(c, c->ids1.ensure(size_t(n) * 4), "alloc ids");
+        CoverArgs ca{};
+        ca.stored64 = c->stored64.as<double4>();
+        ca.perm = c->perm.as<int>();
+        ca.counters = c->counters.as<int>();
+        ca.dim = c->dim;
+        ca.n = n;
+        ca.dvox = dd->dvox;
+        ca.n_dvox = dd->n_dvox;
+        ca.dist = dd->dist;
+        ca.g = dd->g;
+        ca.vs = dd->vs;
+        ca.wmin = wmin;
+        ca.out = c->out.as<double4>();
+        ca.status = c->status.as<int>();
+        ca.kused = c->kused.as<int>();
+        int kept = 0;
+        HIPCHK(c, hipEventRecord(c->ev[1][0], c->stream));
+        RPDCHK(c, rpd_cover_skip(c->stream, ca, c->cull_scratch, c->ids1.as<int>(), &kept), "domain cover");
+        HIPCHK(c, hipEventRecord(c->ev[1][1], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[1][1]));
+        st.ms_cover = elapsed_ms(c->ev[1][0], c->ev[1][1]);
+        ids1 = c->ids1.as<int>();
+        n1 = kept;
+        st.skipped_cells = n - kept;
     }
 
     // One pass = kNN + bisector kernel (which waits for the queue length)
@@ -583,7 +635,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     Pass p1;
     p1.slot = 0;
     p1.form = form0;
-    p1.n_ids = n;
+    p1.n_ids = n1;
     p1.caps[0] = K;
     p1.caps[1] = P;
     p1.caps[2] = V;
@@ -594,21 +646,21 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         // list 1: launch the two halves separately
         hipStream_t s = c->streams[0];
         PassBufs &b = c->pb[0];
-        RPDCHK(c, b.knn.ensure(size_t(n) * K * 4), "alloc knn");
-        RPDCHK(c, b.rec.ensure(size_t(n) * rpd_clip_record_bytes(form0, P, V)), "alloc records");
+        RPDCHK(c, b.knn.ensure(size_t(std::max<int64_t>(n1, 1)) * K * 4), "alloc knn");
+        RPDCHK(c, b.rec.ensure(size_t(std::max<int64_t>(n1, 1)) * rpd_clip_record_bytes(form0, P, V)), "alloc records");
         RPDCHK(c, b.qcount.ensure(sizeof(int)), "alloc qcount");
         HIPCHK(c, hipEventRecord(c->ev[0][0], s));
         RPDCHK(c,
-               rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, nullptr, n, K,
+               rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids1, n1, K,
                               b.knn.as<int>()),
                "pass 1 knn");
         HIPCHK(c, hipEventRecord(c->ev[0][1], s));
         ClipLaunch L{};
         L.seeds = c->stored64.as<double4>();
         L.perm = c->perm.as<int>();
-        L.ids = nullptr;
+        L.ids = ids1;
         L.knn = b.knn.as<int>();
-        L.n_ids = n;
+        L.n_ids = n1;
         L.n_seeds = n;
         L.K = K;
         L.P = P;

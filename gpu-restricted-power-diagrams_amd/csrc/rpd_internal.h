@@ -63,6 +63,8 @@ struct DomainDev {
     const uint32_t *tvox = nullptr;  // per triangle: sorted voxels (z<<20 | y<<10 | x) whose list holds it
     const uint2 *tinfo = nullptr;    // per triangle-list entry: its voxel, earlier neighbours holding it
     const uint8_t *dist = nullptr;   // Chebyshev voxel distance to domain voxels
+    const int *dvox = nullptr;       // the domain voxels (distance 0), ascending
+    int n_dvox = 0;
     bool has_border_vertices = false;
 };
 
@@ -93,6 +95,27 @@ struct ClipLaunch {
 };
 
 constexpr int kFailCounters = 16;
+
+// Cells whose restriction is provably empty, before the kNN (rpd_cull.hip).
+struct CoverArgs {
+    const double4 *stored64;  // sorted seeds, FP64
+    const int *perm;          // stored -> original
+    const int *counters;      // kNN grid: dim^3 + 1 exclusive prefix
+    int dim;
+    int64_t n;
+    const int *dvox;          // domain voxels
+    int n_dvox;
+    const uint8_t *dist;      // Chebyshev voxel distance to the domain voxels
+    int g;
+    double vs;
+    double wmin;              // minimum weight (no border seeds)
+    double4 *out;             // original order: skipped cells get 0
+    int *status;
+    int *kused;
+};
+// Writes the outputs of the skipped cells and the stored ids of the others,
+// ascending, to ids_out (n entries); *n_out = their number.  Synchronizes st.
+int rpd_cover_skip(hipStream_t st, const CoverArgs &c, DevBuf &scratch, int *ids_out, int *n_out);
 
 // Bytes of one queued-cell record.
 size_t rpd_clip_record_bytes(int form, int P, int V);

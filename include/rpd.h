@@ -164,6 +164,9 @@ typedef struct {
     int32_t pass_cells[RPD_MAX_PASSES];   /* cells entering the pass        */
     int32_t pass_caps[RPD_MAX_PASSES][4]; /* K, P, V, T of the pass          */
     double pass_ms_knn[RPD_MAX_PASSES], pass_ms_clip[RPD_MAX_PASSES];
+    int64_t skipped_cells;          /* cells proven outside the domain before
+                                       the kNN (cull = 1): volume 0          */
+    double ms_cover;                /* device time of that test              */
 } rpd_stats;
 
 /* Replaces OpenCLContext::createContext (openCL.cpp:151-207) + VoroAlgo ctor

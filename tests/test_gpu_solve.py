@@ -95,3 +95,27 @@ def test_coarse_seeds_fine_grid(rpd, orc, ctx, prec, tol):
     assert set(zip(status[diff], ref_st[diff])) <= {(4, 6)}
     assert st["passes"] <= 3
     assert ev <= tol and em <= tol
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_domain_cover_skip(rpd, ctx, prec):
+    """Seeds proven outside the domain before the kNN (cull = 1) end with
+    volume 0, and every cell agrees with the run that clips everything
+    (cull = 0)."""
+    n = 200000
+    h = (1.0 / n) ** (1 / 3)
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=7, wmax=0.25 * h * h * 1e6)
+    dom = ctx.domain(rpd.Grid.shape(rpd.SHAPE_TORUS, 64, (0.5, 0.5, 0.5, 0.3, 0.12)))
+    ctx.set_seeds(x)
+    st0 = ctx.solve(dom, K=64, P=40, V=60, T=100, precision=prec, cull=0)
+    full = ctx.moments()
+    st1 = ctx.solve(dom, K=64, P=40, V=60, T=100, precision=prec, cull=1)
+    got = ctx.moments()
+    dom.close()
+    print("cover skip", prec, st1["skipped_cells"], st1["ms_cover"], st0["ms_total"], st1["ms_total"], flush=True)
+    assert st0["skipped_cells"] == 0
+    assert st1["skipped_cells"] > n // 2
+    ev, em = moment_errors(got, full, n)
+    assert ev <= 1e-12 and em <= 1e-12
+    # the cells the cover test zeroed have (numerically) zero volume when clipped
+    assert (np.abs(full[(got == 0).all(1), 3]) <= 1e-12 * 1e9 / n).all()
