@@ -34,7 +34,8 @@ def log(*a):
 
 def kernel_flops(st, n_cells, mixed):
     """Algorithmic flops per cell of the two clip kernels, from the oracle's
-    event counters on the same inputs (single pass):
+    event counters on the same inputs (single pass, with the library's culls:
+    cells proven outside the domain are not clipped, so they add no work):
         F = c_pred*n_pred + 50*n_radius_vertex + 25*n_planes + 41*n_bbox_vertex
             + 420*n_integrated_vertex
     (op counts of ConvexCell.cl:79-97, 416-448, 344-373, 314-328, 930-935,
@@ -263,32 +264,37 @@ def main():
         t1 = time.perf_counter()
         cpu = {"value": float(ids.size / (t1 - t0)), "unit": "cells/s", "cores": int(threads), "kind": "port",
                "sample": "%d cells (every %d-th seed) of the 1M-seed torus workload, all seeds as neighbours: "
-                         "binning + kNN + clip + every re-queue pass of the FP64 (USE_DOUBLE) oracle on %d threads; "
+                         "binning + kNN + clip + every re-queue pass of the FP64 (USE_DOUBLE) oracle, the "
+                         "reference's algorithm (every cell clipped), on %d threads; "
                          "%.1f s" % (ids.size, stride, threads, t1 - t0)}
         log("bench: cpu oracle", cpu)
-        _, _, _, ost1 = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads, single_pass=1)
+        _, _, _, ost1 = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads, single_pass=1, cull=1)
         flop = {"mixed": kernel_flops(ost1, ids.size, True), "fp64": kernel_flops(ost1, ids.size, False)}
         err_vs_oracle = {k: rel_errors(res[k]["out"][ids], ref, n) for k in ("mixed", "fp64")}
         log("bench: vs oracle", err_vs_oracle)
 
-    # ---- roofline of the dominant kernel: the pass-1 bisector kernel (mixed);
-    # launch times from HIP events recorded on the solve's stream ------------
+    # ---- rooflines: the dominant kernel (pass-1 finish kernel, mixed form),
+    # the bisector kernel and the kNN; launch times from HIP events recorded on
+    # the solve's stream ------------------------------------------------------
     sm = res["mixed"]["stats"]
     t_cells = float(np.mean([s["ms_cells_first"] for s in sm])) * 1e-3
     t_finish = float(np.mean([s["ms_finish_first"] for s in sm])) * 1e-3
-    roofline = roofline_finish = None
+    roofline = roofline_cells = None
     if flop is not None:
         ka, kb = "clip_cells_kernel<float, double, true, 1>", "finish_cells_kernel<float, double, true, 1>"
-        roofline = valu_roofline(*flop["mixed"]["cells"], n, t_cells, ka + " (mixed form, pass 1)",
-                                 measured_traffic(ka))
-        roofline_finish = valu_roofline(*flop["mixed"]["finish"], n, t_finish, kb + " (mixed form, pass 1)",
-                                        measured_traffic(kb))
+        roofline_cells = valu_roofline(*flop["mixed"]["cells"], n, t_cells, ka + " (mixed form, pass 1)",
+                                       measured_traffic(ka))
+        roofline = valu_roofline(*flop["mixed"]["finish"], n, t_finish, kb + " (mixed form, pass 1)",
+                                 measured_traffic(kb))
+    # kNN launch: each query reads its seed and writes K ids; the sorted seed
+    # array (FP32 xyzw) is read at least once
     t_knn = float(np.mean([s["ms_knn_first"] for s in sm])) * 1e-3
-    knn_bytes = (60 + 4 * K) * n
+    n_query = n - int(sm[-1]["skipped_cells"])
+    knn_bytes = (16 + 4 * K) * n_query + 16 * n
     roofline_knn = {"bound": "hbm", "achieved": knn_bytes / t_knn / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": knn_bytes / t_knn / 1e9 / PEAK_HBM_GBS, "bytes_per_seed": 60 + 4 * K,
-                    "launch_ms": t_knn * 1e3, "kernel": "knn_kernel<float, 1>",
-                    "traffic": measured_traffic("knn_kernel<float, 1>")}
+                    "frac": knn_bytes / t_knn / 1e9 / PEAK_HBM_GBS, "queries": n_query,
+                    "bytes": "(16 + 4K) per query + 16 per seed", "launch_ms": t_knn * 1e3,
+                    "kernel": "knn_kernel<float, 1>", "traffic": measured_traffic("knn_kernel<float, 1>")}
 
     serve = None
     if not args.no_serving and args.serve_requests > 0:
@@ -319,8 +325,13 @@ def main():
         "err_vs_oracle_sample": err_vs_oracle,
         "fallback_fraction": res["mixed"]["stats"][-1]["fallback_cells"] / n,
         "status_hist_mixed": res["mixed"]["stats"][-1]["status_hist"],
+        "skipped_cells": int(res["mixed"]["stats"][-1]["skipped_cells"]),
+        "stage_ms": {k: {"cover": st["ms_cover"], "knn": st["ms_knn_first"], "cells": st["ms_cells_first"],
+                         "finish": st["ms_finish_first"], "pass_cells": st["pass_cells"],
+                         "pass_ms": [a + b for a, b in zip(st["pass_ms_knn"], st["pass_ms_clip"])]}
+                     for k, st in (("mixed", res["mixed"]["stats"][-1]), ("fp64", res["fp64"]["stats"][-1]))},
         "roofline": roofline,
-        "roofline_finish": roofline_finish,
+        "roofline_cells": roofline_cells,
         "roofline_knn": roofline_knn,
         "cpu_baseline": cpu,
         "serving": serve,

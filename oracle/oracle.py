@@ -41,7 +41,7 @@ class OrcGrid(ctypes.Structure):
 class OrcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "K", "P", "V", "T", "precision", "vhs0", "vhs1", "apex_mode",
-        "fix_exact_requeue", "max_extra_steps", "threads", "single_pass", "security_mode")]
+        "fix_exact_requeue", "max_extra_steps", "threads", "single_pass", "security_mode", "cull")]
 
 
 class OrcStats(ctypes.Structure):
@@ -150,11 +150,11 @@ def knn(xyzw, K, precision=1, method=0):
 
 
 def params(K=25, P=35, V=40, T=100, precision=1, vhs0=2, vhs1=2, apex_mode=1, fix_exact_requeue=1,
-           max_extra_steps=5, threads=0, single_pass=0, security_mode=0):
+           max_extra_steps=5, threads=0, single_pass=0, security_mode=0, cull=0):
     p = OrcParams()
     for k, v in dict(K=K, P=P, V=V, T=T, precision=precision, vhs0=vhs0, vhs1=vhs1, apex_mode=apex_mode,
                      fix_exact_requeue=fix_exact_requeue, max_extra_steps=max_extra_steps, threads=threads,
-                     single_pass=single_pass, security_mode=security_mode).items():
+                     single_pass=single_pass, security_mode=security_mode, cull=cull).items():
         setattr(p, k, int(v))
     return p
 

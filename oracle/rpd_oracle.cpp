@@ -264,6 +264,16 @@ struct Cell {
         return r;
     }
 
+    // largest squared distance of a vertex from the seed
+    R max_r2() const {
+        R vd = 0;
+        for (int i = 0; i < nb_vertex; ++i) {
+            const R4<R> d = sub4(vertex_coords(verts[i], true), seed);
+            vd = std::max(dot3(d, d), vd);
+        }
+        return vd;
+    }
+
     // CCIsSecurityRadiusReached :364-382 (weights ignored, as the reference)
     bool radius_reached(const R4<R> &last) const {
         R vd = 0;
@@ -504,7 +514,7 @@ inline bool merge_union(const std::vector<int> &a, const int *b, int nb, int cap
 template <class R>
 CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, int P, int V, int T, int vhs,
                            const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
-                           R4<R> *out, Counters *ctr, bool all_listed) {
+                           R4<R> *out, Counters *ctr, bool all_listed, double cull_r2 = 0) {
     CellResult<R> res;
     const R vol_eps2 = sizeof(R) == sizeof(float) ? R(1.e-6) : R(1.e-12);  // voronoi.cpp:919-922
     if (double(pts[seed].w) <= kNoCellLimits) {
@@ -532,6 +542,12 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
         cc.clip_by_plane(cur);
         if (cc.status != kSuccess) {
             res.status = cc.status;
+            res.kused = i + 1;
+            return res;
+        }
+        // cull = 1: a cell inside the ball of radius sqrt(cull_r2) about its
+        // seed, which holds no domain point, has restricted volume 0
+        if (cull_r2 > 0 && double(cc.max_r2()) * 1.001 < cull_r2) {
             res.kused = i + 1;
             return res;
         }
@@ -901,13 +917,88 @@ int knn_all(const KnnGrid<R> &kg, int K, int method, const int *ids, int nids, i
     return kg.dim;
 }
 
+// ---------------------------------------------------------------------------
+// cull = 1: the library's tests for cells whose restriction is empty.  A
+// domain voxel is one that holds triangles or whose centre class is not
+// "outside"; dist is the Chebyshev voxel distance to the nearest one (BFS over
+// the 26 neighbours, capped at 255).  A seed whose voxel lies d >= 2 voxels
+// away is at least (d - 1) vs from every domain point.
+// ---------------------------------------------------------------------------
+constexpr double kCoverRel = 1e-9, kCoverAbs = 1e-6;
+
+std::vector<uint8_t> domain_distance(const orc_grid *gr) {
+    const int g = gr->size;
+    const size_t nv = size_t(g) * g * g;
+    std::vector<uint8_t> dist(nv, 255);
+    std::vector<int> q;
+    for (size_t v = 0; v < nv; ++v) {
+        const bool tri = gr->offsets[v + 1] > gr->offsets[v];
+        const bool in = gr->centre ? gr->centre[v] != 0 : gr->in_domain[v] != 0;
+        if (tri || in) {
+            dist[v] = 0;
+            q.push_back(int(v));
+        }
+    }
+    for (size_t h = 0; h < q.size(); ++h) {
+        const int v = q[h], d = dist[size_t(v)];
+        if (d >= 254) continue;
+        const int x = v % g, y = (v / g) % g, z = v / (g * g);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int X = x + dx, Y = y + dy, Z = z + dz;
+                    if (X < 0 || Y < 0 || Z < 0 || X >= g || Y >= g || Z >= g) continue;
+                    const size_t u = size_t(X) + size_t(Y) * g + size_t(Z) * g * g;
+                    if (dist[u] == 255) {
+                        dist[u] = uint8_t(d + 1);
+                        q.push_back(int(u));
+                    }
+                }
+    }
+    return dist;
+}
+
+// Upper bound M of U(x) - w_min over the domain points x, U(x) the smallest
+// power distance of x to a seed: for each domain voxel (centre c, half
+// diagonal r) the smallest (|c - t| + r)^2 + w_t - w_min over the seeds t of
+// the 3x3x3 kNN-grid cells around c; +inf when those cells are empty.
+template <class R>
+double cover_bound(const KnnGrid<R> &kg, const std::vector<uint8_t> &dist, int g, double wmin) {
+    const double vs = 1000.0 / double(g), rv = 0.8660254037844387 * vs * (1.0 + 1e-12);
+    const int dim = kg.dim;
+    double M = 0;
+    for (size_t v = 0; v < dist.size(); ++v) {
+        if (dist[v] != 0) continue;
+        const double cx = (double(v % g) + 0.5) * vs, cy = (double((v / g) % g) + 0.5) * vs,
+                     cz = (double(v / (size_t(g) * g)) + 0.5) * vs;
+        auto cc = [&](double c) { return std::max(0, std::min(dim - 1, int(c * double(dim) / 1000.0))); };
+        const int ci = cc(cx), cj = cc(cy), ck = cc(cz);
+        double best = std::numeric_limits<double>::infinity();
+        for (int z = std::max(0, ck - 1); z <= std::min(dim - 1, ck + 1); ++z)
+            for (int y = std::max(0, cj - 1); y <= std::min(dim - 1, cj + 1); ++y) {
+                const int row = (y + z * dim) * dim;
+                for (int j = kg.counters[size_t(row + std::max(0, ci - 1))],
+                         je = kg.counters[size_t(row + std::min(dim - 1, ci + 1) + 1)];
+                     j < je; ++j) {
+                    const R4<R> t = kg.stored[size_t(j)];
+                    const double dx = double(t.x) - cx, dy = double(t.y) - cy, dz = double(t.z) - cz;
+                    const double dd = std::sqrt(dx * dx + dy * dy + dz * dz) + rv;
+                    best = std::min(best, dd * dd + (double(t.w) - wmin));
+                }
+            }
+        M = std::max(M, best * (1.0 + kCoverRel) + kCoverAbs);
+    }
+    return M;
+}
+
 // One clip pass over a list of stored ids (VoroAlgoComputeData::launch,
 // voronoi.cpp:861-1036): kNN with K, then one cell per id.
 template <class R>
 void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, int V, int T, int vhs,
                const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
                std::vector<R4<R>> &out,
-               std::vector<int> &status, std::vector<int> &kused, Counters &ctr, double &t_knn, double &t_clip) {
+               std::vector<int> &status, std::vector<int> &kused, Counters &ctr, double &t_knn, double &t_clip,
+               const std::vector<double> &cull_r2) {
     const long long nids = (long long)ids.size();
     if (nids == 0) return;
     std::vector<int> nb(size_t(nids) * size_t(K));
@@ -923,7 +1014,8 @@ void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, 
             const int s = ids[size_t(t)];
             CellResult<R> r = compute_cell<R>(kg.stored.data(), s, nb.data() + t * K, K, P, V, T, vhs, grid,
                                               apex_mode, out_domain, security_mode, wmin, &out[size_t(s)], &local,
-                                              K >= int(kg.stored.size()) - 1);
+                                              K >= int(kg.stored.size()) - 1,
+                                              cull_r2.empty() ? 0.0 : cull_r2[size_t(s)]);
             status[size_t(s)] = r.status;
             kused[size_t(s)] = r.kused;
         }
@@ -969,14 +1061,51 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
         for (int i = 0; i < n; ++i) all[size_t(i)] = i;
     }
 
+    // cull = 1 (the library's shortcuts, see rpd_cull.hip and the bisector
+    // kernel): per stored seed the squared distance below which its cell
+    // holds no domain point, and the seeds whose cover test proves volume 0
+    std::vector<double> cull_r2;
+    std::vector<int> todo = all;  // cells to clip (all keeps the output order)
+    if (prm->cull && gr) {
+        const int g = gr->size;
+        const double vs = 1000.0 / double(g);
+        const std::vector<uint8_t> dist = domain_distance(gr);
+        auto vox = [&](double x) { return int(std::min(std::max(x / vs, 0.0), double(g - 1))); };
+        auto dist_of = [&](int s) {
+            const double *p = xyzw + 4 * size_t(kg.perm[size_t(s)]);
+            return int(dist[size_t(vox(p[0])) + size_t(vox(p[1])) * g + size_t(vox(p[2])) * g * g]);
+        };
+        cull_r2.assign(size_t(n), 0.0);
+        for (int s = 0; s < n; ++s) {
+            const int d = dist_of(s);
+            if (d >= 2) cull_r2[size_t(s)] = double(d - 1) * vs * double(d - 1) * vs;
+        }
+        bool border = false;
+        for (const auto &p : pts) border = border || double(p.w) <= kNoCellLimits;
+        if (sm == 0 && !border) {
+            const double M = cover_bound(kg, dist, g, wmin);
+            std::vector<int> kept;
+            kept.reserve(all.size());
+            for (int s : all) {
+                const double D = double(std::max(0, dist_of(s) - 1)) * vs;
+                const double w = xyzw[4 * size_t(kg.perm[size_t(s)]) + 3];
+                if (D * D + (w - wmin) > M * (1.0 + kCoverRel) + kCoverAbs)
+                    status[size_t(s)] = kSuccess;  // out stays 0
+                else
+                    kept.push_back(s);
+            }
+            todo.swap(kept);
+        }
+    }
+
     if (prm->single_pass) {
-        clip_pass(kg, all, K, P, V, T, vhs, grid, prm->apex_mode, grid && gv.has_border_vertices, sm, wmin, out, status,
-                  kused, ctr, t_knn, t_clip);
+        clip_pass(kg, todo, K, P, V, T, vhs, grid, prm->apex_mode, grid && gv.has_border_vertices, sm, wmin, out,
+                  status, kused, ctr, t_knn, t_clip, cull_r2);
         passes = 1;
     } else {
         // first pass: T = 0 defers every boundary cell (voronoi.cpp:1294-1297)
-        clip_pass(kg, all, K, P, V, 0, vhs, grid, prm->apex_mode, false, sm, wmin, out, status, kused, ctr, t_knn,
-                  t_clip);
+        clip_pass(kg, todo, K, P, V, 0, vhs, grid, prm->apex_mode, false, sm, wmin, out, status, kused, ctr, t_knn,
+                  t_clip, cull_r2);
         passes = 1;
         auto collect = [&](const std::vector<int> &from, auto cond, std::vector<int> &ids0, std::vector<int> &ids1) {
             ids0.clear();
@@ -988,10 +1117,10 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
             }
         };
         std::vector<int> ids0, ids1;
-        collect(all, [](int s) { return s == kTriangleOverflow; }, ids0, ids1);
+        collect(todo, [](int s) { return s == kTriangleOverflow; }, ids0, ids1);
         if (!ids1.empty()) {  // tr_overflow pass (voronoi.cpp:1304-1310)
             clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, false, sm, wmin, out, status, kused, ctr,
-                      t_knn, t_clip);
+                      t_knn, t_clip, cull_r2);
             passes++;
         }
         if (!ids0.empty()) {
@@ -1016,7 +1145,7 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
                 if (P >= 256) P = 255;  // uchar plane ids (voronoi.cpp:867-870)
                 K = std::min(K, n - 1 > 0 ? n - 1 : 1);
                 clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, out_domain, sm, wmin, out, status, kused,
-                          ctr, t_knn, t_clip);
+                          ctr, t_knn, t_clip, cull_r2);
                 passes++;
                 cur = ids0;
                 collect(cur, [](int s) { return s != kEmptyCell; }, ids0, ids1);

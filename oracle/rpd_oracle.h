@@ -47,6 +47,11 @@ typedef struct {
     int security_mode;           /* 0: exact power stop d(d-2R) >= w_s - w_min;
                                     1: reference stop d^2 > 4R^2 (weights
                                     ignored, ConvexCell.cl:364-382)            */
+    int cull;                    /* 1: the library's shortcuts for cells whose
+                                    restriction is provably empty (the seed's
+                                    cover test before the kNN, the circumradius
+                                    test in the neighbour loop); 0: none, as
+                                    the reference                              */
 } orc_params;
 
 typedef struct {

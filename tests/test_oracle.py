@@ -177,3 +177,39 @@ def test_border_seeds_are_copied(rpd, orc):
     out, st, _, _ = orc.solve(x, None, K=40, P=40, V=60)
     assert np.array_equal(out[:5], x[:5])
     assert np.all(st[5:] == 4)
+
+
+@pytest.mark.parametrize("shape,prm", [(1, (0.5, 0.5, 0.5, 0.3, 0.12)), (0, (0.5, 0.5, 0.5, 0.2))])
+def test_cull_is_exact(rpd, orc, shape, prm):
+    """cull = 1 (the cover test before the kNN and the circumradius test in
+    the neighbour loop) changes no cell: every moment equals the full clip's,
+    and the cells it ends early have volume 0 when clipped.  Both tests must
+    also be taken, by most seeds outside the domain."""
+    n, g = 30000, 40
+    h = (1.0 / n) ** (1 / 3)
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=8, wmax=0.25 * h * h * 1e6)
+    og = oracle_grid(orc, rpd.Grid.shape(shape, g, prm))
+    full, st0, _, c0 = orc.solve(x, og, K=64, P=40, V=60, T=200, single_pass=1)
+    got, st1, ku1, c1 = orc.solve(x, og, K=64, P=40, V=60, T=200, single_pass=1, cull=1)
+    assert np.array_equal(full, got) or np.abs(full - got).max() <= 1e-12 * 1e9 / n
+    ended = (st1 == 4) & (got == 0).all(1) & ((st0 != 4) | (full != 0).any(1))
+    assert (np.abs(full[ended, 3]) <= 1e-12 * 1e9 / n).all()
+    skipped = (st1 == 4) & (ku1 == 0)
+    assert skipped.sum() > n // 3
+    assert c1["n_pred"] < 0.6 * c0["n_pred"]
+
+
+def test_cull_with_reference_stop_and_walls(rpd, orc):
+    """The cover test needs the exact power stop and no border seeds: with
+    either missing only the circumradius test runs, and results still match."""
+    n, g = 8000, 32
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=9)
+    og = oracle_grid(orc, rpd.Grid.shape(1, g, (0.5, 0.5, 0.5, 0.3, 0.12)))
+    for sm, walls in ((1, False), (0, True)):
+        y = x.copy()
+        if walls:
+            y[:40, 3] = -1e6
+        full, _, _, _ = orc.solve(y, og, K=64, P=40, V=60, T=200, single_pass=1, security_mode=sm)
+        got, st, ku, _ = orc.solve(y, og, K=64, P=40, V=60, T=200, single_pass=1, security_mode=sm, cull=1)
+        assert np.abs(full - got).max() <= 1e-12 * 1e9 / n
+        assert not ((st == 4) & (ku == 0) & (y[:, 3] > -1e5)).any()
