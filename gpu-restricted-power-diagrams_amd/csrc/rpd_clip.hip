@@ -888,16 +888,18 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     // ---- restriction to the domain grid --------------------------------------
     bool add_volume = true;
     int ntri = 0;
+    double rcell = 0;  // largest distance of a vertex from the seed (FP64)
     double P0x = 0, P0y = 0, P0z = 0;  // apex, global frame
     if (a.g > 0) {
         const int g = a.g;
         const double vs = a.vs;
-        double bmn[3] = {1000.0, 1000.0, 1000.0}, bmx[3] = {0.0, 0.0, 0.0};
+        double bmn[3] = {1000.0, 1000.0, 1000.0}, bmx[3] = {0.0, 0.0, 0.0}, vr2 = 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             if (!((c.valid[s] >> lane) & 1)) continue;
             RP lx, ly, lz;
             vertex_rp(s, lx, ly, lz);
+            vr2 = fmax(vr2, double(lx) * double(lx) + double(ly) * double(ly) + double(lz) * double(lz));
             const double gx = S0.x + double(lx), gy = S0.y + double(ly), gz = S0.z + double(lz);
             bmn[0] = fmin(bmn[0], gx);
             bmn[1] = fmin(bmn[1], gy);
@@ -906,6 +908,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
             bmx[1] = fmax(bmx[1], gy);
             bmx[2] = fmax(bmx[2], gz);
         }
+        rcell = sqrt(wave_max(vr2));
         int vmn[3], vmx[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -1142,9 +1145,15 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
             }
             return ballot(sep) != 0;
         };
+        // A cone plane (unit normal n, offset d) with d < -rcell keeps no point
+        // of the ball |y| <= rcell about the seed, which holds the cell: it
+        // removes every vertex (kept side n.y + d >= 0), with a margin far
+        // above the rounding of d and rcell.
+        const double rsk = rcell * (1.0 + 1e-9) + 1e-9;
         for (int jb = sub; jb < ntri; jb += 16 * W) {
             // cone planes (A,B,C), (A,D,B), (A,C,D), (C,D,B), A = apex
             // (CCClipTetFromPointsInfinite): lane = 4 * triangle + plane
+            uint64_t sph;
             {
                 const int i = lane >> 2, f = lane & 3;
                 const int k = jb + W * i;
@@ -1211,6 +1220,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                     pl = Pl<RP>{RP(nx_), RP(ny_), RP(nz_), RP(nd)};
                 }
                 cpl[lane] = pl;
+                sph = ballot(k < ntri && double(pl.w) < -rsk);
             }
             wsync();
             for (int i = 0; i < 16; ++i) {
@@ -1220,7 +1230,8 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                 const Pl<RP> q0 = cpl[4 * i], q1 = cpl[4 * i + 1], q2 = cpl[4 * i + 2], q3 = cpl[4 * i + 3];
                 double4 ck = make_double4(0.0, 0.0, 0.0, 0.0);
                 const bool skip = tid > fail_t ||
-                                  (may_skip && (c.removes_all(cvt<R>(q0)) || c.removes_all(cvt<R>(q1)) ||
+                                  (may_skip && (((sph >> (4 * i)) & 0xfull) != 0 ||
+                                                c.removes_all(cvt<R>(q0)) || c.removes_all(cvt<R>(q1)) ||
                                                 c.removes_all(cvt<R>(q2)) || c.removes_all(cvt<R>(q3)) ||
                                                 cone_beyond_a_face(i)));
                 if (!skip) {
