@@ -16,10 +16,11 @@
 //       voronoi.cpp:1349-1356; here they are always recomputed); from the
 //       FP64 form they are final (a recomputation fails the same way);
 //     * empty_cell is final (volume 0);
-//   every re-queue pass runs in the FP64 form.  The first one takes the
-//   failures of pass 1's bisector kernel and runs on a second stream while
-//   pass 1's finish kernel still runs; the failures of both go to the next
-//   pass.  A cell's result does not depend on the caps it succeeds with.
+//   every re-queue pass runs in the FP64 form, in two chains: the failures
+//   of pass 1's bisector kernel (and theirs, and so on) on a second stream
+//   while pass 1's finish kernel still runs, then the failures of that
+//   finish kernel, each chain growing its own caps.  A cell's result does not
+//   depend on the caps it succeeds with.
 // Boundary cells are processed in the same pass as interior cells (the
 // reference defers them with a T = 0 first pass and recomputes them).
 
@@ -66,7 +67,7 @@ struct rpd_ctx {
     DevBuf out, status, kused;
     // pass work buffers: two passes may run at once (streams[0], streams[1])
     PassBufs pb[2];
-    FailBufs fl[2];
+    FailBufs fl[4];
     DevBuf knn, ids, ids1, cnt, wred, cull_scratch;
     bool solved = false;
     hipEvent_t ev[2][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
@@ -577,35 +578,31 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
             ms_fb += mc;
         }
     };
-    // Caps of the next pass from the counters of a failure list, and the
-    // cells it takes (copied to c->ids on stream s).  Returns the cell count
-    // (0: no further pass).
-    auto next_pass = [&](const int *cnt, int fl, hipStream_t s, int caps[4]) -> int64_t {
+    // Caps of the next pass from the counters of a failure list (grown from
+    // cap), and the cells it takes, copied to c->ids on stream s.  Returns
+    // the cell count (0: no further pass, -1: copy failed).
+    struct Caps {
+        int K, P, V, T;
+    };
+    auto next_pass = [&](const int *cnt, int fl, hipStream_t s, Caps &cap) -> int64_t {
         const int n_exact = cnt[0], n_over = cnt[1], tneed = cnt[13];
         const int *h = cnt + 2;
-        int K2 = K, P2 = P, V2 = V, T2 = T;
-        if (h[RPD_VERTEX_OVERFLOW]) V2 = std::min(256, int(V * 1.5));
-        if (h[RPD_PLANE_OVERFLOW]) P2 = std::min(255, int(P * 1.5));
+        Caps nc = cap;
+        if (h[RPD_VERTEX_OVERFLOW]) nc.V = std::min(256, int(cap.V * 1.5));
+        if (h[RPD_PLANE_OVERFLOW]) nc.P = std::min(255, int(cap.P * 1.5));
         // x2: a cell's result does not depend on K once its security radius
         // is reached, and one wider kNN costs less than another pass
-        if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) K2 = int(std::min<int64_t>(n - 1, std::min(256, 2 * K)));
+        if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) nc.K = int(std::min<int64_t>(n - 1, std::min(256, 2 * cap.K)));
         // the largest triangle union the finish kernel met, within what one
         // cell of that kernel holds in LDS
-        if (h[RPD_TRIANGLE_OVERFLOW]) T2 = std::max(int(T * 1.5) + 1, tneed);
-        T2 = std::min(T2, rpd_clip_max_T(RPD_PREC_FP64, P2, V2));
-        const bool caps_changed = K2 != K || P2 != P || V2 != V || T2 != T;
+        if (h[RPD_TRIANGLE_OVERFLOW]) nc.T = std::max(int(cap.T * 1.5) + 1, tneed);
+        nc.T = std::min(nc.T, rpd_clip_max_T(RPD_PREC_FP64, nc.P, nc.V));
+        const bool caps_changed = nc.K != cap.K || nc.P != cap.P || nc.V != cap.V || nc.T != cap.T;
         // exact-predicate failures (only listed from the FP32 forms) always
         // go to FP64; overflows only when a cap grew
         const bool take_over = n_over > 0 && caps_changed;
         if (n_exact == 0 && !take_over) return 0;
-        K = K2;
-        P = P2;
-        V = V2;
-        T = T2;
-        caps[0] = K;
-        caps[1] = P;
-        caps[2] = V;
-        caps[3] = T;
+        cap = nc;
         int64_t m = 0;
         if (n_exact > 0) {
             if (hipMemcpyAsync(c->ids.ptr, c->fl[fl].exact.ptr, size_t(n_exact) * 4, hipMemcpyDeviceToDevice, s) !=
@@ -625,12 +622,47 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     auto clear_list = [&](int fl, hipStream_t s) {
         return hipMemsetAsync(c->fl[fl].cnt.ptr, 0, kFailCounters * sizeof(int), s) == hipSuccess;
     };
+    // A chain of re-queue passes on stream slot sl: the first takes the
+    // failures of list `from` (host counters cnt), each pass writes its own
+    // failures to outs[0], outs[1], outs[0], ...  Pass k of the chain is
+    // accounted as pass index0 + k.  Returns the passes run, or -1.
+    auto run_chain = [&](int sl, int from, const int *cnt, const int outs[2], Caps cap, int index0,
+                         Caps &cap_out) -> int {
+        hipStream_t s = c->streams[sl];
+        int cur = from, k = 0, o = 0;
+        int cnt_cur[kFailCounters];
+        std::memcpy(cnt_cur, cnt, sizeof(cnt_cur));
+        for (; k < prm->max_extra_steps; ++k) {
+            Pass ps;
+            ps.slot = sl;
+            ps.form = RPD_PREC_FP64;
+            ps.n_ids = next_pass(cnt_cur, cur, s, cap);
+            if (ps.n_ids < 0) return -1;
+            if (ps.n_ids == 0) break;
+            ps.caps[0] = cap.K;
+            ps.caps[1] = cap.P;
+            ps.caps[2] = cap.V;
+            ps.caps[3] = cap.T;
+            const int nxt = outs[o];
+            if (!clear_list(nxt, s)) return -1;
+            if (launch_pass(ps, c->ids.as<int>(), nxt, nullptr) != RPD_OK) return -1;
+            if (hipMemcpyAsync(cnt_cur, c->fl[nxt].cnt.ptr, sizeof(cnt_cur), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                return -1;
+            account(ps, index0 + k);
+            cur = nxt;
+            o ^= 1;
+        }
+        cap_out = cap;
+        return k;
+    };
 
     // Pass 1 on stream 0: the bisector kernel's failures go to list 0, the
-    // finish kernel's to list 1.  The first re-queue pass (the bisector
-    // failures, known as soon as that kernel is done) runs on stream 1 while
-    // pass 1's finish kernel still runs; its failures go to list 1 too.
-    // Later passes run one after another, alternating the lists.
+    // finish kernel's to list 1.  Two chains of re-queue passes follow:
+    //   * the bisector failures, known as soon as that kernel is done, run
+    //     on stream 1 while pass 1's finish kernel still runs (lists 2, 3);
+    //   * the finish kernel's failures, once it is done, on stream 0
+    //     (lists 0, 1), with the caps of pass 1 grown only for them.
     if (!clear_list(0, c->stream) || !clear_list(1, c->stream)) return set_err(c, RPD_E_DEVICE, "clear lists");
     Pass p1;
     p1.slot = 0;
@@ -688,49 +720,28 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         RPDCHK(c, rpd_launch_finish(s, form0, L, p1.nq), "pass 1 finish");
         HIPCHK(c, hipEventRecord(c->ev[0][3], s));
     }
-    st.passes = 1;
-    int steps = 0;
-    Pass p2;
-    bool have_p2 = false;
+    const Caps caps1{K, P, V, T};
+    Caps cap_a = caps1, cap_b = caps1;
+    int steps_a = 0, steps_b = 0;
     if (prm->max_extra_steps > 0) {
-        p2.slot = 1;
-        p2.form = RPD_PREC_FP64;
-        p2.n_ids = next_pass(cnt_a, 0, c->streams[1], p2.caps);
-        if (p2.n_ids < 0) return set_err(c, RPD_E_DEVICE, "re-queue copy");
-        if (p2.n_ids > 0) {
-            RPDCHK(c, launch_pass(p2, c->ids.as<int>(), 1, nullptr), "re-queue pass");
-            have_p2 = true;
-            st.passes++;
-            steps = 1;
-        }
+        const int outs_a[2] = {2, 3};
+        steps_a = run_chain(1, 0, cnt_a, outs_a, caps1, 1, cap_a);
+        if (steps_a < 0) return set_err(c, RPD_E_DEVICE, "re-queue chain (bisector failures)");
     }
     HIPCHK(c, hipStreamSynchronize(c->streams[0]));
-    HIPCHK(c, hipStreamSynchronize(c->streams[1]));
     account(p1, 0);
-    if (have_p2) account(p2, 1);
-    // counters of list 1 (pass 1 finish + first re-queue pass)
-    HIPCHK(c, hipMemcpyAsync(cnt_b, c->fl[1].cnt.ptr, sizeof(cnt_b), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    int cur = 1;
-    int *cur_cnt = cnt_b, *nxt_cnt = cnt_a;
-    for (; steps < prm->max_extra_steps; ++steps) {
-        Pass ps;
-        ps.slot = 0;
-        ps.form = RPD_PREC_FP64;
-        ps.n_ids = next_pass(cur_cnt, cur, c->stream, ps.caps);
-        if (ps.n_ids < 0) return set_err(c, RPD_E_DEVICE, "re-queue copy");
-        if (ps.n_ids == 0) break;
-        const int nxt = 1 - cur;
-        if (!clear_list(nxt, c->stream)) return set_err(c, RPD_E_DEVICE, "clear list");
-        RPDCHK(c, launch_pass(ps, c->ids.as<int>(), nxt, nullptr), "re-queue pass");
-        HIPCHK(c, hipMemcpyAsync(nxt_cnt, c->fl[nxt].cnt.ptr, kFailCounters * sizeof(int), hipMemcpyDeviceToHost,
-                                 c->stream));
+    if (prm->max_extra_steps > 0) {
+        HIPCHK(c, hipMemcpyAsync(cnt_b, c->fl[1].cnt.ptr, sizeof(cnt_b), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        account(ps, st.passes);
-        st.passes++;
-        cur = nxt;
-        std::swap(cur_cnt, nxt_cnt);
+        const int outs_b[2] = {0, 1};
+        steps_b = run_chain(0, 1, cnt_b, outs_b, caps1, 1, cap_b);
+        if (steps_b < 0) return set_err(c, RPD_E_DEVICE, "re-queue chain (finish failures)");
     }
+    st.passes = 1 + std::max(steps_a, steps_b);
+    K = std::max(cap_a.K, cap_b.K);
+    P = std::max(cap_a.P, cap_b.P);
+    V = std::max(cap_a.V, cap_b.V);
+    T = std::max(cap_a.T, cap_b.T);
     // final status histogram
     HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));
     status_hist_kernel<<<unsigned(std::min<int64_t>(1024, (n + 255) / 256)), 256, 0, c->stream>>>(
