@@ -48,13 +48,15 @@ struct Request {
     double lat_ms = 0;
 };
 
-// Device bytes one solve of n seeds holds at its peak (seeds, sorted copies,
-// kNN lists, cell records of the first pass, outputs, id lists, binning).
+// Device bytes one solve of n seeds holds at its peak: seeds and their two
+// sorted copies, permutation, outputs, kNN lists and cell records of pass 1
+// and (bounded by the same size) of the re-queue pass that runs beside it,
+// id lists, the four failure lists, the cover flags, binning scratch.
 int64_t footprint_bytes(int64_t n, const rpd_params &p) {
     const int64_t K = std::max(1, p.K), P = std::max(1, p.P);
     const int64_t S = p.V <= 64 ? 1 : (p.V <= 128 ? 2 : 4);
     const int64_t rec = 16 + P * 32 + 256 * S;
-    return n * (32 + 32 + 32 + 4 + 32 + 8 + 4 * K + rec + 12 + 16) + (int64_t(1) << 20);
+    return n * (32 + 32 + 32 + 4 + 32 + 8 + 2 * (4 * K + rec) + 8 + 32 + 1 + 16) + (int64_t(1) << 20);
 }
 
 }  // namespace
