@@ -47,15 +47,16 @@ struct rpd_domain {
 struct PassBufs {
     DevBuf knn, rec, qcount;
 };
-// failed cells handed to the next pass: counters, then stored ids
+// failed cells handed to the next pass: counters, then stored ids of the
+// exact-predicate failures, the overflows and the resumable cells
 struct FailBufs {
-    DevBuf cnt, exact, over;
+    DevBuf cnt, exact, over, sec;
 };
 
 struct rpd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;      // == streams[0]
-    hipStream_t streams[2] = {nullptr, nullptr};
+    hipStream_t streams[3] = {nullptr, nullptr, nullptr};
     std::string err;
     int64_t n = 0;
     DevBuf seeds;  // original order, FP64 xyzw
@@ -65,12 +66,16 @@ struct rpd_ctx {
     DevBuf perm, counters, stored, stored64, bin_scratch;
     // outputs, original order
     DevBuf out, status, kused;
-    // pass work buffers: two passes may run at once (streams[0], streams[1])
-    PassBufs pb[2];
+    // pass work buffers: up to three passes may run at once, one per stream
+    PassBufs pb[3];
     FailBufs fl[4];
     DevBuf knn, ids, ids1, cnt, wred, cull_scratch;
+    // states of resumable cells (two record buffers, their counters) and,
+    // per stored id, the record slot and the last neighbour used
+    DevBuf rs[2], rs_cnt, rslot, lastnb;
     bool solved = false;
-    hipEvent_t ev[2][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
+    hipEvent_t ev[3][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
+    hipEvent_t dep = nullptr;  // orders work between streams
 };
 
 namespace {
@@ -230,6 +235,10 @@ int rpd_create(int device, rpd_ctx **out) {
                 rpd_destroy(c);
                 return RPD_E_DEVICE;
             }
+    if (hipEventCreateWithFlags(&c->dep, hipEventDisableTiming) != hipSuccess) {
+        rpd_destroy(c);
+        return RPD_E_DEVICE;
+    }
     *out = c;
     return RPD_OK;
 }
@@ -242,6 +251,7 @@ void rpd_destroy(rpd_ctx *c) {
     for (auto &es : c->ev)
         for (auto &e : es)
             if (e) hipEventDestroy(e);
+    if (c->dep) hipEventDestroy(c->dep);
     for (auto &s : c->streams)
         if (s) hipStreamDestroy(s);
     delete c;
@@ -447,7 +457,11 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         RPDCHK(c, l.cnt.ensure(kFailCounters * sizeof(int)), "alloc failure list");
         RPDCHK(c, l.exact.ensure(size_t(n) * 4), "alloc failure list");
         RPDCHK(c, l.over.ensure(size_t(n) * 4), "alloc failure list");
+        RPDCHK(c, l.sec.ensure(size_t(n) * 4), "alloc failure list");
     }
+    RPDCHK(c, c->rs_cnt.ensure(2 * sizeof(int)), "alloc resume counters");
+    RPDCHK(c, c->rslot.ensure(size_t(n) * 4), "alloc resume slots");
+    RPDCHK(c, c->lastnb.ensure(size_t(n) * 4), "alloc resume slots");
     RPDCHK(c, c->cnt.ensure(64), "alloc cnt");
 
     const DomainDev *dd = dom ? &dom->dev : nullptr;
@@ -511,7 +525,16 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         int caps[4] = {0, 0, 0, 0};
         int nq = 0;
     };
-    auto launch_pass = [&](Pass &ps, const int *ids, int fl_out, int *fl_cnt_host) -> int {
+    // resumable cells: records read (in, FP64 passes) and written (out)
+    struct Resume {
+        const char *in = nullptr;  // the cells of this pass resume from here
+        int in_stride = 0, in_P = 0;
+        int out = -1;              // rs buffer written, -1: none
+        int out_stride = 0, out_cap = 0;
+    };
+    // kNN + bisector kernel, the queue length copied to the host (not waited for)
+    auto pass_front = [&](Pass &ps, const int *ids, int fl_out, int *fl_cnt_host, const Resume &rz,
+                          ClipLaunch &L) -> int {
         const int sl = ps.slot;
         hipStream_t s = c->streams[sl];
         PassBufs &b = c->pb[sl];
@@ -522,10 +545,10 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         if (r != RPD_OK) return r;
         if (hipEventRecord(c->ev[sl][0], s) != hipSuccess) return RPD_E_DEVICE;
         r = rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids, ps.n_ids,
-                           K_, b.knn.as<int>());
+                           K_, b.knn.as<int>(), rz.in ? c->lastnb.as<int>() : nullptr);
         if (r != RPD_OK) return r;
         if (hipEventRecord(c->ev[sl][1], s) != hipSuccess) return RPD_E_DEVICE;
-        ClipLaunch L{};
+        L = ClipLaunch{};
         L.seeds = c->stored64.as<double4>();
         L.perm = c->perm.as<int>();
         L.ids = ids;
@@ -550,12 +573,26 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.fl_cnt = c->fl[fl_out].cnt.as<int>();
         L.fl_exact = c->fl[fl_out].exact.as<int>();
         L.fl_over = c->fl[fl_out].over.as<int>();
+        L.fl_sec = c->fl[fl_out].sec.as<int>();
         L.fl_cnt_host = fl_cnt_host;
-        r = rpd_launch_cells(s, ps.form, L, &ps.nq);
+        L.rs_in = rz.in;
+        L.rs_in_stride = rz.in_stride;
+        L.rs_in_P = rz.in_P;
+        L.rs_out = rz.out >= 0 ? static_cast<char *>(c->rs[rz.out].ptr) : nullptr;
+        L.rs_out_stride = rz.out_stride;
+        L.rs_cap = rz.out_cap;
+        L.rs_count = c->rs_cnt.as<int>() + std::max(rz.out, 0);
+        L.rslot = c->rslot.as<int>();
+        L.lastnb = c->lastnb.as<int>();
+        return rpd_launch_cells(s, ps.form, L, &ps.nq, false);
+    };
+    // waits for the queue length, then the finish kernel (not waited for)
+    auto pass_back = [&](Pass &ps, const ClipLaunch &L) -> int {
+        hipStream_t s = c->streams[ps.slot];
+        if (hipStreamSynchronize(s) != hipSuccess) return RPD_E_DEVICE;
+        const int r = rpd_launch_finish(s, ps.form, L, ps.nq);
         if (r != RPD_OK) return r;
-        r = rpd_launch_finish(s, ps.form, L, ps.nq);
-        if (r != RPD_OK) return r;
-        return hipEventRecord(c->ev[sl][3], s) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+        return hipEventRecord(c->ev[ps.slot][3], s) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
     };
     double ms_knn = 0, ms_fb_knn = 0, ms_fb = 0;
     auto account = [&](const Pass &ps, int index) {
@@ -578,78 +615,118 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
             ms_fb += mc;
         }
     };
-    // Caps of the next pass from the counters of a failure list (grown from
-    // cap), and the cells it takes, copied to c->ids on stream s.  Returns
-    // the cell count (0: no further pass, -1: copy failed).
+    // Caps of the next pass from the counters of a failure list, grown from
+    // cap: vertex/plane overflow x1.5, security radius x2 (a cell's result
+    // does not depend on K once its radius is reached, and one wider kNN costs
+    // less than another pass), triangle overflow to the largest union the
+    // finish kernel met (within the LDS of one of its cells).
     struct Caps {
         int K, P, V, T;
     };
-    auto next_pass = [&](const int *cnt, int fl, hipStream_t s, Caps &cap) -> int64_t {
-        const int n_exact = cnt[0], n_over = cnt[1], tneed = cnt[13];
+    auto grow = [&](const int *cnt, const Caps &cap) {
         const int *h = cnt + 2;
         Caps nc = cap;
         if (h[RPD_VERTEX_OVERFLOW]) nc.V = std::min(256, int(cap.V * 1.5));
         if (h[RPD_PLANE_OVERFLOW]) nc.P = std::min(255, int(cap.P * 1.5));
-        // x2: a cell's result does not depend on K once its security radius
-        // is reached, and one wider kNN costs less than another pass
         if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) nc.K = int(std::min<int64_t>(n - 1, std::min(256, 2 * cap.K)));
-        // the largest triangle union the finish kernel met, within what one
-        // cell of that kernel holds in LDS
-        if (h[RPD_TRIANGLE_OVERFLOW]) nc.T = std::max(int(cap.T * 1.5) + 1, tneed);
+        if (h[RPD_TRIANGLE_OVERFLOW]) nc.T = std::max(int(cap.T * 1.5) + 1, cnt[13]);
         nc.T = std::min(nc.T, rpd_clip_max_T(RPD_PREC_FP64, nc.P, nc.V));
-        const bool caps_changed = nc.K != cap.K || nc.P != cap.P || nc.V != cap.V || nc.T != cap.T;
-        // exact-predicate failures (only listed from the FP32 forms) always
-        // go to FP64; overflows only when a cap grew
-        const bool take_over = n_over > 0 && caps_changed;
-        if (n_exact == 0 && !take_over) return 0;
-        cap = nc;
-        int64_t m = 0;
-        if (n_exact > 0) {
-            if (hipMemcpyAsync(c->ids.ptr, c->fl[fl].exact.ptr, size_t(n_exact) * 4, hipMemcpyDeviceToDevice, s) !=
-                hipSuccess)
-                return -1;
-            m += n_exact;
-            st.fallback_cells += n_exact;
-        }
-        if (take_over) {
-            if (hipMemcpyAsync(c->ids.as<int>() + m, c->fl[fl].over.ptr, size_t(n_over) * 4,
-                               hipMemcpyDeviceToDevice, s) != hipSuccess)
-                return -1;
-            m += n_over;
-        }
-        return m;
+        return nc;
     };
     auto clear_list = [&](int fl, hipStream_t s) {
         return hipMemsetAsync(c->fl[fl].cnt.ptr, 0, kFailCounters * sizeof(int), s) == hipSuccess;
     };
+    auto copy_ids = [&](int *dst, const DevBuf &src, int cnt_, hipStream_t s) {
+        return cnt_ <= 0 ||
+               hipMemcpyAsync(dst, src.ptr, size_t(cnt_) * 4, hipMemcpyDeviceToDevice, s) == hipSuccess;
+    };
+    // Record buffer `which` for cap records of stride bytes, counter cleared.
+    auto resume_buf = [&](int which, int64_t cap, int stride, hipStream_t s) -> int {
+        const int r = c->rs[which].ensure(size_t(std::max<int64_t>(cap, 1)) * size_t(stride));
+        if (r != RPD_OK) return r;
+        return hipMemsetAsync(c->rs_cnt.as<int>() + which, 0, sizeof(int), s) == hipSuccess ? RPD_OK
+                                                                                              : RPD_E_DEVICE;
+    };
+    // Where the previous pass of a chain left the states of its resumable
+    // cells (rs buffer, stride, plane room, K), or which = -1.
+    struct Left {
+        int which = -1, stride = 0, P = 0;
+    };
     // A chain of re-queue passes on stream slot sl: the first takes the
     // failures of list `from` (host counters cnt), each pass writes its own
-    // failures to outs[0], outs[1], outs[0], ...  Pass k of the chain is
-    // accounted as pass index0 + k.  Returns the passes run, or -1.
-    auto run_chain = [&](int sl, int from, const int *cnt, const int outs[2], Caps cap, int index0,
+    // failures to outs[0], outs[1], outs[0], ...  A pass resumes the cells
+    // whose state the previous one kept (their kNN lists only the neighbours
+    // after the last one used) and recomputes the others in FP64.  Pass k of
+    // the chain is accounted as pass index0 + k.  Returns the passes run, or -1.
+    auto run_chain = [&](int sl, int from, const int *cnt, const int outs[2], Caps cap, Left left, int index0,
                          Caps &cap_out) -> int {
         hipStream_t s = c->streams[sl];
         int cur = from, k = 0, o = 0;
         int cnt_cur[kFailCounters];
         std::memcpy(cnt_cur, cnt, sizeof(cnt_cur));
         for (; k < prm->max_extra_steps; ++k) {
-            Pass ps;
-            ps.slot = sl;
-            ps.form = RPD_PREC_FP64;
-            ps.n_ids = next_pass(cnt_cur, cur, s, cap);
-            if (ps.n_ids < 0) return -1;
-            if (ps.n_ids == 0) break;
-            ps.caps[0] = cap.K;
-            ps.caps[1] = cap.P;
-            ps.caps[2] = cap.V;
-            ps.caps[3] = cap.T;
+            const int n_exact = cnt_cur[0], n_over = cnt_cur[1], n_sec = cnt_cur[14];
+            const Caps nc = grow(cnt_cur, cap);
+            const bool caps_changed = nc.K != cap.K || nc.P != cap.P || nc.V != cap.V || nc.T != cap.T;
+            // exact-predicate failures (only listed from the FP32 forms) always
+            // go to FP64; overflows only when a cap grew
+            const bool resume = left.which >= 0 && n_sec > 0 && nc.K > cap.K;
+            const int64_t n_res = resume ? n_sec : 0;
+            const int64_t n_scr = n_exact + (caps_changed ? n_over + (resume ? 0 : n_sec) : 0);
+            if (n_res + n_scr == 0) break;
+            int *ids = c->ids.as<int>();
+            if (!copy_ids(ids, c->fl[cur].sec, int(n_res), s) || !copy_ids(ids + n_res, c->fl[cur].exact, n_exact, s))
+                return -1;
+            if (caps_changed && (!copy_ids(ids + n_res + n_exact, c->fl[cur].over, n_over, s) ||
+                                 (!resume && !copy_ids(ids + n_res + n_exact + n_over, c->fl[cur].sec, n_sec, s))))
+                return -1;
+            st.fallback_cells += n_exact;
             const int nxt = outs[o];
             if (!clear_list(nxt, s)) return -1;
-            if (launch_pass(ps, c->ids.as<int>(), nxt, nullptr) != RPD_OK) return -1;
+            // states of this pass's resumable cells go to the other buffer
+            Resume rz;
+            rz.out = left.which == 0 ? 1 : 0;
+            rz.out_stride = int(rpd_clip_record_bytes(RPD_PREC_FP64, nc.P, nc.V));
+            rz.out_cap = int(n_res + n_scr);
+            if (resume_buf(rz.out, rz.out_cap, rz.out_stride, s) != RPD_OK) return -1;
+            // the resumed cells on the chain's stream, the recomputed ones on
+            // stream 2, both at once
+            Pass pr, pz;
+            ClipLaunch Lr, Lz;
+            pr.slot = sl;
+            pz.slot = 2;
+            pr.form = pz.form = RPD_PREC_FP64;
+            pr.n_ids = n_res;
+            pz.n_ids = n_scr;
+            pr.caps[0] = nc.K - cap.K;  // the neighbours after the ones used
+            pz.caps[0] = nc.K;
+            pr.caps[1] = pz.caps[1] = nc.P;
+            pr.caps[2] = pz.caps[2] = nc.V;
+            pr.caps[3] = pz.caps[3] = nc.T;
+            Resume rr = rz;
+            // stream 2 starts after the id copies made on stream s
+            if (n_scr > 0 && (hipEventRecord(c->dep, s) != hipSuccess ||
+                              hipStreamWaitEvent(c->streams[2], c->dep, 0) != hipSuccess))
+                return -1;
+            if (n_res > 0) {
+                rr.in = static_cast<const char *>(c->rs[left.which].ptr);
+                rr.in_stride = left.stride;
+                rr.in_P = left.P;
+                if (pass_front(pr, ids, nxt, nullptr, rr, Lr) != RPD_OK) return -1;
+            }
+            if (n_scr > 0 && pass_front(pz, ids + n_res, nxt, nullptr, rz, Lz) != RPD_OK) return -1;
+            if (n_res > 0 && pass_back(pr, Lr) != RPD_OK) return -1;
+            if (n_scr > 0 && pass_back(pz, Lz) != RPD_OK) return -1;
+            if (hipStreamSynchronize(s) != hipSuccess || hipStreamSynchronize(c->streams[2]) != hipSuccess) return -1;
+            if (n_res > 0) account(pr, index0 + k);
+            if (n_scr > 0) account(pz, index0 + k);
             if (hipMemcpyAsync(cnt_cur, c->fl[nxt].cnt.ptr, sizeof(cnt_cur), hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess)
                 return -1;
-            account(ps, index0 + k);
+            cap = nc;
+            left.which = rz.out;
+            left.stride = rz.out_stride;
+            left.P = nc.P;
             cur = nxt;
             o ^= 1;
         }
@@ -673,6 +750,17 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     p1.caps[2] = V;
     p1.caps[3] = T;
     int cnt_a[kFailCounters] = {0}, cnt_b[kFailCounters] = {0};
+    // states of pass 1's cells whose K neighbours do not reach their
+    // security radius are kept (FP64 planes: FP64 and mixed forms), for up to
+    // 40% of the cells; the others are recomputed from scratch
+    Left left1;
+    const int64_t cap_rs1 = std::min<int64_t>(n1, n1 * 2 / 5 + 1024);
+    if (form0 != RPD_PREC_FP32 && prm->max_extra_steps > 0 && n1 > 0) {
+        left1.which = 0;
+        left1.stride = int(rpd_clip_record_bytes(form0, P, V));
+        left1.P = P;
+        RPDCHK(c, resume_buf(0, cap_rs1, left1.stride, c->streams[0]), "alloc resume records");
+    }
     {
         // the bisector kernel appends to list 0 and the finish kernel to
         // list 1: launch the two halves separately
@@ -712,11 +800,21 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.fl_cnt = c->fl[0].cnt.as<int>();
         L.fl_exact = c->fl[0].exact.as<int>();
         L.fl_over = c->fl[0].over.as<int>();
+        L.fl_sec = c->fl[0].sec.as<int>();
         L.fl_cnt_host = cnt_a;
+        if (left1.which == 0) {
+            L.rs_out = static_cast<char *>(c->rs[0].ptr);
+            L.rs_out_stride = left1.stride;
+            L.rs_cap = int(cap_rs1);
+            L.rs_count = c->rs_cnt.as<int>();
+            L.rslot = c->rslot.as<int>();
+            L.lastnb = c->lastnb.as<int>();
+        }
         RPDCHK(c, rpd_launch_cells(s, form0, L, &p1.nq), "pass 1 cells");
         L.fl_cnt = c->fl[1].cnt.as<int>();
         L.fl_exact = c->fl[1].exact.as<int>();
         L.fl_over = c->fl[1].over.as<int>();
+        L.fl_sec = c->fl[1].sec.as<int>();
         RPDCHK(c, rpd_launch_finish(s, form0, L, p1.nq), "pass 1 finish");
         HIPCHK(c, hipEventRecord(c->ev[0][3], s));
     }
@@ -725,7 +823,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     int steps_a = 0, steps_b = 0;
     if (prm->max_extra_steps > 0) {
         const int outs_a[2] = {2, 3};
-        steps_a = run_chain(1, 0, cnt_a, outs_a, caps1, 1, cap_a);
+        steps_a = run_chain(1, 0, cnt_a, outs_a, caps1, left1, 1, cap_a);
         if (steps_a < 0) return set_err(c, RPD_E_DEVICE, "re-queue chain (bisector failures)");
     }
     HIPCHK(c, hipStreamSynchronize(c->streams[0]));
@@ -734,7 +832,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         HIPCHK(c, hipMemcpyAsync(cnt_b, c->fl[1].cnt.ptr, sizeof(cnt_b), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         const int outs_b[2] = {0, 1};
-        steps_b = run_chain(0, 1, cnt_b, outs_b, caps1, 1, cap_b);
+        steps_b = run_chain(0, 1, cnt_b, outs_b, caps1, Left{}, 1, cap_b);
         if (steps_b < 0) return set_err(c, RPD_E_DEVICE, "re-queue chain (finish failures)");
     }
     st.passes = 1 + std::max(steps_a, steps_b);

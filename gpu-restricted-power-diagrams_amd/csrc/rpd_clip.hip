@@ -91,18 +91,36 @@ struct ClipArgs {
     // largest triangle union of a cell that overflowed T
     int *fl_cnt, *fl_exact, *fl_over;
     int exact_final;      // FP64 form: exact-predicate failures are final
+    // A cell whose list runs out before its security radius is reached can be
+    // resumed: its state goes to rs_out (records of rs_out_stride bytes, slot
+    // from *rs_count below rs_cap, kept in rslot[sid]), the stored id of its
+    // last neighbour to lastnb[sid], the cell to fl_sec (counter fl_cnt[14]).
+    // rs_out null, or the buffer full: a plain overflow.  With rs_in set
+    // (FP64 form), every cell resumes from record rs_in[rslot[sid]] (written
+    // with rs_in_P planes) and its list holds only the neighbours after the
+    // ones it has used.
+    char *rs_out;
+    int rs_out_stride, rs_cap;
+    int *rs_count;
+    const char *rs_in;
+    int rs_in_stride, rs_in_P;
+    int *rslot, *lastnb, *fl_sec;
 };
 
 // Appends a failed cell (stored id sid) to the failure list: overflows
 // (vertex, plane, K, triangle) always; needs_exact_predicates,
 // inconsistent_boundary and find_another_beginning_vertex when the form was
 // not FP64 (in FP64 a recomputation would fail the same way).  One lane.
-__device__ __forceinline__ void note_failure(const ClipArgs &a, int sid, int st) {
+__device__ __forceinline__ void note_failure(const ClipArgs &a, int sid, int st, bool resumable = false) {
     if (!a.fl_cnt || st == RPD_SUCCESS || st == RPD_EMPTY_CELL) return;
     const bool exact = st == RPD_NEEDS_EXACT_PREDICATES || st == RPD_INCONSISTENT_BOUNDARY ||
                        st == RPD_FIND_ANOTHER_BEGINNING_VERTEX;
     if (exact && a.exact_final) return;
     atomicAdd(a.fl_cnt + 2 + st, 1);
+    if (resumable) {
+        a.fl_sec[atomicAdd(a.fl_cnt + 14, 1)] = sid;
+        return;
+    }
     const int k = atomicAdd(a.fl_cnt + (exact ? 0 : 1), 1);
     (exact ? a.fl_exact : a.fl_over)[k] = sid;
 }
@@ -506,13 +524,17 @@ void clip_cells_kernel(ClipArgs a) {
     const double4 S0 = a.seeds[sid];
     const int orig = a.perm[sid];
     const int K = a.K, P = a.P;
+    // resumed cells: record of the state, neighbours used before this pass
+    const bool resume = sizeof(R) == 8 && a.rs_in != nullptr;
+    const char *rs_rec = resume ? a.rs_in + size_t(a.rslot[sid]) * size_t(a.rs_in_stride) : nullptr;
+    const int kbase = resume ? uni(reinterpret_cast<const RecHeader *>(rs_rec)->kused) : 0;
 
-    auto finish = [&](int st, double m0, double m1, double m2, double m3, int ku) {
+    auto finish = [&](int st, double m0, double m1, double m2, double m3, int ku, bool resumable = false) {
         if (lane == 0) {
             a.out[orig] = make_double4(m0, m1, m2, m3);
             a.status[orig] = st;
-            if (a.kused) a.kused[orig] = ku;
-            note_failure(a, sid, st);
+            if (a.kused) a.kused[orig] = kbase + ku;
+            note_failure(a, sid, st, resumable);
         }
     };
     auto fail = [&](int st, int ku) { finish(st, 0.0, 0.0, 0.0, 0.0, ku); };
@@ -537,14 +559,7 @@ void clip_cells_kernel(ClipArgs a) {
     }
     const double wgap = S0.w - a.wmin;  // >= 0 except for the lightest seeds
 
-    // ---- CCInit: bounding cube in the local frame ---------------------------
     const double eps = a.g > 0 ? 0.1 : 0.0;  // CUBE_EPSILON, voronoi.cpp:929
-    if (lane < 6) {
-        c.planes[lane] = cvt<R>(cube_plane(lane, S0, eps));
-        src[lane] = int16_t(-1 - lane);
-    }
-    wsync();
-    c.nb_plane = 6;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         c.valid[s] = 0;
@@ -554,12 +569,37 @@ void clip_cells_kernel(ClipArgs a) {
         c.r2[s] = R(0);
         c.MXYZ[s] = c.MW[s] = 0.f;
     }
-    if (lane < 8) {
-        const uint32_t sh = 8 * uint32_t(lane);
-        c.set_vertex_lds(0, pack3(uint32_t(kCube0 >> sh) & 0xff, uint32_t(kCube1 >> sh) & 0xff,
-                                  uint32_t(kCube2 >> sh) & 0xff));
+    if (resume) {
+        // ---- the state a previous pass left: planes, then the vertices ----
+        const RecHeader hd = *reinterpret_cast<const RecHeader *>(rs_rec);
+        const int nvalid = uni(hd.nvalid);
+        c.nb_plane = uni(hd.nb_plane);
+        const Pl<RP> *rpl = reinterpret_cast<const Pl<RP> *>(rs_rec + sizeof(RecHeader));
+        const uint32_t *rtr =
+            reinterpret_cast<const uint32_t *>(rs_rec + sizeof(RecHeader) + size_t(a.rs_in_P) * sizeof(Pl<RP>));
+        for (int p = lane; p < c.nb_plane; p += 64) c.planes[p] = cvt<R>(rpl[p]);
+        wsync();
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int j = s * 64 + lane;
+            if (j < nvalid) c.set_vertex_lds(s, rtr[j]);
+            c.valid[s] = ballot(j < nvalid);
+        }
+    } else {
+        // ---- CCInit: bounding cube in the local frame ----------------------
+        if (lane < 6) {
+            c.planes[lane] = cvt<R>(cube_plane(lane, S0, eps));
+            src[lane] = int16_t(-1 - lane);
+        }
+        wsync();
+        c.nb_plane = 6;
+        if (lane < 8) {
+            const uint32_t sh = 8 * uint32_t(lane);
+            c.set_vertex_lds(0, pack3(uint32_t(kCube0 >> sh) & 0xff, uint32_t(kCube1 >> sh) & 0xff,
+                                      uint32_t(kCube2 >> sh) & 0xff));
+        }
+        c.valid[0] = 0xffull;
     }
-    c.valid[0] = 0xffull;
     c.refresh_max_r2();
 
     // ---- neighbour loop -----------------------------------------------------
@@ -674,38 +714,58 @@ void clip_cells_kernel(ClipArgs a) {
     }
     // a list that holds every other seed leaves nothing to test: the cell is
     // exact once the list is used up (the reference reads past the seeds there)
-    if (!reached && !complete && int64_t(K) >= a.n_seeds - 1) complete = true;
-    if (!reached && !complete) {
-        fail(RPD_SECURITY_RADIUS_NOT_REACHED, kused);
-        return;
-    }
+    if (!reached && !complete && int64_t(kbase) + K >= a.n_seeds - 1) complete = true;
 
-    // ---- queue the finished cell for restriction + integration -------------
-    int slot = 0;
-    if (lane == 0) slot = atomicAdd(a.qcount, 1);
-    slot = rlane(slot, 0);
-    char *rec = a.rec + size_t(slot) * size_t(a.rec_stride);
-    Pl<RP> *rpl = reinterpret_cast<Pl<RP> *>(rec + sizeof(RecHeader));
-    uint32_t *rtr = reinterpret_cast<uint32_t *>(rec + sizeof(RecHeader) + size_t(P) * sizeof(Pl<RP>));
-    int nv = 0;
-#pragma unroll
-    for (int s = 0; s < S; ++s) nv += __popcll(c.valid[s]);
-    if (lane == 0) *reinterpret_cast<RecHeader *>(rec) = RecHeader{sid, c.nb_plane, nv, kused};
-    wsync();
-    for (int p = lane; p < c.nb_plane; p += 64) {
-        if (sizeof(R) == sizeof(RP)) {
-            rpl[p] = cvt<RP>(c.planes[p]);
-        } else {
-            const int sj = src[p];
-            double bl;
-            rpl[p] = cvt<RP>(sj < 0 ? cube_plane(-1 - sj, S0, eps) : bisector_plane(S0, a.seeds[nrow[sj]], bl));
+    // A finished cell is queued for restriction + integration.  A cell whose
+    // list ran out keeps its state when there is room, so that the next pass
+    // goes on from here with the neighbours after the last one used.
+    const bool ran_out = !reached && !complete;
+    int slot = -1;
+    if (lane == 0) {
+        if (!ran_out) {
+            slot = atomicAdd(a.qcount, 1);
+        } else if (a.rs_out) {
+            slot = atomicAdd(a.rs_count, 1);
+            if (slot >= a.rs_cap) slot = -1;
         }
     }
-    int carry = 0;
+    slot = rlane(slot, 0);
+    if (slot >= 0) {
+        // the cell's state as a record: header, planes in the geometry
+        // precision (rebuilt from their sources when the test precision is
+        // lower), vertex triples; P planes' room.  (Written in line: as a
+        // lambda the FP32-test kernel spills.)
+        char *rec = ran_out ? a.rs_out + size_t(slot) * size_t(a.rs_out_stride)
+                            : a.rec + size_t(slot) * size_t(a.rec_stride);
+        Pl<RP> *rpl = reinterpret_cast<Pl<RP> *>(rec + sizeof(RecHeader));
+        uint32_t *rtr = reinterpret_cast<uint32_t *>(rec + sizeof(RecHeader) + size_t(P) * sizeof(Pl<RP>));
+        int nv = 0;
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        if ((c.valid[s] >> lane) & 1) rtr[carry + mbcnt(c.valid[s])] = c.tri[s];
-        carry += __popcll(c.valid[s]);
+        for (int s = 0; s < S; ++s) nv += __popcll(c.valid[s]);
+        if (lane == 0) *reinterpret_cast<RecHeader *>(rec) = RecHeader{sid, c.nb_plane, nv, kbase + kused};
+        wsync();
+        for (int p = lane; p < c.nb_plane; p += 64) {
+            if (sizeof(R) == sizeof(RP)) {
+                rpl[p] = cvt<RP>(c.planes[p]);
+            } else {
+                const int sj = src[p];
+                double bl;
+                rpl[p] = cvt<RP>(sj < 0 ? cube_plane(-1 - sj, S0, eps) : bisector_plane(S0, a.seeds[nrow[sj]], bl));
+            }
+        }
+        int carry = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if ((c.valid[s] >> lane) & 1) rtr[carry + mbcnt(c.valid[s])] = c.tri[s];
+            carry += __popcll(c.valid[s]);
+        }
+    }
+    if (ran_out) {
+        if (slot >= 0 && lane == 0) {
+            a.rslot[sid] = slot;
+            a.lastnb[sid] = nrow[K - 1];
+        }
+        finish(RPD_SECURITY_RADIUS_NOT_REACHED, 0.0, 0.0, 0.0, 0.0, kused, slot >= 0);
     }
 }
 
@@ -1376,11 +1436,21 @@ static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     a.fl_cnt = L.fl_cnt;
     a.fl_exact = L.fl_exact;
     a.fl_over = L.fl_over;
+    a.fl_sec = L.fl_sec;
+    a.rs_out = L.fl_sec ? L.rs_out : nullptr;
+    a.rs_out_stride = L.rs_out_stride;
+    a.rs_cap = L.rs_cap;
+    a.rs_count = L.rs_count;
+    a.rs_in = form == RPD_PREC_FP64 ? L.rs_in : nullptr;
+    a.rs_in_stride = L.rs_in_stride;
+    a.rs_in_P = L.rs_in_P;
+    a.rslot = L.rslot;
+    a.lastnb = L.lastnb;
     a.exact_final = form == RPD_PREC_FP64 ? 1 : 0;
     return RPD_OK;
 }
 
-int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out) {
+int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out, bool wait) {
     *nq_out = 0;
     if (L.n_ids <= 0) return RPD_OK;
     ClipArgs a;
@@ -1419,6 +1489,7 @@ int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out)
     if (L.fl_cnt && L.fl_cnt_host &&
         hipMemcpyAsync(L.fl_cnt_host, L.fl_cnt, kFailCounters * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
         return RPD_E_DEVICE;
+    if (!wait) return RPD_OK;
     return hipStreamSynchronize(st) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }
 

@@ -46,8 +46,10 @@ struct DevBuf {
 int rpd_knn_dim(int64_t n);
 int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch, int *perm,
                   int *counters, void *stored, double4 *stored64);
+// K nearest of each query (stored ids ids[0..n_ids), nullptr: all), or, with
+// lower set, the K after the neighbour lower[q] (stored-id indexed).
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
-                   const int *ids, int64_t n_ids, int K, int *out);
+                   const int *ids, int64_t n_ids, int K, int *out, const int *lower = nullptr);
 
 // ---- clip (rpd_clip.hip) --------------------------------------------------
 struct DomainDev {
@@ -89,9 +91,17 @@ struct ClipLaunch {
     hipEvent_t ev_mid;     // recorded after the bisector kernel (nullable)
     // failure list the kernels append to (nullable): kFailCounters counters
     // ([0] exact entries, [1] overflow entries, [2 + s] cells per status s,
-    // [13] largest triangle union over T), then the stored ids
-    int *fl_cnt, *fl_exact, *fl_over;
+    // [13] largest triangle union over T, [14] resumable entries), then the
+    // stored ids
+    int *fl_cnt, *fl_exact, *fl_over, *fl_sec;
     int *fl_cnt_host;      // host copy of the counters, filled by rpd_launch_cells (nullable)
+    // resumable cells (security radius not reached), see ClipArgs in rpd_clip.hip
+    char *rs_out;          // records written by this pass (nullable)
+    int rs_out_stride, rs_cap;
+    int *rs_count;
+    const char *rs_in;     // records this pass resumes from (nullable; FP64 form)
+    int rs_in_stride, rs_in_P;
+    int *rslot, *lastnb;   // per stored id
 };
 
 constexpr int kFailCounters = 16;
@@ -123,8 +133,9 @@ size_t rpd_clip_record_bytes(int form, int P, int V);
 // (-1: none).
 int rpd_clip_max_T(int form, int P, int V);
 // form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32.  The bisector-loop
-// kernel of L.n_ids cells; waits for it and returns the number of cells it
-// queued (and the failure counters when L.fl_cnt_host is set).
-int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq);
+// kernel of L.n_ids cells, then copies to the host the number of cells it
+// queued (and the failure counters when L.fl_cnt_host is set); waits for
+// them unless wait is false (the caller synchronizes st before reading).
+int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq, bool wait = true);
 // The finish kernel over the nq records the bisector kernel queued (async).
 int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq);

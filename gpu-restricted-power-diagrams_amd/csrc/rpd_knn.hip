@@ -195,6 +195,9 @@ struct KnnArgs {
     int dim;
     int K;
     int *out;  // n_ids * K stored ids, -1 padded
+    // nullable: per stored id, a neighbour already known; only keys above
+    // its key are listed (the K after it)
+    const int *lower;
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -234,6 +237,12 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
 #pragma unroll
     for (int s = 0; s < S; ++s) top[s] = KeyT::inf();
     KeyT kth = KeyT::inf();
+    const bool has_low = a.lower != nullptr;
+    KeyT klow = KeyT::inf();
+    if (has_low) {
+        const int L = a.lower[q];
+        klow = KeyT::make(dist2_unfused<R>(p, a.pts[L]), uint32_t(a.perm[L]), uint32_t(L));
+    }
     const int kl = (K - 1) & 63, ks = (K - 1) >> 6;
 
     auto refresh_kth = [&]() {
@@ -263,7 +272,7 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
     };
 
     auto merge_batch = [&](KeyT c) {
-        if (!(c < kth)) c = KeyT::inf();
+        if (!(c < kth) || (has_low && !(klow < c))) c = KeyT::inf();
         uint64_t m = ballot(!c.is_inf());
         if (m == 0) return;
         if (__popcll(m) <= 6) {
@@ -448,14 +457,14 @@ int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int
 }
 
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
-                   const int *ids, int64_t n_ids, int K, int *out) {
+                   const int *ids, int64_t n_ids, int K, int *out, const int *lower) {
     if (n_ids <= 0) return RPD_OK;
     if (K <= 0 || K > 256) return RPD_E_LIMIT;
     const unsigned blocks = unsigned((n_ids + kKnnWavesPerBlock - 1) / kKnnWavesPerBlock);
     const dim3 tb(64 * kKnnWavesPerBlock);
 #define RPD_KNN_LAUNCH(R, V4)                                                                       \
     do {                                                                                            \
-        KnnArgs<R> a{static_cast<const V4 *>(stored), perm, counters, ids, n_ids, dim, K, out};     \
+        KnnArgs<R> a{static_cast<const V4 *>(stored), perm, counters, ids, n_ids, dim, K, out, lower}; \
         if (K <= 64)                                                                                \
             knn_kernel<R, 1><<<blocks, tb, 0, st>>>(a);                                             \
         else if (K <= 128)                                                                          \
