@@ -20,7 +20,10 @@
 //   of pass 1's bisector kernel (and theirs, and so on) on a second stream
 //   while pass 1's finish kernel still runs, then the failures of that
 //   finish kernel, each chain growing its own caps.  A cell's result does not
-//   depend on the caps it succeeds with.
+//   depend on the caps it succeeds with.  A cell whose list ran out keeps its
+//   state (planes, vertices, last neighbour) when there is room; the next
+//   pass lists only the neighbours after that one and clips on from the
+//   state, which is the clip sequence of one run with the longer list.
 // Boundary cells are processed in the same pass as interior cells (the
 // reference defers them with a T = 0 first pass and recomputes them).
 
@@ -104,10 +107,6 @@ int set_err(rpd_ctx *c, int code, const char *fmt, ...) {
         if (r_ != RPD_OK) return set_err(ctx, r_, "%s failed (%d)", what, r_); \
     } while (0)
 
-// failed cells of a list: compacts stored ids whose status (by original
-// index) is not success into two lists, with one atomic per wave and list,
-// and counts statuses in LDS with one atomic per block and status.
-// per-block minimum of the weights of non-border seeds
 // per-block minimum of the weights of non-border seeds (out[b]) and of all
 // weights (out[gridDim.x + b])
 __global__ __launch_bounds__(256) void wmin_kernel(const double4 *__restrict__ seeds, int64_t n,
