@@ -39,8 +39,10 @@
 
 #include "rpd_internal.h"
 
+// A domain keeps no pointer to the context that made it: it may outlive that
+// context and is shared by the contexts of one device.
 struct rpd_domain {
-    rpd_ctx *ctx = nullptr;
+    int device = 0;
     DomainDev dev;
     DevBuf flags, centre, pts, tris, tlist, toffs, tvoffs, tvox, tinfo, dist, dvox;
     int64_t bytes = 0;
@@ -278,7 +280,7 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     if (g->size > 1024) return set_err(c, RPD_E_LIMIT, "rpd_domain_create: at most 1024 voxels per axis");
     hipSetDevice(c->device);
     auto *d = new rpd_domain();
-    d->ctx = c;
+    d->device = c->device;
     const size_t nv = size_t(g->size) * g->size * g->size;
     auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
         if (bytes == 0) bytes = 4;
@@ -385,8 +387,9 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
 
 void rpd_domain_destroy(rpd_domain *d) {
     if (!d) return;
-    hipSetDevice(d->ctx->device);
-    hipStreamSynchronize(d->ctx->stream);
+    // wait for the work of any context that may still read the grid
+    hipSetDevice(d->device);
+    hipDeviceSynchronize();
     delete d;
 }
 
@@ -432,7 +435,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     if (prm->K <= 0 || prm->K > 256 || prm->P < 7 || prm->P > 255 || prm->V < 8 || prm->V > 256 || prm->T < 0)
         return set_err(c, RPD_E_ARG, "rpd_solve: caps out of range (0<K<=256, 7<=P<=255, 8<=V<=256, T>=0)");
     // a domain is read-only device memory: any context on its device may use it
-    if (dom && dom->ctx->device != c->device) return set_err(c, RPD_E_ARG, "rpd_solve: domain on another device");
+    if (dom && dom->device != c->device) return set_err(c, RPD_E_ARG, "rpd_solve: domain on another device");
     hipSetDevice(c->device);
     const int64_t n = c->n;
     const int form0 = prm->precision;

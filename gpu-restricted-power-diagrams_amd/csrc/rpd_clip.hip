@@ -126,6 +126,8 @@ __device__ __forceinline__ void note_failure(const ClipArgs &a, int sid, int st,
 }
 
 __device__ __forceinline__ uint32_t pack3(uint32_t a, uint32_t b, uint32_t c) { return a | (b << 8) | (c << 16); }
+// no vertex (plane triples use 24 bits)
+constexpr uint32_t kNoTri = 0xffffffffu;
 __device__ __forceinline__ uint32_t t0(uint32_t t) { return t & 0xff; }
 __device__ __forceinline__ uint32_t t1(uint32_t t) { return (t >> 8) & 0xff; }
 __device__ __forceinline__ uint32_t t2(uint32_t t) { return (t >> 16) & 0xff; }
@@ -570,9 +572,10 @@ void clip_cells_kernel(ClipArgs a) {
         c.MXYZ[s] = c.MW[s] = 0.f;
     }
     if (resume) {
-        // ---- the state a previous pass left: planes, then the vertices ----
+        // ---- the state a previous pass left: planes, then the vertices at
+        // their lanes (nent = 64 x the S of the pass that left it) ----------
         const RecHeader hd = *reinterpret_cast<const RecHeader *>(rs_rec);
-        const int nvalid = uni(hd.nvalid);
+        const int nent = uni(hd.nvalid);
         c.nb_plane = uni(hd.nb_plane);
         const Pl<RP> *rpl = reinterpret_cast<const Pl<RP> *>(rs_rec + sizeof(RecHeader));
         const uint32_t *rtr =
@@ -582,8 +585,9 @@ void clip_cells_kernel(ClipArgs a) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int j = s * 64 + lane;
-            if (j < nvalid) c.set_vertex_lds(s, rtr[j]);
-            c.valid[s] = ballot(j < nvalid);
+            const uint32_t tr = j < nent ? rtr[j] : kNoTri;
+            if (tr != kNoTri) c.set_vertex_lds(s, tr);
+            c.valid[s] = ballot(tr != kNoTri);
         }
     } else {
         // ---- CCInit: bounding cube in the local frame ----------------------
@@ -739,9 +743,13 @@ void clip_cells_kernel(ClipArgs a) {
                             : a.rec + size_t(slot) * size_t(a.rec_stride);
         Pl<RP> *rpl = reinterpret_cast<Pl<RP> *>(rec + sizeof(RecHeader));
         uint32_t *rtr = reinterpret_cast<uint32_t *>(rec + sizeof(RecHeader) + size_t(P) * sizeof(Pl<RP>));
+        // a queued cell's vertices are compacted; a kept state holds each
+        // triple at its lane (kNoTri where none), so that the resumed cell
+        // has the lanes, and so the sums, of one uninterrupted run
         int nv = 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) nv += __popcll(c.valid[s]);
+        if (ran_out) nv = 64 * S;
         if (lane == 0) *reinterpret_cast<RecHeader *>(rec) = RecHeader{sid, c.nb_plane, nv, kbase + kused};
         wsync();
         for (int p = lane; p < c.nb_plane; p += 64) {
@@ -756,7 +764,11 @@ void clip_cells_kernel(ClipArgs a) {
         int carry = 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if ((c.valid[s] >> lane) & 1) rtr[carry + mbcnt(c.valid[s])] = c.tri[s];
+            const bool v = (c.valid[s] >> lane) & 1;
+            if (ran_out)
+                rtr[64 * s + lane] = v ? c.tri[s] : kNoTri;
+            else if (v)
+                rtr[carry + mbcnt(c.valid[s])] = c.tri[s];
             carry += __popcll(c.valid[s]);
         }
     }
@@ -792,9 +804,11 @@ __device__ __forceinline__ uint32_t box_neighbours(bool xl, bool xh, bool yl, bo
 // re-queue passes whose run time is set by their heaviest cell): every wave
 // rebuilds the cell and its triangle list, wave w takes the triangles
 // k = w, w + W, ...  Each triangle starts from the same backed-up cell, so its
-// contribution does not depend on which wave computes it; contributions go to
-// LDS by triangle index and wave 0 sums them in one fixed order, so the result
-// of a cell is the same bit for bit for every W and every batch it is in.
+// contribution does not depend on which wave computes it.  The contributions
+// are summed in increasing k, one after another: with W = 1 as they come (a
+// wave-uniform running sum), with W > 1 from LDS, where the waves put them by
+// triangle index, by wave 0.  Both make the same additions, so the result of
+// a cell is the same bit for bit for every W and every batch it is in.
 //
 // Cone planes are built lane-parallel, 16 triangles x 4 planes per batch.  A
 // triangle one of whose four planes removes every vertex of the cell with a
@@ -806,6 +820,8 @@ __device__ __forceinline__ uint32_t box_neighbours(bool xl, bool xh, bool yl, bo
 // outcome is the reference's.
 // ---------------------------------------------------------------------------
 template <typename R, typename RP, bool FILTER, int S>
+// 4 waves per SIMD for S = 1 (128 VGPRs): measured at 1M seeds against 5 and
+// 6, whose spills cost more than the extra residency gives.
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == 1 ? 4 : 1)))
 void finish_cells_kernel(ClipArgs a, int nq, int W) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
@@ -818,8 +834,8 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     if (slot >= nq) return;  // uniform over the waves of a cell
 
     char *cell_l = lds_raw + size_t(ci) * a.cell_stride;
-    double4 *contrib = reinterpret_cast<double4 *>(cell_l);              // [T]
-    int *fails = reinterpret_cast<int *>(cell_l + 32 * size_t(a.T));  // [W][2]
+    int *fails = reinterpret_cast<int *>(cell_l);                  // [W][2]
+    double4 *contrib = reinterpret_cast<double4 *>(cell_l + 64);  // [T], W > 1 only
     WaveCell<R, RP, FILTER, S> c;
     int *tu = nullptr, *goo = nullptr;
     Pl<RP> *cpl = nullptr;
@@ -1169,6 +1185,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     // id.  Triangles are visited here in list order, so after a failure only
     // triangles of smaller id are still clipped.
     int fail_t = 0x7fffffff, fail_st = 0;
+    double sm = 0, sx = 0, sy = 0, sz = 0;  // running sums of the contributions (W = 1; wave-uniform)
     if (ntri > 0) {
         // backup (ConvexCell.cl:692-715): the vertex registers and the plane count
         const WaveCell<R, RP, FILTER, S> bk = c;
@@ -1308,7 +1325,14 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                     }
                     c = bk;
                 }
-                if (lane == 0) contrib[k] = ck;
+                if (W == 1) {
+                    sx = uni(sx + ck.x);
+                    sy = uni(sy + ck.y);
+                    sz = uni(sz + ck.z);
+                    sm = uni(sm + ck.w);
+                } else if (lane == 0) {
+                    contrib[k] = ck;
+                }
             }
             wsync();  // the next batch overwrites cpl / goo
         }
@@ -1335,18 +1359,18 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         fail(fs);
         return;
     }
-    // fixed summation order: lane-strided partial sums, one wave reduction,
-    // then the whole-cell term
-    double sm = 0, sx = 0, sy = 0, sz = 0;
-    for (int k = lane; k < ntri; k += 64) {
-        const double4 q = contrib[k];
-        sx += q.x;
-        sy += q.y;
-        sz += q.z;
-        sm += q.w;
+    // fixed summation order: the contributions in increasing k, then the
+    // whole-cell term
+    if (W > 1) {
+        for (int k = 0; k < ntri; ++k) {
+            const double4 q = contrib[k];
+            sx += q.x;
+            sy += q.y;
+            sz += q.z;
+            sm += q.w;
+        }
     }
-    const double acc_m = cadd.w + wave_sum(sm), acc_x = cadd.x + wave_sum(sx), acc_y = cadd.y + wave_sum(sy),
-                 acc_z = cadd.z + wave_sum(sz);
+    const double acc_m = cadd.w + sm, acc_x = cadd.x + sx, acc_y = cadd.y + sy, acc_z = cadd.z + sz;
     // local first moments -> global frame
     finish(RPD_SUCCESS, acc_x + acc_m * S0.x, acc_y + acc_m * S0.y, acc_z + acc_m * S0.z, acc_m);
 }
@@ -1367,12 +1391,14 @@ static size_t lds_limit() {
     return size_t(v);
 }
 
-// LDS of one finish-kernel cell handled by a single wave: the cell's
-// contributions (T) and fail slots, then the wave's cell copy.
+// LDS of the cell part of a finish-kernel workgroup: fail slots, then, when
+// W > 1 waves share the cell, its contributions (T).
+static size_t finish_cell_stride(int T, int W) { return W > 1 ? (64 + 32 * size_t(T) + 15) & ~size_t(15) : 64; }
+
+// LDS of one finish-kernel cell handled by a single wave.
 static size_t finish_cell_lds(int form, int P, int V, int T) {
     const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
-    const size_t cstride = (32 * size_t(T) + 8 * 8 + 15) & ~size_t(15);
-    return cstride + finish_wave_lds(P, S, T, plane_bytes(form), true);
+    return finish_cell_stride(T, 1) + finish_wave_lds(P, S, T, plane_bytes(form), true);
 }
 
 int rpd_clip_max_T(int form, int P, int V) {
@@ -1463,7 +1489,8 @@ int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out,
     {
         const size_t stride = finish_wave_lds(L.P, S, 0, form == RPD_PREC_FP64 ? 32 : 16);  // planes in R
         a.lds_stride = int(stride);
-        const int wpb = 4;
+        // one wave per workgroup: each cell's wave starts and ends on its own
+        const int wpb = 1;
         const unsigned blocks = unsigned((L.n_ids + wpb - 1) / wpb);
         const size_t shm = stride * wpb;
 #define RPD_CLIP_A(R, RP, F)                                                          \
@@ -1504,19 +1531,19 @@ int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq) {
     {
         const size_t pl = plane_bytes(form);
         const size_t wstride = finish_wave_lds(L.P, S, a.T, pl, true);
-        const size_t cstride = (32 * size_t(a.T) + 8 * 8 + 15) & ~size_t(15);  // contrib[T], fails[8][2]
         const size_t kLdsMax = lds_limit();
-        if (cstride + wstride > kLdsMax) return RPD_E_LIMIT;
+        if (finish_cell_stride(a.T, 1) + wstride > kLdsMax) return RPD_E_LIMIT;
+        int W = nq <= 2048 ? 8 : (nq <= 8192 ? 4 : (nq <= 32768 ? 2 : 1));
+        while (W > 1 && finish_cell_stride(a.T, W) + W * wstride > kLdsMax) W >>= 1;
+        const size_t cstride = finish_cell_stride(a.T, W);
         a.lds_stride = int(wstride);
         a.cell_stride = int(cstride);
-        int W = nq <= 2048 ? 8 : (nq <= 8192 ? 4 : (nq <= 32768 ? 2 : 1));
-        while (W > 1 && cstride + W * wstride > kLdsMax) W >>= 1;
-        int wpb = W;
-        if (W == 1) {
-            wpb = 4;
-            while (wpb > 1 && wpb * (cstride + wstride) > kLdsMax) wpb >>= 1;
-        }
-        const int cpb = wpb / W;
+        // One cell per workgroup: a workgroup keeps its LDS until its last
+        // wave ends, and cell costs are skewed (interior cells are cheap), so
+        // several cells per workgroup would hold the LDS of finished cells for
+        // the slowest one.  LDS bounds the residency of this kernel.
+        const int wpb = W;
+        const int cpb = 1;
         const unsigned blocks = unsigned((nq + cpb - 1) / cpb);
         const size_t shm = size_t(cpb) * cstride + size_t(wpb) * wstride;
 #define RPD_CLIP_B(R, RP, F)                                                                  \

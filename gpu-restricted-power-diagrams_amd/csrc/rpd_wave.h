@@ -30,6 +30,9 @@ __device__ __forceinline__ uint64_t uni(uint64_t v) {
     const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32))));
     return (uint64_t(hi) << 32) | lo;
 }
+__device__ __forceinline__ double uni(double v) {
+    return __longlong_as_double((long long)uni(uint64_t(__double_as_longlong(v))));
+}
 
 // value of v in lane l (l wave-uniform)
 __device__ __forceinline__ int rlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }

@@ -180,6 +180,8 @@ void rpd_default_params(rpd_params *out);
 /* Replaces VoroAlgo::setGrid / GridAccess::init (voronoi.cpp:1079-1082,
    78-144): copy the grid to HBM; it stays resident until rpd_domain_destroy. */
 int rpd_domain_create(rpd_ctx *ctx, const rpd_grid_desc *grid, rpd_domain **out);
+/* A domain may be destroyed before or after the context that created it; it
+   may be used by any context of the same device. */
 void rpd_domain_destroy(rpd_domain *dom);
 
 /* Replaces VoroAlgo::setInputPoints(std::vector<real>) (voronoi.cpp:1098-1107)

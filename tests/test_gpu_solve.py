@@ -119,3 +119,48 @@ def test_domain_cover_skip(rpd, ctx, prec):
     assert ev <= 1e-12 and em <= 1e-12
     # the cells the cover test zeroed have (numerically) zero volume when clipped
     assert (np.abs(full[(got == 0).all(1), 3]) <= 1e-12 * 1e9 / n).all()
+
+
+@pytest.mark.parametrize("caps", [dict(K=64, P=40, V=60, T=8), dict(K=12, P=16, V=40, T=8)])
+def test_caps_do_not_change_results(rpd, ctx, caps):
+    """FP64 form: a solve whose small caps send cells through re-queue passes
+    (recomputed, or resumed after their list ran out, and restricted by
+    several waves per cell in the short queues) gives the same bits as one
+    whose caps are ample (every cell done in pass 1, one wave per cell)."""
+    n = 100000
+    h = (1.0 / n) ** (1 / 3)
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=41, wmax=0.25 * h * h * 1e6)
+    grid = rpd.Grid.shape(rpd.SHAPE_TORUS, 100, (0.5, 0.5, 0.5, 0.3, 0.12))
+    dom = ctx.domain(grid)
+    ctx.set_seeds(x)
+    st0 = ctx.solve(dom, K=64, P=40, V=60, T=100, precision=rpd.PREC_FP64, cull=0)
+    ref, ref_st = ctx.moments(), ctx.status()
+    assert st0["queued_first"] > 32768
+    st1 = ctx.solve(dom, precision=rpd.PREC_FP64, cull=0, max_extra_steps=12, **caps)
+    got, status = ctx.moments(), ctx.status()
+    print("caps", caps, st1["passes"], st1["pass_cells"], st1["status_hist"], flush=True)
+    assert st1["passes"] > 1
+    if caps["K"] == 64:  # only T overflows: short re-queue queues, several waves per cell
+        assert 0 < st1["pass_cells"][1] <= 32768
+    assert np.array_equal(status, ref_st)
+    assert np.array_equal(got, ref)
+    dom.close()
+
+
+def test_domain_outlives_its_context(rpd, ctx):
+    """A domain made by one context serves another after the first is
+    destroyed, and is destroyed last."""
+    n = 5000
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=5)
+    grid = rpd.Grid.shape(rpd.SHAPE_SPHERE, 32, (0.5, 0.5, 0.5, 0.45))
+    ctx.set_seeds(x)
+    dom0 = ctx.domain(grid)
+    ctx.solve(dom0, K=50, P=30, V=40, precision=rpd.PREC_FP64)
+    ref = ctx.moments()
+    dom0.close()
+    other = rpd.Context(0)
+    dom = other.domain(grid)
+    other.close()
+    ctx.solve(dom, K=50, P=30, V=40, precision=rpd.PREC_FP64)
+    assert np.array_equal(ctx.moments(), ref)
+    dom.close()
