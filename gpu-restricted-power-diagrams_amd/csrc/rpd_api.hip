@@ -16,10 +16,12 @@
 //       voronoi.cpp:1349-1356; here they are always recomputed); from the
 //       FP64 form they are final (a recomputation fails the same way);
 //     * empty_cell is final (volume 0);
-//   every re-queue pass runs in the FP64 form, in two chains: the failures
-//   of pass 1's bisector kernel (and theirs, and so on) on a second stream
-//   while pass 1's finish kernel still runs, then the failures of that
-//   finish kernel, each chain growing its own caps.  A cell's result does not
+//   every re-queue pass runs in the FP64 form, in two chains that run at
+//   once, each growing its own caps: the failures of pass 1's bisector
+//   kernel (and theirs, and so on), whose first pass is queued before pass
+//   1's finish kernel, and the failures of that finish kernel.  The host
+//   drives both by polling their streams; small counters come back through
+//   pinned memory, so that no copy waits for a busy stream.  A cell's result does not
 //   depend on the caps it succeeds with.  A cell whose list ran out keeps its
 //   state (planes, vertices, last neighbour) when there is room; the next
 //   pass lists only the neighbours after that one and clips on from the
@@ -35,6 +37,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rpd_internal.h"
@@ -53,15 +56,21 @@ struct PassBufs {
     DevBuf knn, rec, qcount;
 };
 // failed cells handed to the next pass: counters, then stored ids of the
-// exact-predicate failures, the overflows and the resumable cells
+// exact-predicate failures and the overflows (eo, 2n entries: exact ones
+// down from the middle, overflows up from it) and of the resumable cells
 struct FailBufs {
-    DevBuf cnt, exact, over, sec;
+    DevBuf cnt, eo, sec;
 };
+
+// pinned host words of a context: queue lengths (one per stream), then the
+// counters of each failure list
+constexpr int kPinCounters = 16;
+constexpr int kPinWords = kPinCounters + 6 * kFailCounters;
 
 struct rpd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;      // == streams[0]
-    hipStream_t streams[3] = {nullptr, nullptr, nullptr};
+    hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
     int64_t n = 0;
     DevBuf seeds;  // original order, FP64 xyzw
@@ -72,15 +81,18 @@ struct rpd_ctx {
     // outputs, original order
     DevBuf out, status, kused;
     // pass work buffers: up to three passes may run at once, one per stream
-    PassBufs pb[3];
-    FailBufs fl[4];
-    DevBuf knn, ids, ids1, cnt, wred, cull_scratch;
-    // states of resumable cells (two record buffers, their counters) and,
-    // per stored id, the record slot and the last neighbour used
-    DevBuf rs[2], rs_cnt, rslot, lastnb;
+    PassBufs pb[4];
+    // pass 1 writes lists 0 (bisector kernel) and 1 (finish kernel); the
+    // re-queue chains 2, 3 and 4, 5
+    FailBufs fl[6];
+    DevBuf knn, ids1, cnt, wred, cull_scratch;
+    // states of resumable cells (two record buffers per chain, their
+    // counters) and, per stored id, the record slot and the last neighbour
+    DevBuf rs[4], rs_cnt, rslot, lastnb;
     bool solved = false;
-    hipEvent_t ev[3][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
-    hipEvent_t dep = nullptr;  // orders work between streams
+    hipEvent_t ev[4][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
+    hipEvent_t dep[2] = {};    // per chain: orders work between its streams
+    int *pin = nullptr;        // kPinWords pinned host words
 };
 
 namespace {
@@ -236,7 +248,14 @@ int rpd_create(int device, rpd_ctx **out) {
                 rpd_destroy(c);
                 return RPD_E_DEVICE;
             }
-    if (hipEventCreateWithFlags(&c->dep, hipEventDisableTiming) != hipSuccess) {
+    for (auto &e : c->dep)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            rpd_destroy(c);
+            return RPD_E_DEVICE;
+        }
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->pin), kPinWords * sizeof(int), hipHostMallocDefault) !=
+        hipSuccess) {
+        c->pin = nullptr;
         rpd_destroy(c);
         return RPD_E_DEVICE;
     }
@@ -252,9 +271,11 @@ void rpd_destroy(rpd_ctx *c) {
     for (auto &es : c->ev)
         for (auto &e : es)
             if (e) hipEventDestroy(e);
-    if (c->dep) hipEventDestroy(c->dep);
+    for (auto &e : c->dep)
+        if (e) hipEventDestroy(e);
     for (auto &s : c->streams)
         if (s) hipStreamDestroy(s);
+    if (c->pin) hipHostFree(c->pin);
     delete c;
 }
 
@@ -454,14 +475,12 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     RPDCHK(c, c->out.ensure(size_t(n) * 32), "alloc out");
     RPDCHK(c, c->status.ensure(size_t(n) * 4), "alloc status");
     RPDCHK(c, c->kused.ensure(size_t(n) * 4), "alloc kused");
-    RPDCHK(c, c->ids.ensure(size_t(n) * 4), "alloc ids");
     for (auto &l : c->fl) {
         RPDCHK(c, l.cnt.ensure(kFailCounters * sizeof(int)), "alloc failure list");
-        RPDCHK(c, l.exact.ensure(size_t(n) * 4), "alloc failure list");
-        RPDCHK(c, l.over.ensure(size_t(n) * 4), "alloc failure list");
+        RPDCHK(c, l.eo.ensure(size_t(n) * 8), "alloc failure list");
         RPDCHK(c, l.sec.ensure(size_t(n) * 4), "alloc failure list");
     }
-    RPDCHK(c, c->rs_cnt.ensure(2 * sizeof(int)), "alloc resume counters");
+    RPDCHK(c, c->rs_cnt.ensure(4 * sizeof(int)), "alloc resume counters");
     RPDCHK(c, c->rslot.ensure(size_t(n) * 4), "alloc resume slots");
     RPDCHK(c, c->lastnb.ensure(size_t(n) * 4), "alloc resume slots");
     RPDCHK(c, c->cnt.ensure(64), "alloc cnt");
@@ -573,8 +592,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.qcount = b.qcount.as<int>();
         L.ev_mid = c->ev[sl][2];
         L.fl_cnt = c->fl[fl_out].cnt.as<int>();
-        L.fl_exact = c->fl[fl_out].exact.as<int>();
-        L.fl_over = c->fl[fl_out].over.as<int>();
+        L.fl_eo = c->fl[fl_out].eo.as<int>() + n;
         L.fl_sec = c->fl[fl_out].sec.as<int>();
         L.fl_cnt_host = fl_cnt_host;
         L.rs_in = rz.in;
@@ -586,12 +604,12 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.rs_count = c->rs_cnt.as<int>() + std::max(rz.out, 0);
         L.rslot = c->rslot.as<int>();
         L.lastnb = c->lastnb.as<int>();
-        return rpd_launch_cells(s, ps.form, L, &ps.nq, false);
+        return rpd_launch_cells(s, ps.form, L, c->pin + sl, false);
     };
-    // waits for the queue length, then the finish kernel (not waited for)
+    // the finish kernel of a pass whose bisector kernel (and queue length) is done
     auto pass_back = [&](Pass &ps, const ClipLaunch &L) -> int {
         hipStream_t s = c->streams[ps.slot];
-        if (hipStreamSynchronize(s) != hipSuccess) return RPD_E_DEVICE;
+        ps.nq = c->pin[ps.slot];
         const int r = rpd_launch_finish(s, ps.form, L, ps.nq);
         if (r != RPD_OK) return r;
         return hipEventRecord(c->ev[ps.slot][3], s) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
@@ -638,10 +656,6 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     auto clear_list = [&](int fl, hipStream_t s) {
         return hipMemsetAsync(c->fl[fl].cnt.ptr, 0, kFailCounters * sizeof(int), s) == hipSuccess;
     };
-    auto copy_ids = [&](int *dst, const DevBuf &src, int cnt_, hipStream_t s) {
-        return cnt_ <= 0 ||
-               hipMemcpyAsync(dst, src.ptr, size_t(cnt_) * 4, hipMemcpyDeviceToDevice, s) == hipSuccess;
-    };
     // Record buffer `which` for cap records of stride bytes, counter cleared.
     auto resume_buf = [&](int which, int64_t cap, int stride, hipStream_t s) -> int {
         const int r = c->rs[which].ensure(size_t(std::max<int64_t>(cap, 1)) * size_t(stride));
@@ -650,98 +664,147 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
                                                                                               : RPD_E_DEVICE;
     };
     // Where the previous pass of a chain left the states of its resumable
-    // cells (rs buffer, stride, plane room, K), or which = -1.
+    // cells (rs buffer, stride, plane room), or which = -1.
     struct Left {
         int which = -1, stride = 0, P = 0;
     };
-    // A chain of re-queue passes on stream slot sl: the first takes the
-    // failures of list `from` (host counters cnt), each pass writes its own
-    // failures to outs[0], outs[1], outs[0], ...  A pass resumes the cells
-    // whose state the previous one kept (their kNN lists only the neighbours
-    // after the last one used) and recomputes the others in FP64.  Pass k of
-    // the chain is accounted as pass index0 + k.  Returns the passes run, or -1.
-    auto run_chain = [&](int sl, int from, const int *cnt, const int outs[2], Caps cap, Left left, int index0,
-                         Caps &cap_out) -> int {
-        hipStream_t s = c->streams[sl];
-        int cur = from, k = 0, o = 0;
-        int cnt_cur[kFailCounters];
-        std::memcpy(cnt_cur, cnt, sizeof(cnt_cur));
-        for (; k < prm->max_extra_steps; ++k) {
-            const int n_exact = cnt_cur[0], n_over = cnt_cur[1], n_sec = cnt_cur[14];
-            const Caps nc = grow(cnt_cur, cap);
-            const bool caps_changed = nc.K != cap.K || nc.P != cap.P || nc.V != cap.V || nc.T != cap.T;
-            // exact-predicate failures (only listed from the FP32 forms) always
-            // go to FP64; overflows only when a cap grew
-            const bool resume = left.which >= 0 && n_sec > 0 && nc.K > cap.K;
-            const int64_t n_res = resume ? n_sec : 0;
-            const int64_t n_scr = n_exact + (caps_changed ? n_over + (resume ? 0 : n_sec) : 0);
-            if (n_res + n_scr == 0) break;
-            int *ids = c->ids.as<int>();
-            if (!copy_ids(ids, c->fl[cur].sec, int(n_res), s) || !copy_ids(ids + n_res, c->fl[cur].exact, n_exact, s))
-                return -1;
-            if (caps_changed && (!copy_ids(ids + n_res + n_exact, c->fl[cur].over, n_over, s) ||
-                                 (!resume && !copy_ids(ids + n_res + n_exact + n_over, c->fl[cur].sec, n_sec, s))))
-                return -1;
-            st.fallback_cells += n_exact;
-            const int nxt = outs[o];
-            if (!clear_list(nxt, s)) return -1;
-            // states of this pass's resumable cells go to the other buffer
-            Resume rz;
-            rz.out = left.which == 0 ? 1 : 0;
-            rz.out_stride = int(rpd_clip_record_bytes(RPD_PREC_FP64, nc.P, nc.V));
-            rz.out_cap = int(n_res + n_scr);
-            if (resume_buf(rz.out, rz.out_cap, rz.out_stride, s) != RPD_OK) return -1;
-            // the resumed cells on the chain's stream, the recomputed ones on
-            // stream 2, both at once
-            Pass pr, pz;
-            ClipLaunch Lr, Lz;
-            pr.slot = sl;
-            pz.slot = 2;
-            pr.form = pz.form = RPD_PREC_FP64;
-            pr.n_ids = n_res;
-            pz.n_ids = n_scr;
-            pr.caps[0] = nc.K - cap.K;  // the neighbours after the ones used
-            pz.caps[0] = nc.K;
-            pr.caps[1] = pz.caps[1] = nc.P;
-            pr.caps[2] = pz.caps[2] = nc.V;
-            pr.caps[3] = pz.caps[3] = nc.T;
-            Resume rr = rz;
-            // stream 2 starts after the id copies made on stream s
-            if (n_scr > 0 && (hipEventRecord(c->dep, s) != hipSuccess ||
-                              hipStreamWaitEvent(c->streams[2], c->dep, 0) != hipSuccess))
-                return -1;
-            if (n_res > 0) {
-                rr.in = static_cast<const char *>(c->rs[left.which].ptr);
-                rr.in_stride = left.stride;
-                rr.in_P = left.P;
-                if (pass_front(pr, ids, nxt, nullptr, rr, Lr) != RPD_OK) return -1;
-            }
-            if (n_scr > 0 && pass_front(pz, ids + n_res, nxt, nullptr, rz, Lz) != RPD_OK) return -1;
-            if (n_res > 0 && pass_back(pr, Lr) != RPD_OK) return -1;
-            if (n_scr > 0 && pass_back(pz, Lz) != RPD_OK) return -1;
-            if (hipStreamSynchronize(s) != hipSuccess || hipStreamSynchronize(c->streams[2]) != hipSuccess) return -1;
-            if (n_res > 0) account(pr, index0 + k);
-            if (n_scr > 0) account(pz, index0 + k);
-            if (hipMemcpyAsync(cnt_cur, c->fl[nxt].cnt.ptr, sizeof(cnt_cur), hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess)
-                return -1;
-            cap = nc;
-            left.which = rz.out;
-            left.stride = rz.out_stride;
-            left.P = nc.P;
-            cur = nxt;
-            o ^= 1;
+    // 1: done, 0: still running, -1: error
+    auto done = [&](hipStream_t s) {
+        const hipError_t e = hipStreamQuery(s);
+        return e == hipSuccess ? 1 : (e == hipErrorNotReady ? 0 : -1);
+    };
+    // A chain of re-queue passes, driven by polling so that two chains and
+    // pass 1's finish kernel run at once.  Each pass takes the failures of
+    // the previous one (list cur, counters cnt): the cells whose state was
+    // kept resume on stream sl (their kNN lists only the neighbours after the
+    // last one used), the others are recomputed in FP64 on stream sl2, both
+    // at once.  Pass k writes its failures to list outs[k % 2] and the states
+    // of its resumable cells to rs buffer rsb[k % 2].
+    struct Chain {
+        int sl = 0, sl2 = 0, dep = 0;
+        int outs[2] = {0, 0}, rsb[2] = {0, 0};
+        int cur = 0;
+        const int *cnt = nullptr;
+        Caps cap{0, 0, 0, 0}, nc{0, 0, 0, 0};
+        Left left;
+        int k = 0, nxt = 0;
+        int phase = 0;  // 0 idle, 1 bisector kernels queued, 2 finish kernels queued, 3 done, 4 waiting for pass 1
+        int64_t n_res = 0, n_scr = 0;
+        Pass pr, pz;
+        ClipLaunch Lr, Lz;
+        Resume rz;
+    };
+    auto cnt_pin = [&](int list) { return c->pin + kPinCounters + kFailCounters * list; };
+    // plans the chain's next pass from the counters and queues its kNN and
+    // bisector kernels (phase 1), or ends the chain (phase 3)
+    auto chain_front = [&](Chain &ch) -> int {
+        if (ch.k >= prm->max_extra_steps) {
+            ch.phase = 3;
+            return RPD_OK;
         }
-        cap_out = cap;
-        return k;
+        const int *cnt = ch.cnt;
+        const int n_exact = cnt[0], n_over = cnt[1], n_sec = cnt[14];
+        const Caps nc = grow(cnt, ch.cap);
+        const bool caps_changed = nc.K != ch.cap.K || nc.P != ch.cap.P || nc.V != ch.cap.V || nc.T != ch.cap.T;
+        // exact-predicate failures (only listed from the FP32 forms) always
+        // go to FP64; overflows only when a cap grew; kept states when K grew
+        // (else they would run out again: they end as they are)
+        ch.n_res = n_sec > 0 && ch.left.which >= 0 && nc.K > ch.cap.K ? n_sec : 0;
+        ch.n_scr = n_exact + (caps_changed ? n_over : 0);
+        if (ch.n_res + ch.n_scr == 0) {
+            ch.phase = 3;
+            return RPD_OK;
+        }
+        st.fallback_cells += n_exact;
+        hipStream_t s = c->streams[ch.sl], s2 = c->streams[ch.sl2];
+        ch.nc = nc;
+        ch.nxt = ch.outs[ch.k & 1];
+        if (!clear_list(ch.nxt, s)) return RPD_E_DEVICE;
+        ch.rz = Resume{};
+        ch.rz.out = ch.rsb[ch.k & 1];
+        ch.rz.out_stride = int(rpd_clip_record_bytes(RPD_PREC_FP64, nc.P, nc.V));
+        ch.rz.out_cap = int(ch.n_res + ch.n_scr);
+        int r = resume_buf(ch.rz.out, ch.rz.out_cap, ch.rz.out_stride, s);
+        if (r != RPD_OK) return r;
+        ch.pr = Pass{};
+        ch.pz = Pass{};
+        ch.pr.slot = ch.sl;
+        ch.pz.slot = ch.sl2;
+        ch.pr.form = ch.pz.form = RPD_PREC_FP64;
+        ch.pr.n_ids = ch.n_res;
+        ch.pz.n_ids = ch.n_scr;
+        ch.pr.caps[0] = nc.K - ch.cap.K;  // the neighbours after the ones used
+        ch.pz.caps[0] = nc.K;
+        ch.pr.caps[1] = ch.pz.caps[1] = nc.P;
+        ch.pr.caps[2] = ch.pz.caps[2] = nc.V;
+        ch.pr.caps[3] = ch.pz.caps[3] = nc.T;
+        // the recomputed cells start once the list and the record counter are cleared
+        if (ch.n_scr > 0 && (hipEventRecord(c->dep[ch.dep], s) != hipSuccess ||
+                             hipStreamWaitEvent(s2, c->dep[ch.dep], 0) != hipSuccess))
+            return RPD_E_DEVICE;
+        if (ch.n_res > 0) {
+            Resume rr = ch.rz;
+            rr.in = static_cast<const char *>(c->rs[ch.left.which].ptr);
+            rr.in_stride = ch.left.stride;
+            rr.in_P = ch.left.P;
+            r = pass_front(ch.pr, c->fl[ch.cur].sec.as<int>(), ch.nxt, nullptr, rr, ch.Lr);
+            if (r != RPD_OK) return r;
+        }
+        if (ch.n_scr > 0) {
+            // exact entries and (when recomputed) the overflows are one range
+            r = pass_front(ch.pz, c->fl[ch.cur].eo.as<int>() + n - n_exact, ch.nxt, nullptr, ch.rz, ch.Lz);
+            if (r != RPD_OK) return r;
+        }
+        ch.phase = 1;
+        return RPD_OK;
+    };
+    // advances the chain as far as its streams allow, without waiting
+    auto chain_poll = [&](Chain &ch, bool &progress) -> int {
+        hipStream_t s = c->streams[ch.sl], s2 = c->streams[ch.sl2];
+        if (ch.phase == 1) {
+            const int d1 = done(s), d2 = ch.n_scr > 0 ? done(s2) : 1;
+            if (d1 < 0 || d2 < 0) return RPD_E_DEVICE;
+            if (!d1 || !d2) return RPD_OK;
+            if (ch.n_res > 0 && pass_back(ch.pr, ch.Lr) != RPD_OK) return RPD_E_DEVICE;
+            if (ch.n_scr > 0 && pass_back(ch.pz, ch.Lz) != RPD_OK) return RPD_E_DEVICE;
+            // the counters of the pass, once both finish kernels are done
+            if (ch.n_scr > 0 && (hipEventRecord(c->dep[ch.dep], s2) != hipSuccess ||
+                                 hipStreamWaitEvent(s, c->dep[ch.dep], 0) != hipSuccess))
+                return RPD_E_DEVICE;
+            if (hipMemcpyAsync(cnt_pin(ch.nxt), c->fl[ch.nxt].cnt.ptr, kFailCounters * sizeof(int),
+                               hipMemcpyDeviceToHost, s) != hipSuccess)
+                return RPD_E_DEVICE;
+            ch.phase = 2;
+            progress = true;
+        }
+        if (ch.phase == 2) {
+            const int d = done(s);
+            if (d < 0) return RPD_E_DEVICE;
+            if (!d) return RPD_OK;
+            if (ch.n_res > 0) account(ch.pr, 1 + ch.k);
+            if (ch.n_scr > 0) account(ch.pz, 1 + ch.k);
+            ch.cap = ch.nc;
+            ch.left.which = ch.rz.out;
+            ch.left.stride = ch.rz.out_stride;
+            ch.left.P = ch.nc.P;
+            ch.cur = ch.nxt;
+            ch.cnt = cnt_pin(ch.nxt);
+            ch.k += 1;
+            ch.phase = 0;
+            progress = true;
+            return chain_front(ch);
+        }
+        return RPD_OK;
     };
 
     // Pass 1 on stream 0: the bisector kernel's failures go to list 0, the
-    // finish kernel's to list 1.  Two chains of re-queue passes follow:
-    //   * the bisector failures, known as soon as that kernel is done, run
-    //     on stream 1 while pass 1's finish kernel still runs (lists 2, 3);
-    //   * the finish kernel's failures, once it is done, on stream 0
-    //     (lists 0, 1), with the caps of pass 1 grown only for them.
+    // finish kernel's to list 1.  Two chains of re-queue passes follow, at
+    // once:
+    //   * chain A, the bisector failures, known as soon as that kernel is
+    //     done: its first pass is queued (streams 1, 2) before pass 1's
+    //     finish kernel, so that the two share the device from the start;
+    //   * chain B, the finish kernel's failures, once it is done (streams 0,
+    //     3), with the caps of pass 1 grown only for them.
     if (!clear_list(0, c->stream) || !clear_list(1, c->stream)) return set_err(c, RPD_E_DEVICE, "clear lists");
     Pass p1;
     p1.slot = 0;
@@ -751,7 +814,6 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     p1.caps[1] = P;
     p1.caps[2] = V;
     p1.caps[3] = T;
-    int cnt_a[kFailCounters] = {0}, cnt_b[kFailCounters] = {0};
     // states of pass 1's cells whose K neighbours do not reach their
     // security radius are kept (FP64 planes: FP64 and mixed forms), for up to
     // 40% of the cells; the others are recomputed from scratch
@@ -763,9 +825,8 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         left1.P = P;
         RPDCHK(c, resume_buf(0, cap_rs1, left1.stride, c->streams[0]), "alloc resume records");
     }
+    ClipLaunch L1{};
     {
-        // the bisector kernel appends to list 0 and the finish kernel to
-        // list 1: launch the two halves separately
         hipStream_t s = c->streams[0];
         PassBufs &b = c->pb[0];
         RPDCHK(c, b.knn.ensure(size_t(std::max<int64_t>(n1, 1)) * K * 4), "alloc knn");
@@ -777,7 +838,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
                               b.knn.as<int>()),
                "pass 1 knn");
         HIPCHK(c, hipEventRecord(c->ev[0][1], s));
-        ClipLaunch L{};
+        ClipLaunch &L = L1;
         L.seeds = c->stored64.as<double4>();
         L.perm = c->perm.as<int>();
         L.ids = ids1;
@@ -800,10 +861,9 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.qcount = b.qcount.as<int>();
         L.ev_mid = c->ev[0][2];
         L.fl_cnt = c->fl[0].cnt.as<int>();
-        L.fl_exact = c->fl[0].exact.as<int>();
-        L.fl_over = c->fl[0].over.as<int>();
+        L.fl_eo = c->fl[0].eo.as<int>() + n;
         L.fl_sec = c->fl[0].sec.as<int>();
-        L.fl_cnt_host = cnt_a;
+        L.fl_cnt_host = cnt_pin(0);
         if (left1.which == 0) {
             L.rs_out = static_cast<char *>(c->rs[0].ptr);
             L.rs_out_stride = left1.stride;
@@ -812,31 +872,54 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
             L.rslot = c->rslot.as<int>();
             L.lastnb = c->lastnb.as<int>();
         }
-        RPDCHK(c, rpd_launch_cells(s, form0, L, &p1.nq), "pass 1 cells");
+        RPDCHK(c, rpd_launch_cells(s, form0, L, c->pin + 0, true), "pass 1 cells");
+        p1.nq = c->pin[0];
         L.fl_cnt = c->fl[1].cnt.as<int>();
-        L.fl_exact = c->fl[1].exact.as<int>();
-        L.fl_over = c->fl[1].over.as<int>();
+        L.fl_eo = c->fl[1].eo.as<int>() + n;
         L.fl_sec = c->fl[1].sec.as<int>();
-        RPDCHK(c, rpd_launch_finish(s, form0, L, p1.nq), "pass 1 finish");
-        HIPCHK(c, hipEventRecord(c->ev[0][3], s));
+        L.fl_cnt_host = nullptr;
     }
     const Caps caps1{K, P, V, T};
-    Caps cap_a = caps1, cap_b = caps1;
-    int steps_a = 0, steps_b = 0;
-    if (prm->max_extra_steps > 0) {
-        const int outs_a[2] = {2, 3};
-        steps_a = run_chain(1, 0, cnt_a, outs_a, caps1, left1, 1, cap_a);
-        if (steps_a < 0) return set_err(c, RPD_E_DEVICE, "re-queue chain (bisector failures)");
+    Chain ca, cb;
+    ca.sl = 1, ca.sl2 = 2, ca.dep = 0;
+    ca.outs[0] = 2, ca.outs[1] = 3;
+    ca.rsb[0] = 1, ca.rsb[1] = 0;
+    ca.cur = 0;
+    ca.cnt = cnt_pin(0);
+    ca.cap = caps1;
+    ca.left = left1;
+    cb.sl = 0, cb.sl2 = 3, cb.dep = 1;
+    cb.outs[0] = 4, cb.outs[1] = 5;
+    cb.rsb[0] = 2, cb.rsb[1] = 3;
+    cb.cur = 1;
+    cb.cnt = cnt_pin(1);
+    cb.cap = caps1;
+    cb.phase = 4;
+    RPDCHK(c, chain_front(ca), "re-queue chain (bisector failures)");
+    {
+        hipStream_t s = c->streams[0];
+        RPDCHK(c, rpd_launch_finish(s, form0, L1, p1.nq), "pass 1 finish");
+        HIPCHK(c, hipEventRecord(c->ev[0][3], s));
+        HIPCHK(c, hipMemcpyAsync(cnt_pin(1), c->fl[1].cnt.ptr, kFailCounters * sizeof(int), hipMemcpyDeviceToHost, s));
     }
-    HIPCHK(c, hipStreamSynchronize(c->streams[0]));
-    account(p1, 0);
-    if (prm->max_extra_steps > 0) {
-        HIPCHK(c, hipMemcpyAsync(cnt_b, c->fl[1].cnt.ptr, sizeof(cnt_b), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        const int outs_b[2] = {0, 1};
-        steps_b = run_chain(0, 1, cnt_b, outs_b, caps1, Left{}, 1, cap_b);
-        if (steps_b < 0) return set_err(c, RPD_E_DEVICE, "re-queue chain (finish failures)");
+    while (ca.phase != 3 || cb.phase != 3) {
+        bool progress = false;
+        if (cb.phase == 4) {
+            const int d = done(c->streams[0]);
+            if (d < 0) return set_err(c, RPD_E_DEVICE, "pass 1 finish");
+            if (d) {
+                account(p1, 0);
+                cb.phase = 0;
+                progress = true;
+                RPDCHK(c, chain_front(cb), "re-queue chain (finish failures)");
+            }
+        }
+        RPDCHK(c, chain_poll(ca, progress), "re-queue chain (bisector failures)");
+        RPDCHK(c, chain_poll(cb, progress), "re-queue chain (finish failures)");
+        if (!progress) std::this_thread::yield();
     }
+    const int steps_a = ca.k, steps_b = cb.k;
+    const Caps cap_a = ca.cap, cap_b = cb.cap;
     st.passes = 1 + std::max(steps_a, steps_b);
     K = std::max(cap_a.K, cap_b.K);
     P = std::max(cap_a.P, cap_b.P);

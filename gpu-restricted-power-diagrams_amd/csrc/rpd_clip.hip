@@ -86,10 +86,12 @@ struct ClipArgs {
     char *rec;            // queue of finished cells (records of rec_stride bytes)
     int rec_stride;
     int *qcount;          // number of queued records
-    // failed cells for the next pass (nullable): fl_cnt[0] / [1] entries of
-    // fl_exact / fl_over, [2 + s] cells that ended with status s, [13] the
-    // largest triangle union of a cell that overflowed T
-    int *fl_cnt, *fl_exact, *fl_over;
+    // failed cells for the next pass (nullable): fl_cnt[0] exact-predicate
+    // entries, stored at fl_eo[-1], fl_eo[-2], ..., [1] overflow entries at
+    // fl_eo[0], fl_eo[1], ... (so that both kinds together are one range),
+    // [2 + s] cells that ended with status s, [13] the largest triangle union
+    // of a cell that overflowed T
+    int *fl_cnt, *fl_eo;
     int exact_final;      // FP64 form: exact-predicate failures are final
     // A cell whose list runs out before its security radius is reached can be
     // resumed: its state goes to rs_out (records of rs_out_stride bytes, slot
@@ -122,7 +124,7 @@ __device__ __forceinline__ void note_failure(const ClipArgs &a, int sid, int st,
         return;
     }
     const int k = atomicAdd(a.fl_cnt + (exact ? 0 : 1), 1);
-    (exact ? a.fl_exact : a.fl_over)[k] = sid;
+    a.fl_eo[exact ? -1 - k : k] = sid;
 }
 
 __device__ __forceinline__ uint32_t pack3(uint32_t a, uint32_t b, uint32_t c) { return a | (b << 8) | (c << 16); }
@@ -1460,8 +1462,7 @@ static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     a.rec_stride = int(rpd_clip_record_bytes(form, L.P, L.V));
     a.qcount = L.qcount;
     a.fl_cnt = L.fl_cnt;
-    a.fl_exact = L.fl_exact;
-    a.fl_over = L.fl_over;
+    a.fl_eo = L.fl_eo;
     a.fl_sec = L.fl_sec;
     a.rs_out = L.fl_sec ? L.rs_out : nullptr;
     a.rs_out_stride = L.rs_out_stride;

@@ -92,8 +92,9 @@ struct ClipLaunch {
     // failure list the kernels append to (nullable): kFailCounters counters
     // ([0] exact entries, [1] overflow entries, [2 + s] cells per status s,
     // [13] largest triangle union over T, [14] resumable entries), then the
-    // stored ids
-    int *fl_cnt, *fl_exact, *fl_over, *fl_sec;
+    // stored ids: exact entries at fl_eo[-1 - k], overflows at fl_eo[k],
+    // resumable cells at fl_sec[k]
+    int *fl_cnt, *fl_eo, *fl_sec;
     int *fl_cnt_host;      // host copy of the counters, filled by rpd_launch_cells (nullable)
     // resumable cells (security radius not reached), see ClipArgs in rpd_clip.hip
     char *rs_out;          // records written by this pass (nullable)

@@ -48,17 +48,18 @@ struct Request {
     double lat_ms = 0;
 };
 
-// Device bytes one solve of n seeds holds at its peak: seeds and their two
-// sorted copies, permutation, outputs, kNN lists and cell records of pass 1
-// and (bounded by the same size) of the re-queue pass that runs beside it,
-// id lists, the four failure lists, the cover flags, binning scratch, the
-// resume slots and last neighbours, and the kept states of pass 1's cells
-// whose list ran out (room for 2/5 of the cells, plus 1024).
+// Device bytes one solve of n seeds holds at its peak, per seed: the seeds
+// (32) and their two sorted copies (64), permutation (4), outputs, status and
+// K used (40), the pass-1 id list (4), six failure lists (72), resume slots
+// and last neighbours (8), cover flags (1), binning scratch (16), kNN lists
+// and cell records of pass 1 and, bounded by the same size, of the re-queue
+// passes that run beside it; plus the kept states of pass 1's cells whose
+// list ran out (room for 2/5 of the cells, plus 1024).
 int64_t footprint_bytes(int64_t n, const rpd_params &p) {
     const int64_t K = std::max(1, p.K), P = std::max(1, p.P);
     const int64_t S = p.V <= 64 ? 1 : (p.V <= 128 ? 2 : 4);
     const int64_t rec = 16 + P * 32 + 256 * S;
-    return n * (32 + 32 + 32 + 4 + 32 + 8 + 2 * (4 * K + rec) + 8 + 32 + 1 + 16 + 8) +
+    return n * (32 + 64 + 4 + 40 + 4 + 72 + 8 + 1 + 16 + 2 * (4 * K + rec)) +
            (std::min(n, n * 2 / 5 + 1024)) * rec + (int64_t(1) << 20);
 }
 
