@@ -65,6 +65,8 @@ struct FailBufs {
 // pinned host words of a context: queue lengths (one per stream), then the
 // counters of each failure list
 constexpr int kPinCounters = 16;
+// smallest triangle cap a solve uses
+constexpr int kTriangleFloor = 256;
 constexpr int kPinWords = kPinCounters + 6 * kFailCounters;
 
 struct rpd_ctx {
@@ -462,6 +464,14 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     const int form0 = prm->precision;
     const int kprec = form0 == RPD_PREC_FP64 ? 1 : 0;
     int K = std::min<int64_t>(prm->K, n - 1), P = prm->P, V = prm->V, T = prm->T;
+    // The triangle cap is raised to kTriangleFloor where it is lower: a cell
+    // done by one wave keeps 4 bytes of LDS per union entry, which costs the
+    // finish kernel no residency, and a cell's result does not depend on the
+    // cap it succeeds with; a lower cap only sends cells to re-queue passes.
+    T = std::max(T, std::min(kTriangleFloor, rpd_clip_max_T(form0, P, V)));
+    // Likewise the vertex cap is the room of the vertex slots it needs
+    // (64 per slot): a cap below it saves nothing.
+    V = V <= 64 ? 64 : (V <= 128 ? 128 : 256);
     rpd_stats st{};
     st.n = n;
     auto t_start = std::chrono::steady_clock::now();
@@ -734,7 +744,9 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         ch.pr.n_ids = ch.n_res;
         ch.pz.n_ids = ch.n_scr;
         ch.pr.caps[0] = nc.K - ch.cap.K;  // the neighbours after the ones used
-        ch.pz.caps[0] = nc.K;
+        // recomputed cells need the grown K only when some of them ran out
+        // of neighbours without a kept state (the record buffer was full)
+        ch.pz.caps[0] = cnt[2 + RPD_SECURITY_RADIUS_NOT_REACHED] > n_sec ? nc.K : ch.cap.K;
         ch.pr.caps[1] = ch.pz.caps[1] = nc.P;
         ch.pr.caps[2] = ch.pz.caps[2] = nc.V;
         ch.pr.caps[3] = ch.pz.caps[3] = nc.T;

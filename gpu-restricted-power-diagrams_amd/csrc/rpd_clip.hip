@@ -454,12 +454,23 @@ __device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, 
     if (tcor) *tcor = reinterpret_cast<double *>(L + off);
 }
 
+// A vertex of the finish kernel's cell as the lane-parallel skip tests read
+// it: homogeneous coordinates in the test precision and, for the certified
+// form, the magnitudes of their terms.
+template <typename R>
+struct VtxRec {
+    R X, Y, Z, W;
+    float MXYZ, MW;
+};
+constexpr size_t kVtxRecBytes = 40;  // sizeof(VtxRec<double>), the larger one
+
 // LDS bytes of one wave (pl: bytes of one stored plane; corners: the finish
-// kernel's triangle corners)
+// kernel's triangle corners and vertex copy)
 __host__ __device__ constexpr size_t finish_wave_lds(int P, int S, int T, size_t pl, bool corners = false) {
     return ((((((P * pl + 15) & ~size_t(15)) + ((2 * size_t(P) + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) +
                256 + 4 * size_t(T)) +
-              15) & ~size_t(15)) + 64 * pl + 64 + (corners ? 16 * 9 * 8 : 0) + 15) & ~size_t(15);
+              15) & ~size_t(15)) + 64 * pl + 64 + (corners ? 16 * 9 * 8 + 64 * size_t(S) * kVtxRecBytes : 0) +
+            15) & ~size_t(15);
 }
 
 // Power bisector of the seed S0 and neighbour B in the seed-local frame
@@ -1193,37 +1204,19 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         const WaveCell<R, RP, FILTER, S> bk = c;
         const bool may_skip = c.nb_plane + 4 <= P && nvalid + 6 <= a.V;
         const double lx0 = P0x - S0.x, ly0 = P0y - S0.y, lz0 = P0z - S0.z;
-        // The cone region of a triangle (apex A, beyond the triangle) lies
-        // strictly outside one of the cell's planes when the triangle's three
-        // corners do (with a margin) and the rays from A through them do not
-        // turn back towards the plane.  The cell is inside every plane of its
-        // list (each one cut it when added), so the two are disjoint: the
-        // reference's clip ends empty_cell (same caps guard as above).  The
-        // planes are tested one per lane.
         const double face_margin = sizeof(RP) == 8 ? 1e-7 : 1e-3;
-        auto cone_beyond_a_face = [&](int i) -> bool {
-            double px[3], py[3], pz[3];
+        // The cell's vertices, compacted, for the lane-parallel tests: a cone
+        // plane removes every vertex (with the margin of removes_all) when
+        // each one is strictly on its removed side.
+        VtxRec<R> *vtx = reinterpret_cast<VtxRec<R> *>(tcor + 16 * 9);
+        int nvc = 0;
 #pragma unroll
-            for (int v = 0; v < 3; ++v) {
-                px[v] = tcor[9 * i + 3 * v];
-                py[v] = tcor[9 * i + 3 * v + 1];
-                pz[v] = tcor[9 * i + 3 * v + 2];
-            }
-            bool sep = false;
-            for (int j = lane; j < c.nb_plane; j += 64) {
-                const Pl<RP> q = c.planes[j];
-                const double nx = double(q.x), ny = double(q.y), nz = double(q.z), nw = double(q.w);
-                const double m = face_margin * (1.0 + fabs(nw));
-                bool out = true;
-#pragma unroll
-                for (int v = 0; v < 3; ++v) {
-                    out = out && nx * px[v] + ny * py[v] + nz * pz[v] + nw < -m;
-                    out = out && nx * (px[v] - lx0) + ny * (py[v] - ly0) + nz * (pz[v] - lz0) <= 0.0;
-                }
-                sep = sep || out;
-            }
-            return ballot(sep) != 0;
-        };
+        for (int s = 0; s < S; ++s) {
+            if ((c.valid[s] >> lane) & 1)
+                vtx[nvc + mbcnt(c.valid[s])] = VtxRec<R>{c.X[s], c.Y[s], c.Z[s], c.W[s], c.MXYZ[s], c.MW[s]};
+            nvc += __popcll(c.valid[s]);
+        }
+        wsync();
         // A cone plane (unit normal n, offset d) with d < -rcell keeps no point
         // of the ball |y| <= rcell about the seed, which holds the cell: it
         // removes every vertex (kept side n.y + d >= 0), with a margin far
@@ -1232,108 +1225,164 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         for (int jb = sub; jb < ntri; jb += 16 * W) {
             // cone planes (A,B,C), (A,D,B), (A,C,D), (C,D,B), A = apex
             // (CCClipTetFromPointsInfinite): lane = 4 * triangle + plane
-            uint64_t sph;
-            {
-                const int i = lane >> 2, f = lane & 3;
-                const int k = jb + W * i;
-                Pl<RP> pl{RP(0), RP(0), RP(0), RP(0)};
-                if (k < ntri) {
-                    const int tid = tu[k];
-                    const double4 G1 = a.gpts[a.tris[3 * tid]];
-                    double4 G2 = a.gpts[a.tris[3 * tid + 1]], G3 = a.gpts[a.tris[3 * tid + 2]];
-                    // orientation test in the global frame, as the reference
-                    const double ux = G1.x - P0x, uy = G1.y - P0y, uz = G1.z - P0z;
-                    const double vx = G2.x - P0x, vy = G2.y - P0y, vz = G2.z - P0z;
-                    const double wx = G3.x - P0x, wy = G3.y - P0y, wz = G3.z - P0z;
-                    const double det =
-                        ux * (vy * wz - vz * wy) - vx * (uy * wz - uz * wy) + wx * (uy * vz - uz * vy);
-                    const bool go_out = det > 0.0;
-                    if (!go_out) {
-                        const double4 tmp = G2;
-                        G2 = G3;
-                        G3 = tmp;
-                    }
-                    if (f == 0) goo[i] = go_out ? 1 : 0;
-                    // the three points of plane f in the local frame
-                    const double Bx = G1.x - S0.x, By = G1.y - S0.y, Bz = G1.z - S0.z;
-                    const double Cx = G2.x - S0.x, Cy = G2.y - S0.y, Cz = G2.z - S0.z;
-                    const double Dx = G3.x - S0.x, Dy = G3.y - S0.y, Dz = G3.z - S0.z;
-                    if (f < 3) {  // the corners in the local frame, for cone_beyond_a_face
-                        const double cx = f == 0 ? Bx : (f == 1 ? Cx : Dx);
-                        const double cy = f == 0 ? By : (f == 1 ? Cy : Dy);
-                        const double cz = f == 0 ? Bz : (f == 1 ? Cz : Dz);
-                        tcor[9 * i + 3 * f] = cx;
-                        tcor[9 * i + 3 * f + 1] = cy;
-                        tcor[9 * i + 3 * f + 2] = cz;
-                    }
-                    double px, py, pz, qx, qy, qz, rx, ry, rz;
-                    if (f == 3) {
-                        px = Cx, py = Cy, pz = Cz;
+            const int i = lane >> 2, f = lane & 3;
+            const int k = jb + W * i;
+            Pl<RP> pl{RP(0), RP(0), RP(0), RP(0)};
+            if (k < ntri) {
+                const int tid = tu[k];
+                const double4 G1 = a.gpts[a.tris[3 * tid]];
+                double4 G2 = a.gpts[a.tris[3 * tid + 1]], G3 = a.gpts[a.tris[3 * tid + 2]];
+                // orientation test in the global frame, as the reference
+                const double ux = G1.x - P0x, uy = G1.y - P0y, uz = G1.z - P0z;
+                const double vx = G2.x - P0x, vy = G2.y - P0y, vz = G2.z - P0z;
+                const double wx = G3.x - P0x, wy = G3.y - P0y, wz = G3.z - P0z;
+                const double det = ux * (vy * wz - vz * wy) - vx * (uy * wz - uz * wy) + wx * (uy * vz - uz * vy);
+                const bool go_out = det > 0.0;
+                if (!go_out) {
+                    const double4 tmp = G2;
+                    G2 = G3;
+                    G3 = tmp;
+                }
+                if (f == 0) goo[i] = go_out ? 1 : 0;
+                // the three points of plane f in the local frame
+                const double Bx = G1.x - S0.x, By = G1.y - S0.y, Bz = G1.z - S0.z;
+                const double Cx = G2.x - S0.x, Cy = G2.y - S0.y, Cz = G2.z - S0.z;
+                const double Dx = G3.x - S0.x, Dy = G3.y - S0.y, Dz = G3.z - S0.z;
+                if (f < 3) {  // the corners in the local frame, for the face test
+                    const double cx = f == 0 ? Bx : (f == 1 ? Cx : Dx);
+                    const double cy = f == 0 ? By : (f == 1 ? Cy : Dy);
+                    const double cz = f == 0 ? Bz : (f == 1 ? Cz : Dz);
+                    tcor[9 * i + 3 * f] = cx;
+                    tcor[9 * i + 3 * f + 1] = cy;
+                    tcor[9 * i + 3 * f + 2] = cz;
+                }
+                double px, py, pz, qx, qy, qz, rx, ry, rz;
+                if (f == 3) {
+                    px = Cx, py = Cy, pz = Cz;
+                    qx = Dx, qy = Dy, qz = Dz;
+                    rx = Bx, ry = By, rz = Bz;
+                } else {
+                    px = lx0, py = ly0, pz = lz0;
+                    if (f == 0) {
+                        qx = Bx, qy = By, qz = Bz;
+                        rx = Cx, ry = Cy, rz = Cz;
+                    } else if (f == 1) {
                         qx = Dx, qy = Dy, qz = Dz;
                         rx = Bx, ry = By, rz = Bz;
                     } else {
-                        px = lx0, py = ly0, pz = lz0;
-                        if (f == 0) {
-                            qx = Bx, qy = By, qz = Bz;
-                            rx = Cx, ry = Cy, rz = Cz;
-                        } else if (f == 1) {
-                            qx = Dx, qy = Dy, qz = Dz;
-                            rx = Bx, ry = By, rz = Bz;
-                        } else {
-                            qx = Cx, qy = Cy, qz = Cz;
-                            rx = Dx, ry = Dy, rz = Dz;
-                        }
+                        qx = Cx, qy = Cy, qz = Cz;
+                        rx = Dx, ry = Dy, rz = Dz;
                     }
-                    const double e1x = qx - px, e1y = qy - py, e1z = qz - pz;
-                    const double e2x = rx - px, e2y = ry - py, e2z = rz - pz;
-                    double nx_ = e1y * e2z - e1z * e2y, ny_ = e1z * e2x - e1x * e2z, nz_ = e1x * e2y - e1y * e2x;
-                    double nd = -(nx_ * px + ny_ * py + nz_ * pz);
-                    const double nl = sqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
-                    if (nl > 0) {
-                        const double il = 1.0 / nl;
-                        nx_ *= il;
-                        ny_ *= il;
-                        nz_ *= il;
-                        nd *= il;
-                    }
-                    pl = Pl<RP>{RP(nx_), RP(ny_), RP(nz_), RP(nd)};
                 }
-                cpl[lane] = pl;
-                sph = ballot(k < ntri && double(pl.w) < -rsk);
+                const double e1x = qx - px, e1y = qy - py, e1z = qz - pz;
+                const double e2x = rx - px, e2y = ry - py, e2z = rz - pz;
+                double nx_ = e1y * e2z - e1z * e2y, ny_ = e1z * e2x - e1x * e2z, nz_ = e1x * e2y - e1y * e2x;
+                double nd = -(nx_ * px + ny_ * py + nz_ * pz);
+                const double nl = sqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
+                if (nl > 0) {
+                    const double il = 1.0 / nl;
+                    nx_ *= il;
+                    ny_ *= il;
+                    nz_ *= il;
+                    nd *= il;
+                }
+                pl = Pl<RP>{RP(nx_), RP(ny_), RP(nz_), RP(nd)};
             }
-            wsync();
-            for (int i = 0; i < 16; ++i) {
-                const int k = jb + W * i;
-                if (k >= ntri) break;
-                const int tid = uni(tu[k]);
-                const Pl<RP> q0 = cpl[4 * i], q1 = cpl[4 * i + 1], q2 = cpl[4 * i + 2], q3 = cpl[4 * i + 3];
-                double4 ck = make_double4(0.0, 0.0, 0.0, 0.0);
-                const bool skip = tid > fail_t ||
-                                  (may_skip && (((sph >> (4 * i)) & 0xfull) != 0 ||
-                                                c.removes_all(cvt<R>(q0)) || c.removes_all(cvt<R>(q1)) ||
-                                                c.removes_all(cvt<R>(q2)) || c.removes_all(cvt<R>(q3)) ||
-                                                cone_beyond_a_face(i)));
-                if (!skip) {
-                    int res = kNoClip;
-                    const Pl<RP> *qs[4] = {&q0, &q1, &q2, &q3};
-#pragma unroll
-                    for (int f = 0; f < 4; ++f)
-                        if (res < 0) res = c.nb_plane >= P ? int(RPD_PLANE_OVERFLOW) : c.clip(cvt<R>(*qs[f]), -1, *qs[f]);
-                    if (res >= 0 && res != RPD_EMPTY_CELL) {
-                        fail_t = tid;
-                        fail_st = res;
-                    } else if (res != RPD_EMPTY_CELL) {
-                        ck = integrate(goo[i] ? 1.0 : -1.0);
+            cpl[lane] = pl;
+            // Skip tests, one lane per cone plane: the ball about the seed,
+            // then every vertex (the test of removes_all).
+            bool gone = false;
+            if (may_skip && k < ntri) {
+                gone = double(pl.w) < -rsk;
+                if (!gone) {
+                    const Pl<R> e = cvt<R>(pl);
+                    const float exyz = fabsf(float(e.x)) + fabsf(float(e.y)) + fabsf(float(e.z)),
+                                ew = fabsf(float(e.w));
+                    gone = true;
+                    for (int j = 0; j < nvc && gone; ++j) {
+                        const VtxRec<R> v = vtx[j];
+                        const R sv = e.x * v.X + e.y * v.Y + e.z * v.Z + e.w * v.W;
+                        R m;
+                        if (FILTER)
+                            m = R(kFilterKappa * (exyz * v.MXYZ + ew * v.MW));
+                        else
+                            m = R(sizeof(R) == 8 ? 0x1.0p-33 : 0x1.0p-16) *
+                                (fabs(e.x * v.X) + fabs(e.y * v.Y) + fabs(e.z * v.Z) + fabs(e.w * v.W));
+                        gone = sv > m;
                     }
-                    c = bk;
                 }
+            }
+            uint64_t skipm = ballot(gone);
+            wsync();
+            // The cone region of a triangle (apex A, beyond the triangle) lies
+            // strictly outside one of the cell's planes when the triangle's
+            // three corners do (with a margin) and the rays from A through
+            // them do not turn back towards the plane.  The cell is inside
+            // every plane of its list (each one cut it when added), so the two
+            // are disjoint: the reference's clip ends empty_cell (same caps
+            // guard as above).  Lane 4i + f tests planes f, f + 4, ... against
+            // triangle i.
+            {
+                bool sep = false;
+                if (may_skip && k < ntri && ((skipm >> (4 * i)) & 0xfull) == 0) {
+                    double px[3], py[3], pz[3];
+#pragma unroll
+                    for (int v = 0; v < 3; ++v) {
+                        px[v] = tcor[9 * i + 3 * v];
+                        py[v] = tcor[9 * i + 3 * v + 1];
+                        pz[v] = tcor[9 * i + 3 * v + 2];
+                    }
+                    for (int j = f; j < c.nb_plane && !sep; j += 4) {
+                        const Pl<RP> q = c.planes[j];
+                        const double nx = double(q.x), ny = double(q.y), nz = double(q.z), nw = double(q.w);
+                        const double m = face_margin * (1.0 + fabs(nw));
+                        bool out = true;
+#pragma unroll
+                        for (int v = 0; v < 3; ++v) {
+                            out = out && nx * px[v] + ny * py[v] + nz * pz[v] + nw < -m;
+                            out = out && nx * (px[v] - lx0) + ny * (py[v] - ly0) + nz * (pz[v] - lz0) <= 0.0;
+                        }
+                        sep = out;
+                    }
+                }
+                skipm |= ballot(sep);
+            }
+            // triangles of the batch that are clipped: none of their four
+            // lanes skipped them
+            uint32_t todo = 0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                if (jb + W * t < ntri && ((skipm >> (4 * t)) & 0xfull) == 0) todo |= 1u << t;
+            if (W > 1 && lane < 16 && jb + W * lane < ntri) contrib[jb + W * lane] = make_double4(0.0, 0.0, 0.0, 0.0);
+            // in increasing k: the running sums add the contributions in order
+            // (a skipped triangle adds +0, which leaves them unchanged)
+            while (todo) {
+                const int t = __builtin_ctz(todo);
+                todo &= todo - 1;
+                const int kk = jb + W * t;
+                const int tid = uni(tu[kk]);
+                if (tid > fail_t) continue;
+                const Pl<RP> q0 = cpl[4 * t], q1 = cpl[4 * t + 1], q2 = cpl[4 * t + 2], q3 = cpl[4 * t + 3];
+                double4 ck = make_double4(0.0, 0.0, 0.0, 0.0);
+                int res = kNoClip;
+                const Pl<RP> *qs[4] = {&q0, &q1, &q2, &q3};
+#pragma unroll
+                for (int f2 = 0; f2 < 4; ++f2)
+                    if (res < 0) res = c.nb_plane >= P ? int(RPD_PLANE_OVERFLOW) : c.clip(cvt<R>(*qs[f2]), -1, *qs[f2]);
+                if (res >= 0 && res != RPD_EMPTY_CELL) {
+                    fail_t = tid;
+                    fail_st = res;
+                } else if (res != RPD_EMPTY_CELL) {
+                    ck = integrate(goo[t] ? 1.0 : -1.0);
+                }
+                c = bk;
                 if (W == 1) {
                     sx = uni(sx + ck.x);
                     sy = uni(sy + ck.y);
                     sz = uni(sz + ck.z);
                     sm = uni(sm + ck.w);
                 } else if (lane == 0) {
-                    contrib[k] = ck;
+                    contrib[kk] = ck;
                 }
             }
             wsync();  // the next batch overwrites cpl / goo

@@ -74,9 +74,10 @@ def test_power_torus(rpd, orc, ctx, prec, tol, sm, cull):
 @pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-6)])
 def test_coarse_seeds_fine_grid(rpd, orc, ctx, prec, tol):
     """1000 power-weighted seeds in a sphere voxelized at 96^3: boundary cells
-    meet up to a few thousand triangles, far beyond T = 100.  The re-queue pass
-    takes T from the largest triangle union the finish kernel counted, so the
-    solve ends in at most three passes; the oracle runs with T = 4000."""
+    meet up to a few thousand triangles, far beyond T = 100 (raised to 256 by
+    the solver).  The re-queue pass takes T from the largest triangle union
+    the finish kernel counted, so the solve ends in at most three passes; the
+    oracle runs with T = 4000."""
     n = 1000
     h = (1.0 / n) ** (1 / 3)
     x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=5, wmax=0.25 * h * h * 1e6)
@@ -121,7 +122,7 @@ def test_domain_cover_skip(rpd, ctx, prec):
     assert (np.abs(full[(got == 0).all(1), 3]) <= 1e-12 * 1e9 / n).all()
 
 
-@pytest.mark.parametrize("caps", [dict(K=64, P=40, V=60, T=8), dict(K=12, P=16, V=40, T=8)])
+@pytest.mark.parametrize("caps", [dict(K=64, P=34, V=60, T=100), dict(K=12, P=16, V=40, T=8)])
 def test_caps_do_not_change_results(rpd, ctx, caps):
     """FP64 form: a solve whose small caps send cells through re-queue passes
     (recomputed, or resumed after their list ran out, and restricted by
@@ -139,9 +140,8 @@ def test_caps_do_not_change_results(rpd, ctx, caps):
     st1 = ctx.solve(dom, precision=rpd.PREC_FP64, cull=0, max_extra_steps=12, **caps)
     got, status = ctx.moments(), ctx.status()
     print("caps", caps, st1["passes"], st1["pass_cells"], st1["status_hist"], flush=True)
-    assert st1["passes"] > 1
-    if caps["K"] == 64:  # only T overflows: short re-queue queues, several waves per cell
-        assert 0 < st1["pass_cells"][1] <= 32768
+    # a re-queue pass short enough to spread its cells over several waves
+    assert st1["passes"] > 1 and any(0 < k <= 32768 for k in st1["pass_cells"][1:])
     assert np.array_equal(status, ref_st)
     assert np.array_equal(got, ref)
     dom.close()
