@@ -51,15 +51,52 @@ __device__ __forceinline__ int cell_coord(double c, int dim) {
 
 template <typename R>
 __global__ void bin_cells_kernel(const double4 *__restrict__ seeds, int64_t n, int dim, int *__restrict__ cell,
-                                 int *__restrict__ counts, int *__restrict__ orig) {
+                                 int *__restrict__ counts, int *__restrict__ rank) {
     const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (i >= n) return;
     const double4 s = seeds[i];
     const R x = R(s.x), y = R(s.y), z = R(s.z);
     const int c = cell_coord(x, dim) + cell_coord(y, dim) * dim + cell_coord(z, dim) * dim * dim;
     cell[i] = c;
-    orig[i] = int(i);
-    atomicAdd(&counts[c], 1);
+    rank[i] = atomicAdd(&counts[c], 1);  // a place in the cell, in no set order
+}
+
+// Seed i to its place in its cell's range of perm (order within a cell set
+// by the atomics; sort_cells_kernel fixes it).
+__global__ void scatter_cells_kernel(const int *__restrict__ cell, const int *__restrict__ rank, int64_t n,
+                                     const int *__restrict__ counters, int *__restrict__ perm) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    perm[counters[cell[i]] + rank[i]] = int(i);
+}
+
+// Each cell's range of perm in increasing original index, so that perm is
+// the stable sort of the seeds by cell.  Ranges longer than kSortRun are
+// left to the caller (*big set): a clustered input goes to the radix sort.
+constexpr int kSortRun = 32;
+__global__ void sort_cells_kernel(const int *__restrict__ counters, int64_t nc, int *__restrict__ perm,
+                                  int *__restrict__ big) {
+    const int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (c >= nc) return;
+    const int lo = counters[c], hi = counters[c + 1];
+    if (hi - lo > kSortRun) {
+        *big = 1;
+        return;
+    }
+    for (int j = lo + 1; j < hi; ++j) {
+        const int v = perm[j];
+        int k = j - 1;
+        while (k >= lo && perm[k] > v) {
+            perm[k + 1] = perm[k];
+            --k;
+        }
+        perm[k + 1] = v;
+    }
+}
+
+__global__ void iota_kernel(int *__restrict__ out, int64_t n) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i < n) out[i] = int(i);
 }
 
 template <typename R>
@@ -410,11 +447,16 @@ int rpd_knn_dim(int64_t n) {
 
 // Bins n FP64 seeds in precision prec (0 float, 1 double).  Outputs (device):
 // perm[n], counters[dim^3+1], stored (float4 or double4 by prec) and stored64.
+// perm is the stable sort of the seeds by cell (knearests.cpp:129-191 sorts
+// by cell with a counting sort; here the order within a cell is fixed too):
+// a counting sort whose cells are then sorted by index, or, when a cell holds
+// more than kSortRun seeds, a radix sort of (cell, index).  Waits for the
+// stream.
 int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch, int *perm,
                   int *counters, void *stored, double4 *stored64) {
     const int64_t nc = int64_t(dim) * dim * dim;
     const size_t bytes_cell = size_t(n) * 4;
-    // scratch layout: cell_in, cell_out, orig_in, counts(nc+1), rocprim temp
+    // scratch layout: cell_in, cell_out, rank / orig, counts(nc+1), flag, rocprim temp
     size_t sort_bytes = 0, scan_bytes = 0;
     int *null_i = nullptr;
     unsigned end_bit = 1;
@@ -425,35 +467,49 @@ int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int
     if (rocprim::exclusive_scan(nullptr, scan_bytes, null_i, null_i, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
         hipSuccess)
         return RPD_E_DEVICE;
-    const size_t off_cell_out = bytes_cell, off_orig = 2 * bytes_cell, off_counts = 3 * bytes_cell;
-    const size_t off_tmp = (off_counts + size_t(nc + 1) * 4 + 255) & ~size_t(255);
+    const size_t off_cell_out = bytes_cell, off_rank = 2 * bytes_cell, off_counts = 3 * bytes_cell;
+    const size_t off_flag = (off_counts + size_t(nc + 1) * 4 + 255) & ~size_t(255);
+    const size_t off_tmp = off_flag + 256;
     const size_t need = off_tmp + std::max(sort_bytes, scan_bytes) + 256;
     if (scratch.ensure(need) != RPD_OK) return RPD_E_NOMEM;
     char *base = static_cast<char *>(scratch.ptr);
     int *cell_in = reinterpret_cast<int *>(base);
     int *cell_out = reinterpret_cast<int *>(base + off_cell_out);
-    int *orig = reinterpret_cast<int *>(base + off_orig);
+    int *rank = reinterpret_cast<int *>(base + off_rank);
     int *counts = reinterpret_cast<int *>(base + off_counts);
+    int *big = reinterpret_cast<int *>(base + off_flag);
     void *tmp = base + off_tmp;
-    if (hipMemsetAsync(counts, 0, size_t(nc + 1) * 4, st) != hipSuccess) return RPD_E_DEVICE;
+    if (hipMemsetAsync(counts, 0, size_t(nc + 1) * 4, st) != hipSuccess ||
+        hipMemsetAsync(big, 0, sizeof(int), st) != hipSuccess)
+        return RPD_E_DEVICE;
     const int tb = 256;
     const unsigned nb = unsigned((n + tb - 1) / tb);
     if (prec == 0)
-        bin_cells_kernel<float><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, orig);
+        bin_cells_kernel<float><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, rank);
     else
-        bin_cells_kernel<double><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, orig);
-    size_t sb = sort_bytes;
-    if (rocprim::radix_sort_pairs(tmp, sb, cell_in, cell_out, orig, perm, n, 0u, end_bit, st) != hipSuccess)
-        return RPD_E_DEVICE;
+        bin_cells_kernel<double><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, rank);
     size_t scb = scan_bytes;
     if (rocprim::exclusive_scan(tmp, scb, counts, counters, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
         hipSuccess)
         return RPD_E_DEVICE;
+    scatter_cells_kernel<<<nb, tb, 0, st>>>(cell_in, rank, n, counters, perm);
+    sort_cells_kernel<<<unsigned((nc + tb - 1) / tb), tb, 0, st>>>(counters, nc, perm, big);
+    int big_h = 0;
+    if (hipMemcpyAsync(&big_h, big, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return RPD_E_DEVICE;
+    if (big_h) {
+        iota_kernel<<<nb, tb, 0, st>>>(rank, n);
+        size_t sb = sort_bytes;
+        if (rocprim::radix_sort_pairs(tmp, sb, cell_in, cell_out, rank, perm, n, 0u, end_bit, st) != hipSuccess)
+            return RPD_E_DEVICE;
+    }
     if (prec == 0)
         gather_seeds_kernel<float><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<float4 *>(stored), stored64);
     else
         gather_seeds_kernel<double><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<double4 *>(stored), stored64);
-    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+    if (hipGetLastError() != hipSuccess) return RPD_E_DEVICE;
+    return hipStreamSynchronize(st) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }
 
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,

@@ -12,16 +12,21 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("n,K,kind", [(500, 20, "uniform"), (5000, 50, "uniform"), (20000, 64, "uniform"),
-                                      (20000, 70, "uniform"), (4096, 100, "lattice"), (8000, 26, "lattice_exact")])
+                                      (20000, 70, "uniform"), (4096, 100, "lattice"), (8000, 26, "lattice_exact"),
+                                      (6000, 40, "clustered")])
 def test_knn_bit_exact(rpd, orc, ctx, prec, n, K, kind):
     if kind == "uniform":
         x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=n + K)
     elif kind == "lattice":
         m = int(round(n ** (1 / 3)))
         x = rpd.generate_seeds(rpd.SEEDS_LATTICE, m ** 3, seed=3, p0=m, p1=1e-6)
-    else:  # exact lattice: massive distance ties, order decided by original index
+    elif kind == "lattice_exact":  # massive distance ties, order decided by original index
         m = int(round(n ** (1 / 3)))
         x = rpd.generate_seeds(rpd.SEEDS_LATTICE, m ** 3, seed=3, p0=m, p1=0.0)
+    else:  # two thirds of the seeds in a small ball: grid cells of far more than 32 seeds
+        x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=7)
+        k = 2 * n // 3
+        x[:k, :3] = 500.0 + (x[:k, :3] - 500.0) * 0.02
     ctx.set_seeds(x)
     got = ctx.knn(K, prec)
     perm = ctx.perm()
