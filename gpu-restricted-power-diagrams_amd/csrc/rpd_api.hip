@@ -68,6 +68,8 @@ constexpr int kPinCounters = 16;
 // smallest triangle cap a solve uses
 constexpr int kTriangleFloor = 256;
 constexpr int kPinWords = kPinCounters + 6 * kFailCounters;
+// blocks of the weight-minimum kernel
+constexpr int kWminBlocks = 256;
 
 struct rpd_ctx {
     int device = 0;
@@ -95,6 +97,11 @@ struct rpd_ctx {
     hipEvent_t ev[4][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
     hipEvent_t dep[2] = {};    // per chain: orders work between its streams
     int *pin = nullptr;        // kPinWords pinned host words
+    double *pin_red = nullptr;  // 2 x kWminBlocks pinned doubles: per-block weight minima
+    // smallest weight of the non-border seeds and of all seeds, for the
+    // current seeds (valid once wmin_ok)
+    bool wmin_ok = false;
+    double wmin = 0, wall = 0;
 };
 
 namespace {
@@ -261,6 +268,12 @@ int rpd_create(int device, rpd_ctx **out) {
         rpd_destroy(c);
         return RPD_E_DEVICE;
     }
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->pin_red), 2 * kWminBlocks * sizeof(double),
+                      hipHostMallocDefault) != hipSuccess) {
+        c->pin_red = nullptr;
+        rpd_destroy(c);
+        return RPD_E_DEVICE;
+    }
     *out = c;
     return RPD_OK;
 }
@@ -278,6 +291,7 @@ void rpd_destroy(rpd_ctx *c) {
     for (auto &s : c->streams)
         if (s) hipStreamDestroy(s);
     if (c->pin) hipHostFree(c->pin);
+    if (c->pin_red) hipHostFree(c->pin_red);
     delete c;
 }
 
@@ -424,6 +438,7 @@ static int set_seeds_common(rpd_ctx *c, const double *src, int64_t n, hipMemcpyK
     HIPCHK(c, hipMemcpyAsync(c->seeds.ptr, src, size_t(n) * 32, kind, c->stream));
     c->n = n;
     c->bin_prec = -1;
+    c->wmin_ok = false;
     c->solved = false;
     return RPD_OK;
 }
@@ -477,10 +492,26 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     auto t_start = std::chrono::steady_clock::now();
 
     HIPCHK(c, hipEventRecord(c->ev[0][0], c->stream));
+    // the weight minima of new seeds, read after the binning's wait
+    const bool new_wmin = prm->security_mode == 0 && !c->wmin_ok;
+    if (new_wmin) {
+        RPDCHK(c, c->wred.ensure(2 * kWminBlocks * sizeof(double)), "alloc wred");
+        wmin_kernel<<<kWminBlocks, 256, 0, c->stream>>>(c->seeds.as<double4>(), n, c->wred.as<double>());
+        HIPCHK(c, hipMemcpyAsync(c->pin_red, c->wred.ptr, 2 * kWminBlocks * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
     RPDCHK(c, ensure_binning(c, kprec), "binning");
     HIPCHK(c, hipEventRecord(c->ev[0][1], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     st.ms_bin = elapsed_ms(c->ev[0][0], c->ev[0][1]);
+    if (new_wmin) {
+        c->wmin = c->wall = INFINITY;
+        for (int b = 0; b < kWminBlocks; ++b) {
+            c->wmin = std::min(c->wmin, c->pin_red[b]);
+            c->wall = std::min(c->wall, c->pin_red[kWminBlocks + b]);
+        }
+        c->wmin_ok = true;
+    }
 
     RPDCHK(c, c->out.ensure(size_t(n) * 32), "alloc out");
     RPDCHK(c, c->status.ensure(size_t(n) * 4), "alloc status");
@@ -499,20 +530,8 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     double wmin = 0;
     bool border_seeds = true;
     if (prm->security_mode == 0) {
-        const int nbk = 256;
-        RPDCHK(c, c->wred.ensure(2 * nbk * sizeof(double)), "alloc wred");
-        wmin_kernel<<<nbk, 256, 0, c->stream>>>(c->seeds.as<double4>(), n, c->wred.as<double>());
-        double red[2 * nbk];
-        HIPCHK(c, hipMemcpyAsync(red, c->wred.ptr, sizeof(red), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        wmin = INFINITY;
-        double wall = INFINITY;
-        for (int b = 0; b < nbk; ++b) {
-            wmin = std::min(wmin, red[b]);
-            wall = std::min(wall, red[nbk + b]);
-        }
-        if (!(wmin < INFINITY)) wmin = 0;
-        border_seeds = wall <= RPD_NO_CELL_LIMITS;
+        wmin = c->wmin < INFINITY ? c->wmin : 0.0;
+        border_seeds = c->wall <= RPD_NO_CELL_LIMITS;
     }
     // cells provably outside the domain skip pass 1 (exact stop, no walls)
     const int *ids1 = nullptr;
@@ -617,10 +636,10 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         return rpd_launch_cells(s, ps.form, L, c->pin + sl, false);
     };
     // the finish kernel of a pass whose bisector kernel (and queue length) is done
-    auto pass_back = [&](Pass &ps, const ClipLaunch &L) -> int {
+    auto pass_back = [&](Pass &ps, const ClipLaunch &L, int beside) -> int {
         hipStream_t s = c->streams[ps.slot];
         ps.nq = c->pin[ps.slot];
-        const int r = rpd_launch_finish(s, ps.form, L, ps.nq);
+        const int r = rpd_launch_finish(s, ps.form, L, ps.nq, beside);
         if (r != RPD_OK) return r;
         return hipEventRecord(c->ev[ps.slot][3], s) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
     };
@@ -770,6 +789,8 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         ch.phase = 1;
         return RPD_OK;
     };
+    Pass p1;                 // pass 1 (stream 0)
+    bool p1_running = true;  // its finish kernel is not known to be done
     // advances the chain as far as its streams allow, without waiting
     auto chain_poll = [&](Chain &ch, bool &progress) -> int {
         hipStream_t s = c->streams[ch.sl], s2 = c->streams[ch.sl2];
@@ -777,8 +798,10 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
             const int d1 = done(s), d2 = ch.n_scr > 0 ? done(s2) : 1;
             if (d1 < 0 || d2 < 0) return RPD_E_DEVICE;
             if (!d1 || !d2) return RPD_OK;
-            if (ch.n_res > 0 && pass_back(ch.pr, ch.Lr) != RPD_OK) return RPD_E_DEVICE;
-            if (ch.n_scr > 0 && pass_back(ch.pz, ch.Lz) != RPD_OK) return RPD_E_DEVICE;
+            // pass 1's finish kernel may still run beside these ones
+            const int beside = p1_running ? p1.nq : 0;
+            if (ch.n_res > 0 && pass_back(ch.pr, ch.Lr, beside) != RPD_OK) return RPD_E_DEVICE;
+            if (ch.n_scr > 0 && pass_back(ch.pz, ch.Lz, beside) != RPD_OK) return RPD_E_DEVICE;
             // the counters of the pass, once both finish kernels are done
             if (ch.n_scr > 0 && (hipEventRecord(c->dep[ch.dep], s2) != hipSuccess ||
                                  hipStreamWaitEvent(s, c->dep[ch.dep], 0) != hipSuccess))
@@ -818,7 +841,6 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     //   * chain B, the finish kernel's failures, once it is done (streams 0,
     //     3), with the caps of pass 1 grown only for them.
     if (!clear_list(0, c->stream) || !clear_list(1, c->stream)) return set_err(c, RPD_E_DEVICE, "clear lists");
-    Pass p1;
     p1.slot = 0;
     p1.form = form0;
     p1.n_ids = n1;
@@ -921,6 +943,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
             if (d < 0) return set_err(c, RPD_E_DEVICE, "pass 1 finish");
             if (d) {
                 account(p1, 0);
+                p1_running = false;
                 cb.phase = 0;
                 progress = true;
                 RPDCHK(c, chain_front(cb), "re-queue chain (finish failures)");

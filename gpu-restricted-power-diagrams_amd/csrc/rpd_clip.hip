@@ -1570,7 +1570,7 @@ int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out,
     return hipStreamSynchronize(st) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }
 
-int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq) {
+int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq, int beside) {
     if (nq <= 0) return RPD_OK;
     ClipArgs a;
     const int r = make_args(form, L, a);
@@ -1583,7 +1583,10 @@ int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq) {
         const size_t wstride = finish_wave_lds(L.P, S, a.T, pl, true);
         const size_t kLdsMax = lds_limit();
         if (finish_cell_stride(a.T, 1) + wstride > kLdsMax) return RPD_E_LIMIT;
-        int W = nq <= 2048 ? 8 : (nq <= 8192 ? 4 : (nq <= 32768 ? 2 : 1));
+        // waves per cell from the cells that share the device: this queue's
+        // and those of finish kernels running beside it
+        const int64_t load = int64_t(nq) + std::max(beside, 0);
+        int W = load <= 2048 ? 8 : (load <= 8192 ? 4 : (load <= 32768 ? 2 : 1));
         while (W > 1 && finish_cell_stride(a.T, W) + W * wstride > kLdsMax) W >>= 1;
         const size_t cstride = finish_cell_stride(a.T, W);
         a.lds_stride = int(wstride);

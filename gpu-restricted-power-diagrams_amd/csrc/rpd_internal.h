@@ -138,5 +138,7 @@ int rpd_clip_max_T(int form, int P, int V);
 // queued (and the failure counters when L.fl_cnt_host is set); waits for
 // them unless wait is false (the caller synchronizes st before reading).
 int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq, bool wait = true);
-// The finish kernel over the nq records the bisector kernel queued (async).
-int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq);
+// The finish kernel over the nq records the bisector kernel queued (async);
+// beside: cells of other finish kernels expected to run at the same time
+// (they set how many waves each cell gets).
+int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq, int beside = 0);
