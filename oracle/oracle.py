@@ -41,7 +41,7 @@ class OrcGrid(ctypes.Structure):
 class OrcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "K", "P", "V", "T", "precision", "vhs0", "vhs1", "apex_mode",
-        "fix_exact_requeue", "max_extra_steps", "threads", "single_pass", "security_mode", "cull")]
+        "fix_exact_requeue", "max_extra_steps", "threads", "single_pass", "security_mode", "cull", "H", "HC")]
 
 
 class OrcStats(ctypes.Structure):
@@ -85,6 +85,9 @@ def lib():
         L.orc_knn_dim.argtypes = [ctypes.c_int]
         L.orc_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcParams),
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(OrcStats)]
+        L.orc_solve_hessian.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcGrid),
+                                        ctypes.POINTER(OrcParams), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.POINTER(OrcStats), ctypes.c_void_p, ctypes.c_void_p]
         L.orc_solve_ids.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcParams), ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(OrcStats)]
@@ -150,11 +153,11 @@ def knn(xyzw, K, precision=1, method=0):
 
 
 def params(K=25, P=35, V=40, T=100, precision=1, vhs0=2, vhs1=2, apex_mode=1, fix_exact_requeue=1,
-           max_extra_steps=5, threads=0, single_pass=0, security_mode=0, cull=0):
+           max_extra_steps=5, threads=0, single_pass=0, security_mode=0, cull=0, H=0, HC=1):
     p = OrcParams()
     for k, v in dict(K=K, P=P, V=V, T=T, precision=precision, vhs0=vhs0, vhs1=vhs1, apex_mode=apex_mode,
                      fix_exact_requeue=fix_exact_requeue, max_extra_steps=max_extra_steps, threads=threads,
-                     single_pass=single_pass, security_mode=security_mode, cull=cull).items():
+                     single_pass=single_pass, security_mode=security_mode, cull=cull, H=H, HC=HC).items():
         setattr(p, k, int(v))
     return p
 
@@ -174,6 +177,28 @@ def solve(xyzw, grid=None, **kw):
     if r < 0:
         raise RuntimeError("orc_solve failed %d" % r)
     return out, status, kused, st.as_dict()
+
+
+def solve_hessian(xyzw, grid=None, H=32, HC=1, **kw):
+    """solve() plus the Hessian (ConvexCell.cl:621-674, 1133-1151): per original
+    seed up to H entries, hess[n, H, HC] (HC = 1: face area; 4: area-weighted
+    face centroid xyz, then area) and ids[n, H] (neighbour original index,
+    -1 after the last).  Returns (mg, status, kused, hess, ids, stats)."""
+    xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
+    n = xyzw.shape[0]
+    out = np.zeros((n, 4), np.float64)
+    status = np.zeros(n, np.int32)
+    kused = np.zeros(n, np.int32)
+    hess = np.zeros((n, H, HC), np.float64)
+    ids = np.full((n, H), -1, np.int32)
+    st = OrcStats()
+    p = params(H=H, HC=HC, **kw)
+    g = grid.orc() if grid is not None else None
+    r = lib().orc_solve_hessian(_ptr(xyzw), n, ctypes.byref(g) if g is not None else None, ctypes.byref(p),
+                                _ptr(out), _ptr(status), _ptr(kused), ctypes.byref(st), _ptr(hess), _ptr(ids))
+    if r < 0:
+        raise RuntimeError("orc_solve_hessian failed %d" % r)
+    return out, status, kused, hess, ids, st.as_dict()
 
 
 def solve_ids(xyzw, ids, grid=None, **kw):

@@ -198,6 +198,7 @@ struct Cell {
     double wmin = 0;        // minimum weight over non-border seeds
     int nb_vertex = 0, nb_plane = 0;
     R4<R> planes[256];
+    int pnb[256];  // neighbour (stored id) of each plane, -1: cube face or cone plane
     V3 verts[256];
     uint8_t next[256];
     Counters *ctr = nullptr;
@@ -214,6 +215,7 @@ struct Cell {
         planes[3] = mk4<R>(0, -1, 0, vmax);
         planes[4] = mk4<R>(0, 0, 1, -vmin);
         planes[5] = mk4<R>(0, 0, -1, vmax);
+        for (int i = 0; i < 6; ++i) pnb[i] = -1;
         nb_plane = 6;
         static const uint8_t cube[8][3] = {{2, 5, 0}, {5, 3, 0}, {1, 5, 2}, {5, 1, 3},
                                            {4, 2, 0}, {4, 0, 3}, {2, 4, 1}, {4, 3, 1}};
@@ -235,6 +237,7 @@ struct Cell {
         if (double(B.w) <= kNoCellLimits) dt = R(2) * dot3(B, dir);
         const R nrm = std::sqrt(dot3(dir, dir));
         planes[nb_plane] = mk4<R>(dir.x / nrm, dir.y / nrm, dir.z / nrm, -dt / R(2) / nrm);
+        pnb[nb_plane] = vid;  // CCPlaneNeighbour, ConvexCell.cl:329-331
         return nb_plane++;
     }
 
@@ -434,6 +437,7 @@ struct Cell {
             }
             ctr->n_planes++;
             ctr->n_planes_restrict++;
+            pnb[nb_plane] = -1;  // ConvexCell.cl:544-546
             planes[nb_plane++] = plane_from_points(*tri[f][0], *tri[f][1], *tri[f][2]);
             clip_by_plane(nb_plane - 1);
             if (status != kSuccess) return;
@@ -441,7 +445,11 @@ struct Cell {
     }
 
     // CCExportBaryVolume :577-619.  Accumulates factor/6 * sum into *out.
-    void export_bary_volume(R4<R> *out, R factor, R vol_eps2) {
+    // With surf (CCExportBaryVolumeAndArea :621-674): adds, per plane p of a
+    // vertex, the signed area of each triangle (A, A + APlaneI, A +
+    // ALineII1) times factor to surf[HC p + HC - 1], and for HC = 4 its
+    // area-weighted centroid to surf[4p .. 4p + 2].
+    void export_bary_volume(R4<R> *out, R factor, R vol_eps2, R *surf = nullptr, int HC = 1) {
         R4<R> s = mk4<R>(0, 0, 0, 0);
         const R4<R> C = seed;
         ctr->n_integrated_vertex += nb_vertex;
@@ -471,6 +479,19 @@ struct Cell {
                     s.x += w * (A.x + R(0.25) * sm.x);
                     s.y += w * (A.y + R(0.25) * sm.y);
                     s.z += w * (A.z + R(0.25) * sm.z);
+                    if (surf) {
+                        const int pid = verts[ve][i];
+                        const R area = R(2 * st - 1) * factor * det3r4(onPlane[i], onLine[j], pl[i]) / R(2);
+                        if (HC == 1) {
+                            surf[pid] += area;
+                        } else {
+                            const R4<R> t = add4(onPlane[i], onLine[j]);
+                            surf[4 * pid + 0] += area * (A.x + t.x / R(3));
+                            surf[4 * pid + 1] += area * (A.y + t.y / R(3));
+                            surf[4 * pid + 2] += area * (A.z + t.z / R(3));
+                            surf[4 * pid + 3] += area;
+                        }
+                    }
                 }
             }
         }
@@ -514,8 +535,18 @@ inline bool merge_union(const std::vector<int> &a, const int *b, int nb, int cap
 template <class R>
 CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, int P, int V, int T, int vhs,
                            const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
-                           R4<R> *out, Counters *ctr, bool all_listed, double cull_r2 = 0) {
+                           R4<R> *out, Counters *ctr, bool all_listed, double cull_r2 = 0, int H = 0, int HC = 1,
+                           R *hess = nullptr, int *hids = nullptr) {
     CellResult<R> res;
+    // Hessian (CHESSIAN): per plane the accumulated area (HC = 1) or
+    // area-weighted face centroid and area (HC = 4) over the pieces of the
+    // restricted cell; no entries unless the cell succeeds
+    std::vector<R> surf;
+    if (H > 0) {
+        hids[0] = -1;
+        surf.assign(size_t(HC) * 256, R(0));
+    }
+    R *sp = H > 0 ? surf.data() : nullptr;
     const R vol_eps2 = sizeof(R) == sizeof(float) ? R(1.e-6) : R(1.e-12);  // voronoi.cpp:919-922
     if (double(pts[seed].w) <= kNoCellLimits) {
         *out = pts[seed];
@@ -691,7 +722,7 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
         }
         add_volume = in_domain == 2;
     }
-    if (add_volume) cc.export_bary_volume(out, R(1), vol_eps2);
+    if (add_volume) cc.export_bary_volume(out, R(1), vol_eps2, sp, HC);
     ctr->restricting = true;
     struct PhaseEnd {
         Counters *c;
@@ -725,7 +756,7 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
                 res.status = cc.status;
                 return res;
             }
-            cc.export_bary_volume(out, go_out ? R(1) : R(-1), vol_eps2);
+            cc.export_bary_volume(out, go_out ? R(1) : R(-1), vol_eps2, sp, HC);
             cc.nb_vertex = bk_nv;
             cc.nb_plane = bk_np;
             std::copy(bk.begin(), bk.end(), cc.verts);
@@ -733,6 +764,23 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
         }
     }
     res.status = kSuccess;
+    if (H > 0) {
+        // planes towards a seed whose face keeps more than 0.2 of area, in
+        // plane order; more than H of them: hessian_overflow
+        // (ConvexCell.cl:1133-1151)
+        int cnt = 0;
+        for (int p = 0; p < cc.nb_plane; ++p) {
+            if (cc.pnb[p] < 0 || surf[size_t(HC) * p + HC - 1] <= R(0.2)) continue;
+            if (cnt >= H) {
+                res.status = kHessianOverflow;
+                return res;
+            }
+            for (int c = 0; c < HC; ++c) hess[size_t(HC) * cnt + c] = surf[size_t(HC) * p + c];
+            hids[cnt] = cc.pnb[p];
+            ++cnt;
+        }
+        if (cnt < H) hids[cnt] = -1;
+    }
     return res;
 }
 
@@ -998,7 +1046,7 @@ void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, 
                const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
                std::vector<R4<R>> &out,
                std::vector<int> &status, std::vector<int> &kused, Counters &ctr, double &t_knn, double &t_clip,
-               const std::vector<double> &cull_r2) {
+               const std::vector<double> &cull_r2, int H = 0, int HC = 1, R *hess = nullptr, int *hids = nullptr) {
     const long long nids = (long long)ids.size();
     if (nids == 0) return;
     std::vector<int> nb(size_t(nids) * size_t(K));
@@ -1015,7 +1063,9 @@ void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, 
             CellResult<R> r = compute_cell<R>(kg.stored.data(), s, nb.data() + t * K, K, P, V, T, vhs, grid,
                                               apex_mode, out_domain, security_mode, wmin, &out[size_t(s)], &local,
                                               K >= int(kg.stored.size()) - 1,
-                                              cull_r2.empty() ? 0.0 : cull_r2[size_t(s)]);
+                                              cull_r2.empty() ? 0.0 : cull_r2[size_t(s)], H, HC,
+                                              H > 0 ? hess + size_t(s) * H * HC : nullptr,
+                                              H > 0 ? hids + size_t(s) * H : nullptr);
             status[size_t(s)] = r.status;
             kused[size_t(s)] = r.kused;
         }
@@ -1030,7 +1080,8 @@ void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, 
 
 template <class R>
 int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *prm, double *out_mg,
-               int *out_status, int *out_kused, orc_stats *st, const int *ids_orig = nullptr, int n_ids = 0) {
+               int *out_status, int *out_kused, orc_stats *st, const int *ids_orig = nullptr, int n_ids = 0,
+               double *out_hess = nullptr, int *out_hids = nullptr) {
     auto tstart = std::chrono::steady_clock::now();
     set_threads(prm->threads);
     const std::vector<R4<R>> pts = convert_pts<R>(xyzw, n);
@@ -1045,6 +1096,10 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
     if (!(wmin < std::numeric_limits<double>::infinity())) wmin = 0;
     std::vector<R4<R>> out(size_t(n), mk4<R>(0, 0, 0, 0));
     std::vector<int> status(size_t(n), kSuccess), kused(size_t(n), 0);
+    // Hessian per stored seed (H entries of HC values, neighbour stored ids)
+    const int H = out_hess && out_hids ? std::max(prm->H, 0) : 0, HC = prm->HC == 4 ? 4 : 1;
+    std::vector<R> hess(size_t(n) * H * HC, R(0));
+    std::vector<int> hids(size_t(n) * H, -1);
     Counters ctr;
     double t_knn = 0, t_clip = 0;
     int K = prm->K, P = prm->P, V = prm->V, T = prm->T;
@@ -1100,12 +1155,12 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
 
     if (prm->single_pass) {
         clip_pass(kg, todo, K, P, V, T, vhs, grid, prm->apex_mode, grid && gv.has_border_vertices, sm, wmin, out,
-                  status, kused, ctr, t_knn, t_clip, cull_r2);
+                  status, kused, ctr, t_knn, t_clip, cull_r2, H, HC, hess.data(), hids.data());
         passes = 1;
     } else {
         // first pass: T = 0 defers every boundary cell (voronoi.cpp:1294-1297)
         clip_pass(kg, todo, K, P, V, 0, vhs, grid, prm->apex_mode, false, sm, wmin, out, status, kused, ctr, t_knn,
-                  t_clip, cull_r2);
+                  t_clip, cull_r2, H, HC, hess.data(), hids.data());
         passes = 1;
         auto collect = [&](const std::vector<int> &from, auto cond, std::vector<int> &ids0, std::vector<int> &ids1) {
             ids0.clear();
@@ -1120,7 +1175,7 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
         collect(todo, [](int s) { return s == kTriangleOverflow; }, ids0, ids1);
         if (!ids1.empty()) {  // tr_overflow pass (voronoi.cpp:1304-1310)
             clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, false, sm, wmin, out, status, kused, ctr,
-                      t_knn, t_clip, cull_r2);
+                      t_knn, t_clip, cull_r2, H, HC, hess.data(), hids.data());
             passes++;
         }
         if (!ids0.empty()) {
@@ -1145,7 +1200,7 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
                 if (P >= 256) P = 255;  // uchar plane ids (voronoi.cpp:867-870)
                 K = std::min(K, n - 1 > 0 ? n - 1 : 1);
                 clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, out_domain, sm, wmin, out, status, kused,
-                          ctr, t_knn, t_clip, cull_r2);
+                          ctr, t_knn, t_clip, cull_r2, H, HC, hess.data(), hids.data());
                 passes++;
                 cur = ids0;
                 collect(cur, [](int s) { return s != kEmptyCell; }, ids0, ids1);
@@ -1162,6 +1217,12 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
         out_mg[4 * o + 3] = double(v.w);
         if (out_status) out_status[o] = status[size_t(s)];
         if (out_kused) out_kused[o] = kused[size_t(s)];
+        for (int h = 0; h < H; ++h) {
+            const int nb = hids[size_t(s) * H + h];
+            out_hids[o * H + h] = nb >= 0 ? kg.perm[size_t(nb)] : -1;
+            for (int c = 0; c < HC; ++c) out_hess[(o * H + h) * HC + c] = double(hess[(size_t(s) * H + h) * HC + c]);
+            if (nb < 0) break;
+        }
     }
     if (st) {
         std::memset(st, 0, sizeof(*st));
@@ -1214,6 +1275,18 @@ int orc_solve(const double *xyzw, int n, const orc_grid *grid, const orc_params 
     p.K = std::min(p.K, n - 1);
     return p.precision == 0 ? solve_impl<float>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st)
                             : solve_impl<double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st);
+}
+
+int orc_solve_hessian(const double *xyzw, int n, const orc_grid *grid, const orc_params *prm, double *out_mg,
+                      int *out_status, int *out_kused, orc_stats *st, double *out_hess, int *out_hids) {
+    if (n <= 1 || !xyzw || !prm || !out_mg) return -1;
+    if (prm->P >= 256 || prm->K <= 0 || prm->V <= 0 || prm->H < 0 || (prm->HC != 1 && prm->HC != 4)) return -2;
+    orc_params p = *prm;
+    p.K = std::min(p.K, n - 1);
+    return p.precision == 0
+               ? solve_impl<float>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, nullptr, 0, out_hess, out_hids)
+               : solve_impl<double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, nullptr, 0, out_hess,
+                                    out_hids);
 }
 
 int orc_solve_ids(const double *xyzw, int n, const int *ids_orig, int n_ids, const orc_grid *grid,

@@ -52,6 +52,9 @@ typedef struct {
                                     cover test before the kNN, the circumradius
                                     test in the neighbour loop); 0: none, as
                                     the reference                              */
+    int H, HC;                   /* Hessian (CHESSIAN): up to H neighbours per
+                                    seed, HC = 1 face area, 4 face centroid x
+                                    area and area; H = 0: none                 */
 } orc_params;
 
 typedef struct {
@@ -85,6 +88,13 @@ int orc_knn_dim(int n);
 int orc_solve(const double *xyzw, int n, const orc_grid *grid,
               const orc_params *prm, double *out_mg, int *out_status,
               int *out_kused, orc_stats *st);
+/* orc_solve plus the Hessian (prm->H > 0): out_hess[n*H*HC] and
+   out_hids[n*H] per original seed, neighbours as original indices in plane
+   order, -1 after the last. */
+int orc_solve_hessian(const double *xyzw, int n, const orc_grid *grid,
+                      const orc_params *prm, double *out_mg, int *out_status,
+                      int *out_kused, orc_stats *st, double *out_hess,
+                      int *out_hids);
 
 /* Same as orc_solve for the cells of a subset of seeds only (ids_orig:
    original indices); every seed still acts as a neighbour.  out_mg, out_status
