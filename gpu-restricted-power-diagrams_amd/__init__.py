@@ -63,8 +63,8 @@ class GridDesc(ctypes.Structure):
 
 class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in
-                ("K", "P", "V", "T", "precision", "max_extra_steps", "apex_mode", "security_mode", "cull")] + \
-               [("reserved", ctypes.c_int32 * 3)]
+                ("K", "P", "V", "T", "precision", "max_extra_steps", "apex_mode", "security_mode", "cull", "H",
+                 "HC")] + [("reserved", ctypes.c_int32 * 1)]
 
 
 class Stats(ctypes.Structure):
@@ -152,6 +152,9 @@ def lib():
         "rpd_read_status": (ctypes.c_int, [VP, VP]),
         "rpd_read_perm": (ctypes.c_int, [VP, VP]),
         "rpd_moments_device": (ctypes.c_int, [VP, ctypes.POINTER(VP)]),
+        "rpd_read_hessian": (ctypes.c_int, [VP, VP, VP]),
+        "rpd_hessian_device": (ctypes.c_int, [VP, ctypes.POINTER(VP), ctypes.POINTER(VP), ctypes.POINTER(I32),
+                                              ctypes.POINTER(I32)]),
         "rpd_knn": (ctypes.c_int, [VP, I32, I32, VP]),
         "rpd_compact": (ctypes.c_int, [VP, ctypes.c_double, ctypes.POINTER(I64), VP, VP]),
         "rpd_server_create": (ctypes.c_int, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
@@ -315,13 +318,28 @@ class Context:
         self.n = int(n)
 
     def solve(self, domain=None, K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, apex_mode=1,
-              security_mode=0, cull=1):
+              security_mode=0, cull=1, H=0, HC=1):
+        """rpd_solve; H > 0 also computes the Hessian (up to H neighbours per
+        seed, HC = 1 face area or 4 face centroid times area and area)."""
         p = Params(int(K), int(P), int(V), int(T), int(precision), int(max_extra_steps), int(apex_mode),
-                   int(security_mode), int(cull))
+                   int(security_mode), int(cull), int(H), int(HC))
         st = Stats()
         self.check(lib().rpd_solve(self._h, domain._h if domain is not None else None, ctypes.byref(p),
                                    ctypes.byref(st)), "rpd_solve")
+        self._hess_shape = (int(H), int(HC)) if H > 0 else None
         return st.as_dict()
+
+    def hessian(self):
+        """(vals[n, H, HC], ids[n, H]) of the last solve with H > 0: per
+        original seed the neighbours (original indices, -1 after the last)
+        and their face values, in the cell's plane order."""
+        H, HC = getattr(self, "_hess_shape", None) or (0, 1)
+        if H <= 0:
+            raise RpdError("hessian: the last solve had H = 0")
+        vals = np.zeros((self.n, H, HC), np.float64)
+        ids = np.zeros((self.n, H), np.int32)
+        self.check(lib().rpd_read_hessian(self._h, _p(vals), _p(ids)), "rpd_read_hessian")
+        return vals, ids
 
     def moments(self):
         out = np.zeros((self.n, 4), np.float64)
@@ -463,6 +481,7 @@ class VoroAlgo:
         self.max_extra_steps = 5
         self.apex_mode = 1
         self.security_mode = 0
+        self.m_H, self.m_HC = 0, 1
         self.m_debug = debug
         self._bary = None
         self._perm = None
@@ -480,7 +499,7 @@ class VoroAlgo:
 
     def launch(self, saveData=None, stats=None):
         st = self._ctx.solve(self._dom, self.m_K, self.m_P, self.m_V, self.m_T, self.precision,
-                             self.max_extra_steps, self.apex_mode, self.security_mode)
+                             self.max_extra_steps, self.apex_mode, self.security_mode, H=self.m_H, HC=self.m_HC)
         if stats is not None:
             stats.append(st)
         if self.m_keepDistance < 0 or self._dom is None:
@@ -488,6 +507,28 @@ class VoroAlgo:
             self._perm = np.arange(self._bary.shape[0], dtype=np.int32)
         else:
             self._bary, self._perm = self._ctx.compact(self.m_keepVolume)
+        self._hess = self._hess_ids = None
+        if self.m_H > 0:
+            # compact_result (voronoi.cpp:594-631): the output cells' entries,
+            # neighbours renumbered to output indices, dropped when not output
+            vals, ids = self._ctx.hessian()
+            H = self.m_H
+            pos = np.full(self._ctx.n, -1, np.int64)
+            pos[self._perm] = np.arange(self._perm.size)
+            hv = np.zeros((self._perm.size, H, self.m_HC))
+            hi = np.full((self._perm.size, H), -1, np.int32)
+            for o, u in enumerate(self._perm):
+                w = 0
+                for h in range(H):
+                    j = ids[u, h]
+                    if j < 0:
+                        break
+                    if pos[j] < 0:
+                        continue
+                    hi[o, w] = pos[j]
+                    hv[o, w] = vals[u, h]
+                    w += 1
+            self._hess, self._hess_ids = hv.reshape(-1), hi.reshape(-1)
         ok = True if saveData is None else bool(saveData(self))
         return ok
 
@@ -501,7 +542,9 @@ class VoroAlgo:
         return self._perm
 
     def getHessians(self):
-        return np.zeros(0)
+        """m_HC * m_H values per output cell (voronoi.cpp:1179-1212); empty with m_H = 0."""
+        return self._hess if getattr(self, "_hess", None) is not None else np.zeros(0)
 
     def getHessianIds(self):
-        return np.zeros(0, np.int32)
+        """m_H output indices per output cell, -1 after the last (voronoi.cpp:1214-1232)."""
+        return self._hess_ids if getattr(self, "_hess_ids", None) is not None else np.zeros(0, np.int32)

@@ -84,6 +84,9 @@ struct rpd_ctx {
     DevBuf perm, counters, stored, stored64, bin_scratch;
     // outputs, original order
     DevBuf out, status, kused;
+    // Hessian of the last solve (H > 0): n*H*HC values, n*H neighbours
+    DevBuf hess, hids;
+    int hess_H = 0, hess_HC = 1;
     // pass work buffers: up to three passes may run at once, one per stream
     PassBufs pb[4];
     // pass 1 writes lists 0 (bisector kernel) and 1 (finish kernel); the
@@ -309,7 +312,9 @@ void rpd_default_params(rpd_params *p) {
     p->apex_mode = 1;
     p->security_mode = 0;
     p->cull = 1;
-    p->reserved[0] = p->reserved[1] = p->reserved[2] = 0;
+    p->H = 0;  // no Hessian (VoroAlgo::m_H, voronoi.cpp:1048)
+    p->HC = 1;
+    p->reserved[0] = 0;
 }
 
 int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
@@ -472,6 +477,8 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     if (prm->precision < 0 || prm->precision > 2) return set_err(c, RPD_E_ARG, "rpd_solve: bad precision");
     if (prm->K <= 0 || prm->K > 256 || prm->P < 7 || prm->P > 255 || prm->V < 8 || prm->V > 256 || prm->T < 0)
         return set_err(c, RPD_E_ARG, "rpd_solve: caps out of range (0<K<=256, 7<=P<=255, 8<=V<=256, T>=0)");
+    if (prm->H < 0 || prm->H > 4096 || (prm->H > 0 && prm->HC != 1 && prm->HC != 4))
+        return set_err(c, RPD_E_ARG, "rpd_solve: Hessian H in [0, 4096], HC 1 or 4");
     // a domain is read-only device memory: any context on its device may use it
     if (dom && dom->device != c->device) return set_err(c, RPD_E_ARG, "rpd_solve: domain on another device");
     hipSetDevice(c->device);
@@ -483,7 +490,8 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     // done by one wave keeps 4 bytes of LDS per union entry, which costs the
     // finish kernel no residency, and a cell's result does not depend on the
     // cap it succeeds with; a lower cap only sends cells to re-queue passes.
-    T = std::max(T, std::min(kTriangleFloor, rpd_clip_max_T(form0, P, V)));
+    const bool hess = prm->H > 0;
+    T = std::max(T, std::min(kTriangleFloor, rpd_clip_max_T(form0, P, V, hess)));
     // Likewise the vertex cap is the room of the vertex slots it needs
     // (64 per slot): a cap below it saves nothing.
     V = V <= 64 ? 64 : (V <= 128 ? 128 : 256);
@@ -525,6 +533,14 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     RPDCHK(c, c->rslot.ensure(size_t(n) * 4), "alloc resume slots");
     RPDCHK(c, c->lastnb.ensure(size_t(n) * 4), "alloc resume slots");
     RPDCHK(c, c->cnt.ensure(64), "alloc cnt");
+    c->hess_H = hess ? prm->H : 0;
+    c->hess_HC = hess ? prm->HC : 1;
+    if (hess) {
+        RPDCHK(c, c->hess.ensure(size_t(n) * prm->H * prm->HC * sizeof(double)), "alloc hessian");
+        RPDCHK(c, c->hids.ensure(size_t(n) * prm->H * sizeof(int)), "alloc hessian");
+        // no entries (-1) until a cell writes its own
+        HIPCHK(c, hipMemsetAsync(c->hids.ptr, 0xff, size_t(n) * prm->H * sizeof(int), c->stream));
+    }
 
     const DomainDev *dd = dom ? &dom->dev : nullptr;
     double wmin = 0;
@@ -633,6 +649,12 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.rs_count = c->rs_cnt.as<int>() + std::max(rz.out, 0);
         L.rslot = c->rslot.as<int>();
         L.lastnb = c->lastnb.as<int>();
+        if (hess) {
+            L.hess = c->hess.as<double>();
+            L.hids = c->hids.as<int>();
+            L.H = prm->H;
+            L.hc = prm->HC;
+        }
         return rpd_launch_cells(s, ps.form, L, c->pin + sl, false);
     };
     // the finish kernel of a pass whose bisector kernel (and queue length) is done
@@ -679,7 +701,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         if (h[RPD_PLANE_OVERFLOW]) nc.P = std::min(255, int(cap.P * 1.5));
         if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) nc.K = int(std::min<int64_t>(n - 1, std::min(256, 2 * cap.K)));
         if (h[RPD_TRIANGLE_OVERFLOW]) nc.T = std::max(int(cap.T * 1.5) + 1, cnt[13]);
-        nc.T = std::min(nc.T, rpd_clip_max_T(RPD_PREC_FP64, nc.P, nc.V));
+        nc.T = std::min(nc.T, rpd_clip_max_T(RPD_PREC_FP64, nc.P, nc.V, hess));
         return nc;
     };
     auto clear_list = [&](int fl, hipStream_t s) {
@@ -898,6 +920,12 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         L.fl_eo = c->fl[0].eo.as<int>() + n;
         L.fl_sec = c->fl[0].sec.as<int>();
         L.fl_cnt_host = cnt_pin(0);
+        if (hess) {
+            L.hess = c->hess.as<double>();
+            L.hids = c->hids.as<int>();
+            L.H = prm->H;
+            L.hc = prm->HC;
+        }
         if (left1.which == 0) {
             L.rs_out = static_cast<char *>(c->rs[0].ptr);
             L.rs_out_stride = left1.stride;
@@ -986,6 +1014,27 @@ int rpd_read_moments(rpd_ctx *c, double *out4n) {
     hipSetDevice(c->device);
     HIPCHK(c, hipMemcpyAsync(out4n, c->out.ptr, size_t(c->n) * 32, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPD_OK;
+}
+
+int rpd_read_hessian(rpd_ctx *c, double *vals, int32_t *ids) {
+    if (!c || !vals || !ids) return set_err(c, RPD_E_ARG, "rpd_read_hessian");
+    if (!c->solved || c->hess_H <= 0) return set_err(c, RPD_E_STATE, "rpd_read_hessian: no solve with H > 0");
+    hipSetDevice(c->device);
+    const size_t m = size_t(c->n) * size_t(c->hess_H);
+    HIPCHK(c, hipMemcpyAsync(vals, c->hess.ptr, m * c->hess_HC * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ids, c->hids.ptr, m * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPD_OK;
+}
+
+int rpd_hessian_device(rpd_ctx *c, const double **vals, const int32_t **ids, int32_t *H, int32_t *HC) {
+    if (!c || !vals || !ids || !H || !HC) return set_err(c, RPD_E_ARG, "rpd_hessian_device");
+    if (!c->solved || c->hess_H <= 0) return set_err(c, RPD_E_STATE, "rpd_hessian_device: no solve with H > 0");
+    *vals = c->hess.as<double>();
+    *ids = c->hids.as<int32_t>();
+    *H = c->hess_H;
+    *HC = c->hess_HC;
     return RPD_OK;
 }
 

@@ -107,6 +107,11 @@ struct ClipArgs {
     const char *rs_in;
     int rs_in_stride, rs_in_P;
     int *rslot, *lastnb, *fl_sec;
+    // Hessian (finish kernel, H > 0): per original seed up to H entries of
+    // hc values in hess and the neighbours' original indices in hids
+    double *hess;
+    int *hids;
+    int H, hc;
 };
 
 // Appends a failed cell (stored id sid) to the failure list: overflows
@@ -114,7 +119,7 @@ struct ClipArgs {
 // inconsistent_boundary and find_another_beginning_vertex when the form was
 // not FP64 (in FP64 a recomputation would fail the same way).  One lane.
 __device__ __forceinline__ void note_failure(const ClipArgs &a, int sid, int st, bool resumable = false) {
-    if (!a.fl_cnt || st == RPD_SUCCESS || st == RPD_EMPTY_CELL) return;
+    if (!a.fl_cnt || st == RPD_SUCCESS || st == RPD_EMPTY_CELL || st == RPD_HESSIAN_OVERFLOW) return;
     const bool exact = st == RPD_NEEDS_EXACT_PREDICATES || st == RPD_INCONSISTENT_BOUNDARY ||
                        st == RPD_FIND_ANOTHER_BEGINNING_VERTEX;
     if (exact && a.exact_final) return;
@@ -427,17 +432,17 @@ struct WaveCell {
     }
 };
 
-// LDS of one wave: planes[P] | plane sources src[P] (int16) | rt[64S] |
+// LDS of one wave: planes[P] | plane sources src[P] (int32) | rt[64S] |
 // nvs[64S] | nxt[256] | tu[T] (finish kernel) | chunk / cone planes cpl[64] |
 // go_out flags[16] | triangle corners tcor[16][9] (finish kernel)
 template <typename R, typename RP, bool FILTER, int S, typename PS>
 __device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, FILTER, S, PS> &c,
-                                         int16_t **src = nullptr, int **tu = nullptr, Pl<PS> **cpl = nullptr,
+                                         int **src = nullptr, int **tu = nullptr, Pl<PS> **cpl = nullptr,
                                          int **goo = nullptr, double **tcor = nullptr) {
     c.planes = reinterpret_cast<Pl<PS> *>(L);
     size_t off = (size_t(P) * sizeof(Pl<PS>) + 15) & ~size_t(15);
-    if (src) *src = reinterpret_cast<int16_t *>(L + off);
-    off += (2 * size_t(P) + 15) & ~size_t(15);
+    if (src) *src = reinterpret_cast<int *>(L + off);
+    off += (4 * size_t(P) + 15) & ~size_t(15);
     c.rt = reinterpret_cast<uint32_t *>(L + off);
     off += 4 * 64 * S;
     c.nvs = reinterpret_cast<uint32_t *>(L + off);
@@ -467,10 +472,15 @@ constexpr size_t kVtxRecBytes = 40;  // sizeof(VtxRec<double>), the larger one
 // LDS bytes of one wave (pl: bytes of one stored plane; corners: the finish
 // kernel's triangle corners and vertex copy)
 __host__ __device__ constexpr size_t finish_wave_lds(int P, int S, int T, size_t pl, bool corners = false) {
-    return ((((((P * pl + 15) & ~size_t(15)) + ((2 * size_t(P) + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) +
+    return ((((((P * pl + 15) & ~size_t(15)) + ((4 * size_t(P) + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) +
                256 + 4 * size_t(T)) +
               15) & ~size_t(15)) + 64 * pl + 64 + (corners ? 16 * 9 * 8 + 64 * size_t(S) * kVtxRecBytes : 0) +
             15) & ~size_t(15);
+}
+// LDS the Hessian adds to a finish-kernel wave: per plane 4 sums, per vertex
+// slot 4 values and a plane id
+__host__ __device__ constexpr size_t hessian_wave_lds(int P, int S) {
+    return 32 * size_t(P) + 64 * size_t(S) * (32 + 4);
 }
 
 // Power bisector of the seed S0 and neighbour B in the seed-local frame
@@ -509,10 +519,11 @@ struct RecHeader {
 // ---------------------------------------------------------------------------
 // Minimum waves per SIMD the bisector kernel is compiled for (one vertex
 // slot per lane): its run time is set by latency, so occupancy pays more than
-// the few registers the compiler then spills (measured at 1M seeds: FP64 5,
-// FP32 forms 7).  Wider cells (S > 1, re-queue passes) are left unconstrained.
+// the few registers the compiler then spills (measured at 1M seeds: FP64 5;
+// FP32 forms 8, which they reach without spills).  Wider cells (S > 1,
+// re-queue passes) are left unconstrained.
 template <typename R, int S>
-constexpr int kClipWaves = S > 1 ? 1 : (sizeof(R) == 8 ? 5 : 7);
+constexpr int kClipWaves = S > 1 ? 1 : (sizeof(R) == 8 ? 5 : 8);
 
 template <typename R, typename RP, bool FILTER, int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kClipWaves<R, S>)))
@@ -525,11 +536,12 @@ void clip_cells_kernel(ClipArgs a) {
     if (t >= a.n_ids) return;  // wave-uniform
 
     // planes in the test precision; src[p] records where plane p came from
-    // (neighbour index in the list, or -1 - f for cube face f) so that the
+    // (the neighbour's stored id, or -1 - f for cube face f), so that the
     // record of a queued cell carries the planes in the geometry precision
+    // and each plane's neighbour (CCPlaneNeighbour, ConvexCell.cl:329-331)
     WaveCell<R, RP, FILTER, S, R> c;
     Pl<R> *chunk = nullptr;  // planes of the current 64 neighbours
-    int16_t *src = nullptr;
+    int *src = nullptr;
     bind_lds(lds_raw + size_t(wv) * a.lds_stride, a.P, 0, c, &src, nullptr, &chunk);
     c.P = a.P;
     c.V = a.V;
@@ -593,7 +605,11 @@ void clip_cells_kernel(ClipArgs a) {
         const Pl<RP> *rpl = reinterpret_cast<const Pl<RP> *>(rs_rec + sizeof(RecHeader));
         const uint32_t *rtr =
             reinterpret_cast<const uint32_t *>(rs_rec + sizeof(RecHeader) + size_t(a.rs_in_P) * sizeof(Pl<RP>));
-        for (int p = lane; p < c.nb_plane; p += 64) c.planes[p] = cvt<R>(rpl[p]);
+        const int *rnid = reinterpret_cast<const int *>(rtr + nent);
+        for (int p = lane; p < c.nb_plane; p += 64) {
+            c.planes[p] = cvt<R>(rpl[p]);
+            src[p] = rnid[p];
+        }
         wsync();
 #pragma unroll
         for (int s = 0; s < S; ++s) {
@@ -606,7 +622,7 @@ void clip_cells_kernel(ClipArgs a) {
         // ---- CCInit: bounding cube in the local frame ----------------------
         if (lane < 6) {
             c.planes[lane] = cvt<R>(cube_plane(lane, S0, eps));
-            src[lane] = int16_t(-1 - lane);
+            src[lane] = -1 - lane;
         }
         wsync();
         c.nb_plane = 6;
@@ -709,7 +725,8 @@ void clip_cells_kernel(ClipArgs a) {
         }
         // clip by the plane of neighbour base + f_clip
         const Pl<R> e = chunk[f_clip];
-        if (lane == 0) src[c.nb_plane] = int16_t(base + f_clip);
+        const int nbid = rlane(cnb, f_clip);
+        if (lane == 0) src[c.nb_plane] = nbid;
         const int res = c.clip(e, -1, e);
         kused = base + f_clip + 1;
         if (res >= 0) {
@@ -756,6 +773,7 @@ void clip_cells_kernel(ClipArgs a) {
                             : a.rec + size_t(slot) * size_t(a.rec_stride);
         Pl<RP> *rpl = reinterpret_cast<Pl<RP> *>(rec + sizeof(RecHeader));
         uint32_t *rtr = reinterpret_cast<uint32_t *>(rec + sizeof(RecHeader) + size_t(P) * sizeof(Pl<RP>));
+        int *rnid = reinterpret_cast<int *>(rtr + 64 * S);
         // a queued cell's vertices are compacted; a kept state holds each
         // triple at its lane (kNoTri where none), so that the resumed cell
         // has the lanes, and so the sums, of one uninterrupted run
@@ -771,8 +789,9 @@ void clip_cells_kernel(ClipArgs a) {
             } else {
                 const int sj = src[p];
                 double bl;
-                rpl[p] = cvt<RP>(sj < 0 ? cube_plane(-1 - sj, S0, eps) : bisector_plane(S0, a.seeds[nrow[sj]], bl));
+                rpl[p] = cvt<RP>(sj < 0 ? cube_plane(-1 - sj, S0, eps) : bisector_plane(S0, a.seeds[sj], bl));
             }
+            rnid[p] = src[p];
         }
         int carry = 0;
 #pragma unroll
@@ -832,10 +851,11 @@ __device__ __forceinline__ uint32_t box_neighbours(bool xl, bool xh, bool yl, bo
 // either (at most 3 more planes, at most 2 more vertices per clip), so the
 // outcome is the reference's.
 // ---------------------------------------------------------------------------
-template <typename R, typename RP, bool FILTER, int S>
+template <typename R, typename RP, bool FILTER, int S, bool HESS>
 // 4 waves per SIMD for S = 1 (128 VGPRs): measured at 1M seeds against 5 and
-// 6, whose spills cost more than the extra residency gives.
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == 1 ? 4 : 1)))
+// 6, whose spills cost more than the extra residency gives.  HESS: the
+// Hessian variant (one wave per cell).
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(S == 1 && !HESS ? 4 : 1)))
 void finish_cells_kernel(ClipArgs a, int nq, int W) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     const int lane = lane_id();
@@ -919,53 +939,111 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     // ---- integration (CCExportBaryVolume): factor/6 * (volume, moments) of
     // the current cell in the local frame, FP64 wave sums --------------------
     const RP vol_eps2 = sizeof(RP) == 8 ? RP(1e-12) : RP(1e-6);  // voronoi.cpp:919-922
+    // Hessian sums: per record plane 4 doubles (area-weighted centroid in the
+    // local frame, area), per vertex slot staging for their reduction
+    double *hacc = nullptr, *hval = nullptr;
+    int *hpid = nullptr;
+    const int nbk = c.nb_plane;  // the record's planes; cone planes come after
+    if (HESS) {
+        hacc = reinterpret_cast<double *>(reinterpret_cast<char *>(tcor + 16 * 9) + 64 * size_t(S) * kVtxRecBytes);
+        hval = hacc + 4 * size_t(P);
+        hpid = reinterpret_cast<int *>(hval + 4 * 64);
+        for (int i = lane; i < 4 * nbk; i += 64) hacc[i] = 0.0;
+        wsync();
+    }
     auto integrate = [&](double factor) -> double4 {
         RP sw = 0, sx = 0, sy = 0, sz = 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if (!((c.valid[s] >> lane) & 1)) continue;
-            const uint32_t tr = c.tri[s];
-            RP Ax, Ay, Az;
-            vertex_rp(s, Ax, Ay, Az);
-            const RP ACx = -Ax, ACy = -Ay, ACz = -Az;  // seed at the origin
-            const Pl<RP> pl[3] = {c.planes[t0(tr)], c.planes[t1(tr)], c.planes[t2(tr)]};
-            RP opx[3], opy[3], opz[3], olx[3], oly[3], olz[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const RP nn = pl[k].x * pl[k].x + pl[k].y * pl[k].y + pl[k].z * pl[k].z;
-                const RP f = (ACx * pl[k].x + ACy * pl[k].y + ACz * pl[k].z) / nn;
-                opx[k] = ACx - f * pl[k].x;
-                opy[k] = ACy - f * pl[k].y;
-                opz[k] = ACz - f * pl[k].z;
-            }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const Pl<RP> &p = pl[k], &q = pl[(k + 1) % 3];
-                const RP cx = p.y * q.z - p.z * q.y, cy = p.z * q.x - p.x * q.z, cz = p.x * q.y - p.y * q.x;
-                const RP c2 = cx * cx + cy * cy + cz * cz;
-                const RP pp = p.x * p.x + p.y * p.y + p.z * p.z, qq = q.x * q.x + q.y * q.y + q.z * q.z;
-                if (c2 < vol_eps2 * pp * qq) {
-                    olx[k] = oly[k] = olz[k] = RP(0);
-                } else {
-                    const RP f = (ACx * cx + ACy * cy + ACz * cz) / c2;
-                    olx[k] = f * cx;
-                    oly[k] = f * cy;
-                    olz[k] = f * cz;
+            const bool vok = (c.valid[s] >> lane) & 1;
+            // per plane k of the vertex: area (CCExportBaryVolumeAndArea,
+            // ConvexCell.cl:621-674) and area-weighted centroid
+            double ha[3] = {0.0, 0.0, 0.0}, hx[3] = {0.0, 0.0, 0.0}, hy[3] = {0.0, 0.0, 0.0},
+                   hz[3] = {0.0, 0.0, 0.0};
+            uint32_t tr = c.tri[s];
+            if (!HESS && !vok) continue;
+            if (vok) {
+                RP Ax, Ay, Az;
+                vertex_rp(s, Ax, Ay, Az);
+                const RP ACx = -Ax, ACy = -Ay, ACz = -Az;  // seed at the origin
+                const Pl<RP> pl[3] = {c.planes[t0(tr)], c.planes[t1(tr)], c.planes[t2(tr)]};
+                RP opx[3], opy[3], opz[3], olx[3], oly[3], olz[3];
+    #pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const RP nn = pl[k].x * pl[k].x + pl[k].y * pl[k].y + pl[k].z * pl[k].z;
+                    const RP f = (ACx * pl[k].x + ACy * pl[k].y + ACz * pl[k].z) / nn;
+                    opx[k] = ACx - f * pl[k].x;
+                    opy[k] = ACy - f * pl[k].y;
+                    opz[k] = ACz - f * pl[k].z;
+                }
+    #pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const Pl<RP> &p = pl[k], &q = pl[(k + 1) % 3];
+                    const RP cx = p.y * q.z - p.z * q.y, cy = p.z * q.x - p.x * q.z, cz = p.x * q.y - p.y * q.x;
+                    const RP c2 = cx * cx + cy * cy + cz * cz;
+                    const RP pp = p.x * p.x + p.y * p.y + p.z * p.z, qq = q.x * q.x + q.y * q.y + q.z * q.z;
+                    if (c2 < vol_eps2 * pp * qq) {
+                        olx[k] = oly[k] = olz[k] = RP(0);
+                    } else {
+                        const RP f = (ACx * cx + ACy * cy + ACz * cz) / c2;
+                        olx[k] = f * cx;
+                        oly[k] = f * cy;
+                        olz[k] = f * cz;
+                    }
+                }
+    #pragma unroll
+                for (int k = 0; k < 3; ++k) {
+    #pragma unroll
+                    for (int st = 0; st < 2; ++st) {
+                        const int j = (k + 3 - st) % 3;
+                        // det3 of rows AC, onPlane_k, onLine_j (ConvexCell.cl:72-77)
+                        const RP d = ACx * (opy[k] * olz[j] - opz[k] * oly[j]) - opx[k] * (ACy * olz[j] - ACz * oly[j]) +
+                                     olx[j] * (ACy * opz[k] - ACz * opy[k]);
+                        const RP w = st ? d : -d;
+                        sw += w;
+                        sx += w * (Ax + RP(0.25) * (ACx + opx[k] + olx[j]));
+                        sy += w * (Ay + RP(0.25) * (ACy + opy[k] + oly[j]));
+                        sz += w * (Az + RP(0.25) * (ACz + opz[k] + olz[j]));
+                        if (HESS) {
+                            // det3 of rows onPlane_k, onLine_j, n_k, halved
+                            const double dt = double(opx[k]) * (double(oly[j]) * pl[k].z - double(olz[j]) * pl[k].y) -
+                                              double(opy[k]) * (double(olx[j]) * pl[k].z - double(olz[j]) * pl[k].x) +
+                                              double(opz[k]) * (double(olx[j]) * pl[k].y - double(oly[j]) * pl[k].x);
+                            const double ar = (st ? 0.5 : -0.5) * factor * dt;
+                            ha[k] += ar;
+                            hx[k] += ar * (double(Ax) + (double(opx[k]) + double(olx[j])) / 3.0);
+                            hy[k] += ar * (double(Ay) + (double(opy[k]) + double(oly[j])) / 3.0);
+                            hz[k] += ar * (double(Az) + (double(opz[k]) + double(olz[j])) / 3.0);
+                        }
+                    }
                 }
             }
+            if (HESS) {
+                // per plane, in a fixed order: plane slot k, then lane; the
+                // lane that owns plane p (p = lane, lane + 64, ...) adds
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-#pragma unroll
-                for (int st = 0; st < 2; ++st) {
-                    const int j = (k + 3 - st) % 3;
-                    // det3 of rows AC, onPlane_k, onLine_j (ConvexCell.cl:72-77)
-                    const RP d = ACx * (opy[k] * olz[j] - opz[k] * oly[j]) - opx[k] * (ACy * olz[j] - ACz * oly[j]) +
-                                 olx[j] * (ACy * opz[k] - ACz * opy[k]);
-                    const RP w = st ? d : -d;
-                    sw += w;
-                    sx += w * (Ax + RP(0.25) * (ACx + opx[k] + olx[j]));
-                    sy += w * (Ay + RP(0.25) * (ACy + opy[k] + oly[j]));
-                    sz += w * (Az + RP(0.25) * (ACz + opz[k] + olz[j]));
+                for (int k = 0; k < 3; ++k) {
+                    hpid[lane] = vok ? int(k == 0 ? t0(tr) : (k == 1 ? t1(tr) : t2(tr))) : -1;
+                    hval[4 * lane + 0] = hx[k];
+                    hval[4 * lane + 1] = hy[k];
+                    hval[4 * lane + 2] = hz[k];
+                    hval[4 * lane + 3] = ha[k];
+                    wsync();
+                    for (int p = lane; p < nbk; p += 64) {
+                        double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;
+                        for (int j = 0; j < 64; ++j) {
+                            if (hpid[j] != p) continue;
+                            e0 += hval[4 * j + 0];
+                            e1 += hval[4 * j + 1];
+                            e2 += hval[4 * j + 2];
+                            e3 += hval[4 * j + 3];
+                        }
+                        hacc[4 * p + 0] += e0;
+                        hacc[4 * p + 1] += e1;
+                        hacc[4 * p + 2] += e2;
+                        hacc[4 * p + 3] += e3;
+                    }
+                    wsync();
                 }
             }
         }
@@ -1422,8 +1500,47 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         }
     }
     const double acc_m = cadd.w + sm, acc_x = cadd.x + sx, acc_y = cadd.y + sy, acc_z = cadd.z + sz;
+    // Hessian entries (ConvexCell.cl:1133-1151): the record's planes towards
+    // a seed whose face keeps more than 0.2 of area, in plane order, with the
+    // neighbours' original indices; more than H: hessian_overflow, the first
+    // H written
+    int st_out = RPD_SUCCESS;
+    if (HESS) {
+        const int *rnid = reinterpret_cast<const int *>(rtr + 64 * S);
+        const int H = a.H;
+        int cnt = 0;
+        for (int pb = 0; pb < nbk; pb += 64) {
+            const int p = pb + lane;
+            int nb = -1;
+            bool keep = false;
+            if (p < nbk) {
+                nb = rnid[p];
+                keep = nb >= 0 && hacc[4 * p + 3] > 0.2;
+            }
+            const uint64_t m = ballot(keep);
+            const int pos = cnt + mbcnt(m);
+            if (keep && pos < H) {
+                const size_t e = size_t(orig) * size_t(H) + size_t(pos);
+                const double ar = hacc[4 * p + 3];
+                a.hids[e] = a.perm[nb];
+                if (a.hc == 1) {
+                    a.hess[e] = ar;
+                } else {
+                    a.hess[4 * e + 0] = hacc[4 * p + 0] + ar * S0.x;
+                    a.hess[4 * e + 1] = hacc[4 * p + 1] + ar * S0.y;
+                    a.hess[4 * e + 2] = hacc[4 * p + 2] + ar * S0.z;
+                    a.hess[4 * e + 3] = ar;
+                }
+            }
+            cnt += __popcll(m);
+        }
+        if (cnt > H)
+            st_out = RPD_HESSIAN_OVERFLOW;
+        else if (cnt < H && lane == 0)
+            a.hids[size_t(orig) * size_t(H) + size_t(cnt)] = -1;
+    }
     // local first moments -> global frame
-    finish(RPD_SUCCESS, acc_x + acc_m * S0.x, acc_y + acc_m * S0.y, acc_z + acc_m * S0.z, acc_m);
+    finish(st_out, acc_x + acc_m * S0.x, acc_y + acc_m * S0.y, acc_z + acc_m * S0.z, acc_m);
 }
 
 }  // namespace rpd
@@ -1447,18 +1564,19 @@ static size_t lds_limit() {
 static size_t finish_cell_stride(int T, int W) { return W > 1 ? (64 + 32 * size_t(T) + 15) & ~size_t(15) : 64; }
 
 // LDS of one finish-kernel cell handled by a single wave.
-static size_t finish_cell_lds(int form, int P, int V, int T) {
+static size_t finish_cell_lds(int form, int P, int V, int T, bool hess) {
     const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
-    return finish_cell_stride(T, 1) + finish_wave_lds(P, S, T, plane_bytes(form), true);
+    return finish_cell_stride(T, 1) + finish_wave_lds(P, S, T, plane_bytes(form), true) +
+           (hess ? hessian_wave_lds(P, S) : 0);
 }
 
-int rpd_clip_max_T(int form, int P, int V) {
+int rpd_clip_max_T(int form, int P, int V, bool hess) {
     const size_t lim = lds_limit();
     int lo = 0, hi = 1 << 20;
-    if (finish_cell_lds(form, P, V, 0) > lim) return -1;
+    if (finish_cell_lds(form, P, V, 0, hess) > lim) return -1;
     while (lo < hi) {
         const int mid = lo + (hi - lo + 1) / 2;
-        if (finish_cell_lds(form, P, V, mid) <= lim)
+        if (finish_cell_lds(form, P, V, mid, hess) <= lim)
             lo = mid;
         else
             hi = mid - 1;
@@ -1466,10 +1584,11 @@ int rpd_clip_max_T(int form, int P, int V) {
     return lo;
 }
 
+// header | P planes | 64 S vertex triples | P plane neighbours (int)
 size_t rpd_clip_record_bytes(int form, int P, int V) {
     const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
     const size_t pl = plane_bytes(form);
-    return (sizeof(RecHeader) + size_t(P) * pl + 4 * 64 * size_t(S) + 15) & ~size_t(15);
+    return (sizeof(RecHeader) + size_t(P) * pl + 4 * 64 * size_t(S) + 4 * size_t(P) + 15) & ~size_t(15);
 }
 
 static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
@@ -1522,6 +1641,10 @@ static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     a.rs_in_P = L.rs_in_P;
     a.rslot = L.rslot;
     a.lastnb = L.lastnb;
+    a.hess = L.hess;
+    a.hids = L.hids;
+    a.H = L.hess && L.hids ? L.H : 0;
+    a.hc = L.hc == 4 ? 4 : 1;
     a.exact_final = form == RPD_PREC_FP64 ? 1 : 0;
     return RPD_OK;
 }
@@ -1580,13 +1703,15 @@ int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq, int
     // (re-queue passes) spread each cell over W waves.
     {
         const size_t pl = plane_bytes(form);
-        const size_t wstride = finish_wave_lds(L.P, S, a.T, pl, true);
+        const bool hess = a.H > 0;
+        const size_t wstride = finish_wave_lds(L.P, S, a.T, pl, true) + (hess ? hessian_wave_lds(L.P, S) : 0);
         const size_t kLdsMax = lds_limit();
         if (finish_cell_stride(a.T, 1) + wstride > kLdsMax) return RPD_E_LIMIT;
         // waves per cell from the cells that share the device: this queue's
-        // and those of finish kernels running beside it
+        // and those of finish kernels running beside it (the Hessian variant:
+        // one wave per cell)
         const int64_t load = int64_t(nq) + std::max(beside, 0);
-        int W = load <= 2048 ? 8 : (load <= 8192 ? 4 : (load <= 32768 ? 2 : 1));
+        int W = hess || load > 32768 ? 1 : (load <= 2048 ? 8 : (load <= 8192 ? 4 : 2));
         while (W > 1 && finish_cell_stride(a.T, W) + W * wstride > kLdsMax) W >>= 1;
         const size_t cstride = finish_cell_stride(a.T, W);
         a.lds_stride = int(wstride);
@@ -1599,21 +1724,31 @@ int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq, int
         const int cpb = 1;
         const unsigned blocks = unsigned((nq + cpb - 1) / cpb);
         const size_t shm = size_t(cpb) * cstride + size_t(wpb) * wstride;
-#define RPD_CLIP_B(R, RP, F)                                                                  \
-    do {                                                                                      \
-        if (S == 1)                                                                           \
-            finish_cells_kernel<R, RP, F, 1><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
-        else if (S == 2)                                                                      \
-            finish_cells_kernel<R, RP, F, 2><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
-        else                                                                                  \
-            finish_cells_kernel<R, RP, F, 4><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+#define RPD_CLIP_B(R, RP, F, HS)                                                                  \
+    do {                                                                                          \
+        if (S == 1)                                                                               \
+            finish_cells_kernel<R, RP, F, 1, HS><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+        else if (S == 2)                                                                          \
+            finish_cells_kernel<R, RP, F, 2, HS><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+        else                                                                                      \
+            finish_cells_kernel<R, RP, F, 4, HS><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
     } while (0)
-        if (form == RPD_PREC_FP64)
-            RPD_CLIP_B(double, double, false);
-        else if (form == RPD_PREC_MIXED)
-            RPD_CLIP_B(float, double, true);
-        else
-            RPD_CLIP_B(float, float, false);
+        if (form == RPD_PREC_FP64) {
+            if (hess)
+                RPD_CLIP_B(double, double, false, true);
+            else
+                RPD_CLIP_B(double, double, false, false);
+        } else if (form == RPD_PREC_MIXED) {
+            if (hess)
+                RPD_CLIP_B(float, double, true, true);
+            else
+                RPD_CLIP_B(float, double, true, false);
+        } else {
+            if (hess)
+                RPD_CLIP_B(float, float, false, true);
+            else
+                RPD_CLIP_B(float, float, false, false);
+        }
 #undef RPD_CLIP_B
     }
     return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;

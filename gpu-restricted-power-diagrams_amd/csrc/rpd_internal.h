@@ -103,6 +103,11 @@ struct ClipLaunch {
     const char *rs_in;     // records this pass resumes from (nullable; FP64 form)
     int rs_in_stride, rs_in_P;
     int *rslot, *lastnb;   // per stored id
+    // Hessian (H > 0): per original seed H entries of hc values, neighbours'
+    // original indices (-1 after the last); finish kernel only
+    double *hess;
+    int *hids;
+    int H, hc;
 };
 
 constexpr int kFailCounters = 16;
@@ -132,7 +137,7 @@ int rpd_cover_skip(hipStream_t st, const CoverArgs &c, DevBuf &scratch, int *ids
 size_t rpd_clip_record_bytes(int form, int P, int V);
 // Largest T whose finish-kernel cell fits the current device's workgroup LDS
 // (-1: none).
-int rpd_clip_max_T(int form, int P, int V);
+int rpd_clip_max_T(int form, int P, int V, bool hess = false);
 // form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32.  The bisector-loop
 // kernel of L.n_ids cells, then copies to the host the number of cells it
 // queued (and the failure counters when L.fl_cnt_host is set); waits for

@@ -58,7 +58,7 @@ struct Request {
 int64_t footprint_bytes(int64_t n, const rpd_params &p) {
     const int64_t K = std::max(1, p.K), P = std::max(1, p.P);
     const int64_t S = p.V <= 64 ? 1 : (p.V <= 128 ? 2 : 4);
-    const int64_t rec = 16 + P * 32 + 256 * S;
+    const int64_t rec = 16 + P * 36 + 256 * S;
     return n * (32 + 64 + 4 + 40 + 4 + 72 + 8 + 1 + 16 + 2 * (4 * K + rec)) +
            (std::min(n, n * 2 / 5 + 1024)) * rec + (int64_t(1) << 20);
 }
@@ -102,6 +102,8 @@ struct rpd_server {
             }
             r->out.assign(size_t(r->n) * 4, 0.0);
             int rc = rpd_set_seeds(ctx, r->seeds.data(), r->n);
+            // served requests return moments only: no Hessian
+            r->prm.H = 0;
             if (rc == RPD_OK) rc = rpd_solve(ctx, r->dom, &r->prm, nullptr);
             if (rc == RPD_OK) rc = rpd_read_moments(ctx, r->out.data());
             r->rc = rc;

@@ -141,7 +141,14 @@ typedef struct {
     int32_t cull;                 /* 1 (default): end a cell as soon as its
                                      circumsphere provably misses the domain
                                      (restricted volume 0); 0: clip it fully  */
-    int32_t reserved[3];
+    int32_t H;                    /* Hessian (VoroAlgo::m_H, voronoi.h:214-215;
+                                     CHESSIAN, ConvexCell.cl:621-678, 1133-1151):
+                                     up to H neighbours per seed, 0 (default)
+                                     none                                     */
+    int32_t HC;                   /* values per Hessian entry (m_HC): 1 face
+                                     area, 4 face centroid times area (x, y, z)
+                                     and area                                 */
+    int32_t reserved[1];
 } rpd_params;
 
 /* Per-solve statistics (VoroAlgoStat, voronoi.h:60-103). */
@@ -208,6 +215,16 @@ int rpd_read_status(rpd_ctx *ctx, int32_t *out_n);
 int rpd_read_perm(rpd_ctx *ctx, int32_t *out_n);
 /* Device pointer of the per-original-seed moments (getBarycentersCL). */
 int rpd_moments_device(rpd_ctx *ctx, const double **out_ptr);
+/* Replaces VoroAlgo::getHessians / getHessianIds (voronoi.cpp:1179-1232) for
+   a solve with H > 0, per original seed: vals[n*H*HC] and ids[n*H], the
+   neighbours (original indices) whose shared face keeps more than 0.2 of
+   area in the frame [0,1000]^3, in the cell's plane order, -1 after the last
+   (none for a seed that failed or has volume 0; H entries and status
+   hessian_overflow when there are more).  The face values are restricted to
+   the domain like the moments. */
+int rpd_read_hessian(rpd_ctx *ctx, double *vals, int32_t *ids);
+/* Device pointers of the same (valid until the next solve); H, HC of it. */
+int rpd_hessian_device(rpd_ctx *ctx, const double **vals, const int32_t **ids, int32_t *H, int32_t *HC);
 
 /* kNN only (KNearests::buildKnearests, knearests.cpp:238-315): K nearest of
    every seed, in stored order, ascending (d^2, original index); -1 padded.
