@@ -29,6 +29,8 @@ struct Options {
     int K = 20, P = 25, V = 30, T = 100;  // Options.cpp:20-28
     double keepVolume = 0.001;            // USE_DOUBLE default (Options.cpp:36-40)
     int precision = RPD_PREC_MIXED;
+    int H = 50, HC = 1;                   // Options.cpp:190-191
+    bool hessian = false;                 // --hessian (Options.cpp:77, 269)
     bool help = false, points4D = false, loadBinary = false, checkGrid = false, invariants = false, debug = false;
     std::string grid, saveBinary;
     std::vector<std::string> outputs;
@@ -49,6 +51,9 @@ void usage(const char *prog) {
               << "\t--keepVolume=v: volume below which a cell is reported as unused[default 0.001]\n"
               << "\t--grid=FILE: set the grid input file to FILE\n"
               << "\t--checkGrid: checks if the grid if valid\n"
+              << "\t--H=h: set the max number of Hessian values in a row[default 50]\n"
+              << "\t--HC=C: set the number of Hessian components(1 or 4)[default 1]\n"
+              << "\t--hessian: compute the hessian value\n"
               << "\t--precision=fp64|mixed|fp32: clip precision form[default mixed]\n"
               << "\t--output=xyz|mg|volume: outputs (gridGPU.xyz, gridGPU_mg.xyzw, volumes on stderr)\n"
               << "\t--printInvariants: displays the final volume and barycenter\n"
@@ -81,6 +86,9 @@ bool parse(int argc, char **argv, Options &o, std::string &file) {
         else if (a == "--P") o.P = std::atoi(val().c_str());
         else if (a == "--V") o.V = std::atoi(val().c_str());
         else if (a == "--T") o.T = std::atoi(val().c_str());
+        else if (a == "--H") o.H = std::atoi(val().c_str());
+        else if (a == "--HC") o.HC = std::atoi(val().c_str());
+        else if (a == "--hessian") o.hessian = true;
         else if (a == "--keepVolume") {
             o.keepVolume = std::atof(val().c_str());
             if (o.keepVolume <= 0) o.keepVolume = 0.001;
@@ -183,6 +191,8 @@ int main(int argc, char **argv) {
         algo.m_V = o.V;
         algo.m_T = o.T;
         algo.m_precision = o.precision;
+        algo.m_H = o.hessian ? o.H : 0;  // Options.cpp:268-270
+        algo.m_HC = o.HC;
         algo.setInputPoints(pts);
         if (hasGrid) algo.setGrid(grid);
         if (!algo.launch([&](rpd::VoroAlgo const &a) {

@@ -122,6 +122,8 @@ public:
         p.T = m_T;
         p.precision = m_precision;
         p.max_extra_steps = m_maxExtraSteps;
+        p.H = m_H;
+        p.HC = m_HC;
         rpd_stats st{};
         if (rpd_solve(m_ctx, m_dom, &p, &st) != RPD_OK) {
             m_error = rpd_last_error(m_ctx);
@@ -129,6 +131,13 @@ public:
         }
         m_out.assign(size_t(m_n) * 4, 0.0);
         if (rpd_read_moments(m_ctx, m_out.data()) != RPD_OK) return false;
+        m_hess.clear();
+        m_hids.clear();
+        if (m_H > 0) {
+            m_hess.assign(size_t(m_n) * m_H * m_HC, 0.0);
+            m_hids.assign(size_t(m_n) * m_H, -1);
+            if (rpd_read_hessian(m_ctx, m_hess.data(), m_hids.data()) != RPD_OK) return false;
+        }
         m_perm.resize(size_t(m_n));
         for (int64_t i = 0; i < m_n; ++i) m_perm[size_t(i)] = int(i);  // per-original-seed layout
         const int np = st.passes < RPD_MAX_PASSES ? st.passes : RPD_MAX_PASSES;
@@ -157,8 +166,10 @@ public:
     }
     std::vector<int> const &getPermutationToUser() const { return m_perm; }
     std::vector<int> getPermutationToInputPoints() const { return m_perm; }
-    std::vector<double> getHessians() const { return {}; }  // Hessian output is not built
-    std::vector<int> getHessianIds() const { return {}; }
+    // m_HC * m_H values and m_H neighbour ids (-1 after the last) per seed
+    // (getHessians / getHessianIds, voronoi.cpp:1179-1232); empty with m_H = 0
+    std::vector<double> const &getHessians() const { return m_hess; }
+    std::vector<int> const &getHessianIds() const { return m_hids; }
     std::vector<int32_t> getStatus() const {
         std::vector<int32_t> s(static_cast<size_t>(m_n));
         rpd_read_status(m_ctx, s.data());
@@ -182,8 +193,8 @@ private:
     rpd_ctx *m_ctx = nullptr;
     rpd_domain *m_dom = nullptr;
     int64_t m_n = 0;
-    std::vector<double> m_input, m_out;
-    std::vector<int> m_perm;
+    std::vector<double> m_input, m_out, m_hess;
+    std::vector<int> m_perm, m_hids;
     rpd_stats m_last{};
     std::string m_error;
 };
