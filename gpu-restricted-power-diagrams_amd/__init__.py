@@ -157,6 +157,7 @@ def lib():
                                               ctypes.POINTER(I32)]),
         "rpd_knn": (ctypes.c_int, [VP, I32, I32, VP]),
         "rpd_compact": (ctypes.c_int, [VP, ctypes.c_double, ctypes.POINTER(I64), VP, VP]),
+        "rpd_compact_used": (ctypes.c_int, [VP, ctypes.c_double, I32, ctypes.POINTER(I64), VP, VP]),
         "rpd_server_create": (ctypes.c_int, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
         "rpd_server_destroy": (None, [VP]),
         "rpd_server_submit": (ctypes.c_int, [VP, VP, VP, I64, ctypes.POINTER(Params), ctypes.POINTER(I64)]),
@@ -375,10 +376,22 @@ class Context:
                    "rpd_compact")
         return out, perm
 
+    def compact_used(self, keep_volume, keep_distance):
+        """rpd_compact_used: the used cells (m >= keep_volume) and keep_distance
+        rings of face neighbours around them; last-ring cells without volume
+        get w = NO_CELL_LIMITS.  Returns (out[count, 4], perm_to_user[count])."""
+        cnt = I64(0)
+        out = np.zeros((self.n, 4), np.float64)
+        perm = np.zeros(self.n, np.int32)
+        self.check(lib().rpd_compact_used(self._h, float(keep_volume), int(keep_distance), ctypes.byref(cnt), _p(out),
+                                          _p(perm)), "rpd_compact_used")
+        return out[:cnt.value].copy(), perm[:cnt.value].copy()
 
-def params(K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, apex_mode=1, security_mode=0, cull=1):
+
+def params(K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, apex_mode=1, security_mode=0, cull=1,
+           H=0, HC=1):
     return Params(int(K), int(P), int(V), int(T), int(precision), int(max_extra_steps), int(apex_mode),
-                  int(security_mode), int(cull))
+                  int(security_mode), int(cull), int(H), int(HC))
 
 
 class Server:
@@ -476,7 +489,7 @@ class VoroAlgo:
         self._dom = None
         self.m_K, self.m_P, self.m_V, self.m_T = 25, 35, 40, 100
         self.m_keepVolume = 1.0
-        self.m_keepDistance = 0
+        self.m_keepDistance = 1  # voronoi.cpp:1052: one ring of bordering cells
         self.precision = PREC_MIXED
         self.max_extra_steps = 5
         self.apex_mode = 1
@@ -506,7 +519,7 @@ class VoroAlgo:
             self._bary = self._ctx.moments()
             self._perm = np.arange(self._bary.shape[0], dtype=np.int32)
         else:
-            self._bary, self._perm = self._ctx.compact(self.m_keepVolume)
+            self._bary, self._perm = self._ctx.compact_used(self.m_keepVolume, self.m_keepDistance)
         self._hess = self._hess_ids = None
         if self.m_H > 0:
             # compact_result (voronoi.cpp:594-631): the output cells' entries,

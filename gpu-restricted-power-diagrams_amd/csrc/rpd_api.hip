@@ -87,6 +87,10 @@ struct rpd_ctx {
     // Hessian of the last solve (H > 0): n*H*HC values, n*H neighbours
     DevBuf hess, hids;
     int hess_H = 0, hess_HC = 1;
+    // used-cell marks (per stored id) and the scratch outputs of the
+    // bordering passes; caps and stop test of the last solve
+    DevBuf used, bout, bstatus;
+    int last_K = 0, last_P = 0, last_V = 0, last_sm = 0, last_steps = 0;
     // pass work buffers: up to three passes may run at once, one per stream
     PassBufs pb[4];
     // pass 1 writes lists 0 (bisector kernel) and 1 (finish kernel); the
@@ -205,6 +209,48 @@ __global__ __launch_bounds__(256) void status_hist_kernel(const int *__restrict_
     }
     __syncthreads();
     if (threadIdx.x < RPD_STATUS_NUM && hist[threadIdx.x]) atomicAdd(&out[threadIdx.x], hist[threadIdx.x]);
+}
+
+// used[s] = 1 for the cells whose restricted volume reaches keep
+// (ConvexCell.cl:1118-1121), else 0; s is the stored id
+__global__ __launch_bounds__(256) void used_init_kernel(const double4 *__restrict__ out, const int *__restrict__ perm,
+                                                        int64_t n, double keep, int *__restrict__ used) {
+    const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (s < n) used[s] = out[perm[s]].w >= keep ? 1 : 0;
+}
+
+// The cells ring k expands (find_bordering, voronoi.cpp:693-741): ring 0 the
+// used cells, ring k >= 1 the cells marked k whose volume is below keep.
+__global__ __launch_bounds__(256) void ring_select_kernel(const int *__restrict__ used,
+                                                          const double4 *__restrict__ out, const int *__restrict__ perm,
+                                                          int64_t n, double keep, int k, int *__restrict__ ids,
+                                                          int *__restrict__ count) {
+    const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (s >= n) return;
+    const double w = out[perm[s]].w;
+    const bool sel = k == 0 ? (used[s] == 1 && w >= keep) : (used[s] == k && w < keep);
+    if (sel) ids[atomicAdd(count, 1)] = int(s);
+}
+
+// One wave per queued cell record (FP64 planes): the neighbours of the
+// planes of its vertices that carry no mark get d (CCUpdateUsedCell,
+// ConvexCell.cl:680-690).  Every writer of a ring writes the same d.
+__global__ __launch_bounds__(256) void mark_neighbours_kernel(const char *__restrict__ rec, int stride, int P,
+                                                              int S, int nq, int *__restrict__ used, int d) {
+    const int q = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+    const int lane = int(threadIdx.x & 63);
+    if (q >= nq) return;
+    const char *r = rec + size_t(q) * size_t(stride);
+    const int nvalid = reinterpret_cast<const int *>(r)[2];  // RecHeader{sid, nb_plane, nvalid, kused}
+    const uint32_t *tri = reinterpret_cast<const uint32_t *>(r + 16 + size_t(P) * 32);
+    const int *nid = reinterpret_cast<const int *>(tri + 64 * S);
+    for (int j = lane; j < nvalid; j += 64) {
+        const uint32_t t = tri[j];
+        for (int k = 0; k < 3; ++k) {
+            const int nb = nid[(t >> (8 * k)) & 0xff];
+            if (nb >= 0 && used[nb] == 0) used[nb] = d;
+        }
+    }
 }
 
 int ensure_binning(rpd_ctx *c, int prec) {
@@ -1003,6 +1049,11 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     st.final_T = T;
     st.dim = c->dim;
     st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    c->last_K = K;
+    c->last_P = P;
+    c->last_V = V;
+    c->last_sm = prm->security_mode;
+    c->last_steps = prm->max_extra_steps;
     c->solved = true;
     if (stats) *stats = st;
     return RPD_OK;
@@ -1073,6 +1124,122 @@ int rpd_compact(rpd_ctx *c, double keep_volume, int64_t *out_count, double *out4
         if (!(m[4 * i + 3] >= keep_volume)) continue;
         if (out4)
             for (int j = 0; j < 4; ++j) out4[4 * k + j] = m[4 * i + j];
+        if (perm_to_user) perm_to_user[k] = int32_t(i);
+        ++k;
+    }
+    *out_count = k;
+    return RPD_OK;
+}
+
+int rpd_compact_used(rpd_ctx *c, double keep_volume, int32_t keep_distance, int64_t *out_count, double *out4,
+                     int32_t *perm_to_user) {
+    if (!c || !out_count) return set_err(c, RPD_E_ARG, "rpd_compact_used");
+    if (!c->solved) return set_err(c, RPD_E_STATE, "rpd_compact_used: no solve");
+    if (keep_distance > 254) return set_err(c, RPD_E_ARG, "rpd_compact_used: keep_distance > 254");
+    hipSetDevice(c->device);
+    const int64_t n = c->n;
+    std::vector<double> m(size_t(n) * 4);
+    int r = rpd_read_moments(c, m.data());
+    if (r != RPD_OK) return r;
+    if (keep_distance < 0) {  // every cell, in the per-seed layout
+        for (int64_t i = 0; i < n; ++i) {
+            if (out4)
+                for (int j = 0; j < 4; ++j) out4[4 * i + j] = m[4 * i + j];
+            if (perm_to_user) perm_to_user[i] = int32_t(i);
+        }
+        *out_count = n;
+        return RPD_OK;
+    }
+    hipStream_t s = c->stream;
+    RPDCHK(c, c->used.ensure(size_t(n) * 4), "alloc used");
+    RPDCHK(c, c->ids1.ensure(size_t(n) * 4), "alloc ids");
+    const unsigned nb = unsigned((n + 255) / 256);
+    used_init_kernel<<<nb, 256, 0, s>>>(c->out.as<double4>(), c->perm.as<int>(), n, keep_volume, c->used.as<int>());
+    if (keep_distance > 0) {
+        RPDCHK(c, c->bout.ensure(size_t(n) * 32), "alloc bordering outputs");
+        RPDCHK(c, c->bstatus.ensure(size_t(n) * 4), "alloc bordering outputs");
+    }
+    for (int k = 0; k < keep_distance; ++k) {
+        // the cells of ring k, then their unrestricted cells (FP64, no cull:
+        // compute_bordering, ConvexCell.cl:754-807) mark their neighbours
+        HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, sizeof(int), s));
+        ring_select_kernel<<<nb, 256, 0, s>>>(c->used.as<int>(), c->out.as<double4>(), c->perm.as<int>(), n,
+                                              keep_volume, k, c->ids1.as<int>(), c->cnt.as<int>());
+        int cnt = 0;
+        HIPCHK(c, hipMemcpyAsync(&cnt, c->cnt.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        if (cnt <= 0) break;
+        int K = std::min<int64_t>(std::max(c->last_K, 1), n - 1), P = c->last_P, V = c->last_V;
+        for (int step = 0; cnt > 0 && step <= std::max(c->last_steps, 0); ++step) {
+            PassBufs &b = c->pb[0];
+            const size_t stride = rpd_clip_record_bytes(RPD_PREC_FP64, P, V);
+            RPDCHK(c, b.knn.ensure(size_t(cnt) * K * 4), "alloc knn");
+            RPDCHK(c, b.rec.ensure(size_t(cnt) * stride), "alloc records");
+            RPDCHK(c, b.qcount.ensure(sizeof(int)), "alloc qcount");
+            HIPCHK(c, hipMemsetAsync(c->fl[0].cnt.ptr, 0, kFailCounters * sizeof(int), s));
+            RPDCHK(c,
+                   rpd_launch_knn(s, c->bin_prec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim,
+                                  c->ids1.as<int>(), cnt, K, b.knn.as<int>()),
+                   "bordering knn");
+            ClipLaunch L{};
+            L.seeds = c->stored64.as<double4>();
+            L.perm = c->perm.as<int>();
+            L.ids = c->ids1.as<int>();
+            L.knn = b.knn.as<int>();
+            L.n_ids = cnt;
+            L.n_seeds = n;
+            L.K = K;
+            L.P = P;
+            L.V = V;
+            L.T = 1;
+            L.apex_mode = 1;
+            L.security_mode = c->last_sm;
+            L.wmin = c->wmin_ok && c->wmin < INFINITY ? c->wmin : 0.0;
+            L.cull = false;
+            L.dom = nullptr;
+            L.out = c->bout.as<double4>();
+            L.status = c->bstatus.as<int>();
+            L.rec = b.rec.ptr;
+            L.qcount = b.qcount.as<int>();
+            L.fl_cnt = c->fl[0].cnt.as<int>();
+            L.fl_eo = c->fl[0].eo.as<int>() + n;
+            L.fl_sec = c->fl[0].sec.as<int>();
+            L.fl_cnt_host = c->pin + kPinCounters;
+            RPDCHK(c, rpd_launch_cells(s, RPD_PREC_FP64, L, c->pin + 0, true), "bordering cells");
+            const int nq = c->pin[0];
+            const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
+            if (nq > 0)
+                mark_neighbours_kernel<<<unsigned((nq + 3) / 4), 256, 0, s>>>(
+                    static_cast<const char *>(b.rec.ptr), int(stride), P, S, nq, c->used.as<int>(), k + 1);
+            // cells that overflowed or ran out of neighbours: again with grown caps
+            const int *h = c->pin + kPinCounters + 2;
+            const int n_over = c->pin[kPinCounters + 1];
+            if (n_over <= 0) break;
+            if (h[RPD_VERTEX_OVERFLOW]) V = std::min(256, int(V * 1.5));
+            if (h[RPD_PLANE_OVERFLOW]) P = std::min(255, int(P * 1.5));
+            if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) K = int(std::min<int64_t>(n - 1, std::min(256, 2 * K)));
+            HIPCHK(c, hipMemcpyAsync(c->ids1.ptr, c->fl[0].eo.as<int>() + n, size_t(n_over) * 4,
+                                     hipMemcpyDeviceToDevice, s));
+            cnt = n_over;
+        }
+        HIPCHK(c, hipStreamSynchronize(s));
+    }
+    // compaction (compute_permutation / compact_result, voronoi.cpp:513-691) in
+    // ascending original index; a cell of the last ring without volume is
+    // flagged with w = NO_CELL_LIMITS
+    std::vector<int> used(static_cast<size_t>(n)), perm(static_cast<size_t>(n));
+    HIPCHK(c, hipMemcpyAsync(used.data(), c->used.ptr, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(perm.data(), c->perm.ptr, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    std::vector<int> mark(static_cast<size_t>(n), 0);
+    for (int64_t sid = 0; sid < n; ++sid) mark[size_t(perm[size_t(sid)])] = used[size_t(sid)];
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (mark[size_t(i)] == 0) continue;
+        if (out4) {
+            for (int j = 0; j < 4; ++j) out4[4 * k + j] = m[4 * i + j];
+            if (m[4 * i + 3] < keep_volume && mark[size_t(i)] == keep_distance) out4[4 * k + 3] = RPD_NO_CELL_LIMITS;
+        }
         if (perm_to_user) perm_to_user[k] = int32_t(i);
         ++k;
     }

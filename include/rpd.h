@@ -236,6 +236,17 @@ int rpd_knn(rpd_ctx *ctx, int32_t K, int32_t knn_precision, int32_t *out_nk);
    out_count receives the number of used cells; out4 (4*count) and
    out_perm_to_user (count) may be NULL to query the count. */
 int rpd_compact(rpd_ctx *ctx, double keep_volume, int64_t *out_count, double *out4, int32_t *out_perm_to_user);
+/* Replaces compressResult with used cells and bordering rings (voronoi.cpp:
+   513-691, 693-751, 1381-1393; CCUpdateUsedCell and compute_bordering,
+   ConvexCell.cl:680-690, 754-807; VoroAlgo::m_keepDistance): the used cells
+   (m >= keep_volume), the cells sharing a face of their unrestricted power
+   cell (keep_distance >= 1), and so on for keep_distance rings; a cell of the
+   last ring whose volume is below keep_volume is flagged with w =
+   RPD_NO_CELL_LIMITS (a wall for the next solve).  keep_distance 0: the used
+   cells only (as rpd_compact); < 0: every cell.  Ascending original index;
+   out4 and out_perm_to_user may be NULL to query the count. */
+int rpd_compact_used(rpd_ctx *ctx, double keep_volume, int32_t keep_distance, int64_t *out_count, double *out4,
+                     int32_t *out_perm_to_user);
 
 /* ---- serving --------------------------------------------------------------
    A request loop for streams of solves with different seed counts and

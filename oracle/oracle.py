@@ -88,6 +88,8 @@ def lib():
         L.orc_solve_hessian.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcGrid),
                                         ctypes.POINTER(OrcParams), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.POINTER(OrcStats), ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_cell_neighbours.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
         L.orc_solve_ids.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                     ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcParams), ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(OrcStats)]
@@ -199,6 +201,53 @@ def solve_hessian(xyzw, grid=None, H=32, HC=1, **kw):
     if r < 0:
         raise RuntimeError("orc_solve_hessian failed %d" % r)
     return out, status, kused, hess, ids, st.as_dict()
+
+
+def cell_neighbours(xyzw, ids, K=64, P=60, V=100, security_mode=0, max_nb=128):
+    """Face neighbours of the unrestricted power cells of the seeds `ids`
+    (original indices, -1 padded): [len(ids), max_nb]."""
+    xyzw = np.ascontiguousarray(xyzw, dtype=np.float64).reshape(-1, 4)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    out = np.full((ids.size, max_nb), -1, np.int32)
+    if ids.size == 0:
+        return out
+    r = lib().orc_cell_neighbours(_ptr(xyzw), xyzw.shape[0], _ptr(ids), ids.size, int(K), int(P), int(V),
+                                  int(security_mode), _ptr(out), int(max_nb))
+    if r < 0:
+        raise RuntimeError("orc_cell_neighbours failed %d" % r)
+    return out
+
+
+NO_CELL_LIMITS = -100000.0
+
+
+def compact_used(xyzw, mg, keep_volume, keep_distance, **kw):
+    """Used cells and bordering rings (compressResult with m_keepDistance,
+    voronoi.cpp:513-691, 693-751, 1381-1393): the cells with m >= keep_volume,
+    then keep_distance rings of face neighbours; ring k expands the cells of
+    ring k whose volume is below keep_volume.  A cell of the last ring without
+    volume gets w = NO_CELL_LIMITS.  Returns (out[count, 4], ids[count]) in
+    ascending original index."""
+    n = mg.shape[0]
+    if keep_distance < 0:
+        return mg.copy(), np.arange(n, dtype=np.int32)
+    used = (mg[:, 3] >= keep_volume).astype(np.int32)
+    for k in range(keep_distance):
+        if k == 0:
+            sel = np.nonzero((used == 1) & (mg[:, 3] >= keep_volume))[0]
+        else:
+            sel = np.nonzero((used == k) & (mg[:, 3] < keep_volume))[0]
+        if sel.size == 0:
+            break
+        nb = cell_neighbours(xyzw, sel, **kw)
+        nb = np.unique(nb[nb >= 0])
+        nb = nb[used[nb] == 0]
+        used[nb] = k + 1
+    keep = np.nonzero(used != 0)[0].astype(np.int32)
+    out = mg[keep].copy()
+    flag = (mg[keep, 3] < keep_volume) & (used[keep] == keep_distance)
+    out[flag, 3] = NO_CELL_LIMITS
+    return out, keep
 
 
 def solve_ids(xyzw, ids, grid=None, **kw):

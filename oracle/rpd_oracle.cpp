@@ -536,7 +536,7 @@ template <class R>
 CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, int P, int V, int T, int vhs,
                            const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
                            R4<R> *out, Counters *ctr, bool all_listed, double cull_r2 = 0, int H = 0, int HC = 1,
-                           R *hess = nullptr, int *hids = nullptr) {
+                           R *hess = nullptr, int *hids = nullptr, std::vector<int> *face_nb = nullptr) {
     CellResult<R> res;
     // Hessian (CHESSIAN): per plane the accumulated area (HC = 1) or
     // area-weighted face centroid and area (HC = 4) over the pieces of the
@@ -594,6 +594,17 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
     if (!complete && !cc.radius_reached(pts[neigs[K - 1]])) {
         res.status = kSecurityRadiusNotReached;
         return res;
+    }
+    if (face_nb) {
+        // the seeds whose bisector carries a vertex of the (unrestricted)
+        // cell: CCUpdateUsedCell, ConvexCell.cl:680-690
+        for (int ve = 0; ve < cc.nb_vertex; ++ve)
+            for (int i = 0; i < 3; ++i) {
+                const int nb = cc.pnb[cc.verts[ve][i]];
+                if (nb >= 0) face_nb->push_back(nb);
+            }
+        std::sort(face_nb->begin(), face_nb->end());
+        face_nb->erase(std::unique(face_nb->begin(), face_nb->end()), face_nb->end());
     }
     bool add_volume = true;
     R4<R> Pt0 = mk4<R>(0, 0, 0, 1);
@@ -1287,6 +1298,70 @@ int orc_solve_hessian(const double *xyzw, int n, const orc_grid *grid, const orc
                ? solve_impl<float>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, nullptr, 0, out_hess, out_hids)
                : solve_impl<double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, nullptr, 0, out_hess,
                                     out_hids);
+}
+
+// Face neighbours of the unrestricted power cells of the seeds ids_orig
+// (FP64; caps grown x2 / x1.5 until the cell succeeds, at most 6 times):
+// out_nb[i * max_nb ...] their original indices, ascending, -1 padded.
+// Returns 0, or -1 (bad arguments), -2 (a cell failed), -3 (more than max_nb).
+int orc_cell_neighbours(const double *xyzw, int n, const int *ids_orig, int n_ids, int K, int P, int V,
+                        int security_mode, int *out_nb, int max_nb) {
+    if (n <= 1 || !xyzw || !ids_orig || !out_nb || max_nb <= 0 || K <= 0) return -1;
+    const std::vector<R4<double>> pts = convert_pts<double>(xyzw, n);
+    const KnnGrid<double> kg = build_grid(pts);
+    double wmin = std::numeric_limits<double>::infinity();
+    for (const auto &p : pts)
+        if (double(p.w) > kNoCellLimits) wmin = std::min(wmin, double(p.w));
+    if (!(wmin < std::numeric_limits<double>::infinity())) wmin = 0;
+    std::vector<int> inv(static_cast<size_t>(n));
+    for (int s = 0; s < n; ++s) inv[size_t(kg.perm[size_t(s)])] = s;
+    int rc = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n_ids; ++i) {
+        const int s = inv[size_t(ids_orig[i])];
+        int k = std::min(K, n - 1), p = P, v = V;
+        std::vector<int> nb, row, heap;
+        Counters ctr;
+        bool ok = false;
+        for (int step = 0; step < 7 && !ok; ++step) {
+            row.assign(size_t(k), -1);
+            std::vector<KnnKey> hp;
+            knn_query_grid(kg, s, k, row.data(), hp);
+            R4<double> out;
+            nb.clear();
+            const CellResult<double> r =
+                compute_cell<double>(kg.stored.data(), s, row.data(), k, p, v, 0, 2, nullptr, 1, false, security_mode,
+                                     wmin, &out, &ctr, k >= n - 1, 0.0, 0, 1, nullptr, nullptr, &nb);
+            if (r.status == kSuccess || r.status == kEmptyCell) {  // an empty power cell has no neighbour
+                if (r.status == kEmptyCell) nb.clear();
+                ok = true;
+            } else {
+                if (r.status == kSecurityRadiusNotReached) k = std::min(n - 1, 2 * k);
+                if (r.status == kPlaneOverflow) p = std::min(255, p * 3 / 2);
+                if (r.status == kVertexOverflow) v = std::min(255, v * 3 / 2);
+                if (r.status != kSecurityRadiusNotReached && r.status != kPlaneOverflow &&
+                    r.status != kVertexOverflow)
+                    break;
+            }
+        }
+        int *o = out_nb + size_t(i) * max_nb;
+        for (int j = 0; j < max_nb; ++j) o[j] = -1;
+        if (!ok) {
+#pragma omp critical
+            rc = -2;
+            continue;
+        }
+        std::vector<int> orig;
+        for (int z : nb) orig.push_back(kg.perm[size_t(z)]);
+        std::sort(orig.begin(), orig.end());
+        if (int(orig.size()) > max_nb) {
+#pragma omp critical
+            rc = -3;
+            continue;
+        }
+        for (size_t j = 0; j < orig.size(); ++j) o[j] = orig[j];
+    }
+    return rc;
 }
 
 int orc_solve_ids(const double *xyzw, int n, const int *ids_orig, int n_ids, const orc_grid *grid,

@@ -99,6 +99,12 @@ int orc_solve_hessian(const double *xyzw, int n, const orc_grid *grid,
 /* Same as orc_solve for the cells of a subset of seeds only (ids_orig:
    original indices); every seed still acts as a neighbour.  out_mg, out_status
    and out_kused are indexed like ids_orig.  Used for bounded CPU timings. */
+/* Face neighbours (original indices, ascending, -1 padded to max_nb) of the
+   unrestricted FP64 power cells of the seeds ids_orig: the seeds whose
+   bisector carries a vertex (CCUpdateUsedCell, ConvexCell.cl:680-690).
+   0, or -1 bad arguments, -2 a cell failed, -3 more than max_nb. */
+int orc_cell_neighbours(const double *xyzw, int n, const int *ids_orig, int n_ids, int K, int P, int V,
+                        int security_mode, int *out_nb, int max_nb);
 int orc_solve_ids(const double *xyzw, int n, const int *ids_orig, int n_ids,
                   const orc_grid *grid, const orc_params *prm, double *out_mg,
                   int *out_status, int *out_kused, orc_stats *st);
