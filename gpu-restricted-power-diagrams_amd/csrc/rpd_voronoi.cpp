@@ -55,7 +55,8 @@ void usage(const char *prog) {
               << "\t--HC=C: set the number of Hessian components(1 or 4)[default 1]\n"
               << "\t--hessian: compute the hessian value\n"
               << "\t--precision=fp64|mixed|fp32: clip precision form[default mixed]\n"
-              << "\t--output=xyz|mg|volume: outputs (gridGPU.xyz, gridGPU_mg.xyzw, volumes on stderr)\n"
+              << "\t--output=xyz|mg|geogram|volume: outputs (gridGPU.xyz, gridGPU_mg.xyzw,\n"
+              << "\t    gridGPU.geogram_ascii, volumes on stderr)\n"
               << "\t--printInvariants: displays the final volume and barycenter\n"
               << "\t--debug: print per-pass statistics\n";
 }
@@ -259,6 +260,17 @@ int main(int argc, char **argv) {
                 if (keep(i))
                     f << bary[4 * i] / bary[4 * i + 3] << "  " << bary[4 * i + 1] / bary[4 * i + 3] << "  "
                       << bary[4 * i + 2] / bary[4 * i + 3] << "\n";
+        } else if (s == "geogram") {
+            // Utils::dropXYZGeogram with every cell and its w (Utils.cpp:148-170,
+            // test_voronoi.cpp:162-167): ascii geogram vertices, one number a line
+            std::ofstream f("gridGPU.geogram_ascii");
+            f << std::setprecision(17);
+            f << "[HEAD]\n\"GEOGRAM\"\n\"1.0\"\n[ATTS]\n\"GEO::Mesh::vertices\"\n" << n << "\n";
+            f << "[ATTR]\n\"GEO::Mesh::vertices\"\n\"point\"\n\"double\"\n8\n3\n";
+            for (size_t i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) f << bary[4 * i + d] / bary[4 * i + 3] << "\n";
+            f << "[ATTR]\n\"GEO::Mesh::vertices\"\n\"w\"\n\"double\"\n8\n1\n";
+            for (size_t i = 0; i < n; ++i) f << bary[4 * i + 3] << "\n";
         } else if (s == "mg") {
             // per original seed (test_voronoi.cpp:186-211)
             std::ofstream f("gridGPU_mg.xyzw");

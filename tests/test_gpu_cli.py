@@ -26,18 +26,25 @@ def test_cli_mg_output_matches_binding(rpd, ctx, tmp_path):
     gfile = str(tmp_path / "sphere.grid")
     grid.save(gfile)
     r = subprocess.run([CLI, "--K=40", "--P=40", "--V=60", "--T=100", "--precision=fp64", "--grid=" + gfile,
-                        "--output=mg", "--output=xyz", "--printInvariants", str(pts)],
+                        "--output=mg", "--output=xyz", "--output=geogram", "--printInvariants", str(pts)],
                        cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert 'GTime["stat"]' in r.stdout
     assert "Sum of volumes" in r.stderr
     mg = np.loadtxt(tmp_path / "gridGPU_mg.xyzw", skiprows=1)
     assert mg.shape == (n, 4)
+    # geogram ascii (Utils.cpp:148-170): n vertices, barycentres then w
+    geo = open(tmp_path / "gridGPU.geogram_ascii").read().split("\n")
+    assert geo[:6] == ["[HEAD]", '"GEOGRAM"', '"1.0"', "[ATTS]", '"GEO::Mesh::vertices"', str(n)]
+    w0 = 12 + 3 * n + 6
+    assert geo[12 + 3 * n:w0] == ["[ATTR]", '"GEO::Mesh::vertices"', '"w"', '"double"', "8", "1"]
+    gw = np.array([float(v) for v in geo[w0:w0 + n]])
     loaded = rpd.Grid.load(gfile)
     dom = ctx.domain(loaded)
     ctx.set_seeds(x)
     ctx.solve(dom, K=40, P=40, V=60, T=100, precision=rpd.PREC_FP64)
     ref = ctx.moments()
+    assert np.allclose(gw, ref[:, 3], rtol=1e-15, atol=0)
     dom.close()
     ref[ref[:, 3] <= 0.001] = 0.0
     d = np.abs(mg - ref)
