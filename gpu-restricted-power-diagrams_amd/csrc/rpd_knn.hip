@@ -494,6 +494,14 @@ int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int
         return RPD_E_DEVICE;
     scatter_cells_kernel<<<nb, tb, 0, st>>>(cell_in, rank, n, counters, perm);
     sort_cells_kernel<<<unsigned((nc + tb - 1) / tb), tb, 0, st>>>(counters, nc, perm, big);
+    auto gather = [&]() {
+        if (prec == 0)
+            gather_seeds_kernel<float><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<float4 *>(stored), stored64);
+        else
+            gather_seeds_kernel<double><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<double4 *>(stored), stored64);
+    };
+    // gathered at once; a crowded cell (rare) sorts and gathers again
+    gather();
     int big_h = 0;
     if (hipMemcpyAsync(&big_h, big, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -503,13 +511,10 @@ int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int
         size_t sb = sort_bytes;
         if (rocprim::radix_sort_pairs(tmp, sb, cell_in, cell_out, rank, perm, n, 0u, end_bit, st) != hipSuccess)
             return RPD_E_DEVICE;
+        gather();
+        if (hipStreamSynchronize(st) != hipSuccess) return RPD_E_DEVICE;
     }
-    if (prec == 0)
-        gather_seeds_kernel<float><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<float4 *>(stored), stored64);
-    else
-        gather_seeds_kernel<double><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<double4 *>(stored), stored64);
-    if (hipGetLastError() != hipSuccess) return RPD_E_DEVICE;
-    return hipStreamSynchronize(st) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }
 
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
