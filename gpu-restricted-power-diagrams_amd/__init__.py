@@ -130,6 +130,8 @@ def lib():
     sig = {
         "rpd_voxel_mask": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, VP, VP]),
         "rpd_grid_from_voxels": (ctypes.c_int, [ctypes.c_int, VP, ctypes.POINTER(VP)]),
+        "rpd_grid_from_mesh": (ctypes.c_int, [ctypes.c_int, VP, ctypes.c_int, VP, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(VP)]),
         "rpd_grid_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(VP)]),
         "rpd_grid_save": (ctypes.c_int, [VP, ctypes.c_char_p]),
         "rpd_grid_view": (ctypes.c_int, [VP, ctypes.POINTER(GridDesc)]),
@@ -188,6 +190,40 @@ def _check(rc, ctx=None, what=""):
 # ---------------------------------------------------------------------------
 # host helpers: domains and seeds
 # ---------------------------------------------------------------------------
+def read_mesh(path):
+    """Points [m, 3] and triangles [t, 3] of an ASCII .obj ("v x y z", "f a b
+    c ..." with 1-based or negative ids, "a/b/c" forms) or .off file.  Polygons
+    are split into fans about their first vertex, as geogram's
+    facets.triangulate() (main.cpp:165) does."""
+    pts, faces = [], []
+    with open(path) as f:
+        if path.lower().endswith(".off"):
+            toks = [t for line in f for t in line.split("#")[0].split()]
+            if not toks or toks[0] != "OFF":
+                raise RpdError("%s: not an OFF file" % path)
+            nv, nf = int(toks[1]), int(toks[2])
+            k = 4
+            for _ in range(nv):
+                pts.append([float(x) for x in toks[k:k + 3]])
+                k += 3
+            for _ in range(nf):
+                c = int(toks[k])
+                faces.append([int(x) for x in toks[k + 1:k + 1 + c]])
+                k += 1 + c
+        else:
+            for line in f:
+                w = line.split()
+                if not w:
+                    continue
+                if w[0] == "v":
+                    pts.append([float(x) for x in w[1:4]])
+                elif w[0] == "f":
+                    ids = [int(x.split("/")[0]) for x in w[1:]]
+                    faces.append([i - 1 if i > 0 else len(pts) + i for i in ids])
+    tris = [[fc[0], fc[i], fc[i + 1]] for fc in faces for i in range(1, len(fc) - 1)]
+    return np.asarray(pts, np.float64).reshape(-1, 3), np.asarray(tris, np.int32).reshape(-1, 3)
+
+
 class Grid:
     """Host domain grid (the reference's Grid, Grid.h:14-70) owned by librpd."""
 
@@ -219,6 +255,18 @@ class Grid:
         assert g ** 3 == mask.size
         h = ctypes.c_void_p()
         _check(lib().rpd_grid_from_voxels(g, _p(mask), ctypes.byref(h)), what="rpd_grid_from_voxels")
+        return cls(h.value)
+
+    @classmethod
+    def mesh(cls, points, triangles, g, normalize=True):
+        """Grid of a closed triangle mesh built on the GPU (the mesh path of
+        utils/createGrid/main.cpp:150-294): points [m, 3], triangles [t, 3]
+        point ids; normalize scales into [4.998, 995.22]^3 as main.cpp:168-175."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        tri = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        _check(lib().rpd_grid_from_mesh(int(g), _p(pts), pts.shape[0], _p(tri), tri.shape[0], int(bool(normalize)),
+                                        ctypes.byref(h)), what="rpd_grid_from_mesh")
         return cls(h.value)
 
     @classmethod

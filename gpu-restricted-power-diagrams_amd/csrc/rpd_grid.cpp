@@ -20,27 +20,12 @@
 #include "rpd.h"
 #include "rpd_internal.h"
 
-struct rpd_grid {
-    int32_t size = 0;
-    std::vector<int8_t> in_domain;
-    std::vector<double> points;  // xyzw
-    std::vector<int32_t> triangles;
-    std::vector<int32_t> tri_list;
-    std::vector<int32_t> offsets;
-    std::vector<int8_t> centre;  // voxel-centre classes 0 out / 2 in / 1 unknown
-};
-
-namespace {
-// Fractions of a voxel at which the restriction puts its cone apex (the
-// kernels' apex_mode 1); the voxel classes below are those of these points.
-constexpr double kApexFrac[3] = {0.530901699437495, 0.520710678118655, 0.536602540378444};
-
 // Classes of the voxels' apex points for a grid read from a file: parity of
 // the surface crossings of the ray from the point towards +x, row by row from
 // the voxels' triangle lists (a triangle the ray meets lies in the list of a
 // voxel of the row).  A row whose ray passes within rounding of a triangle
 // edge, and a point within rounding of the surface, keep class 1 (unknown).
-void derive_centre_classes(rpd_grid *gr) {
+void rpd_derive_centre_classes(rpd_grid *gr) {
     const int g = gr->size;
     const size_t nv = gr->in_domain.size();
     const double vs = 1000.0 / double(g);
@@ -49,7 +34,7 @@ void derive_centre_classes(rpd_grid *gr) {
     std::vector<double> xs;
     for (int z = 0; z < g; ++z)
         for (int y = 0; y < g; ++y) {
-            const double py = (y + kApexFrac[1]) * vs, pz = (z + kApexFrac[2]) * vs;
+            const double py = (y + kApexFracHost[1]) * vs, pz = (z + kApexFracHost[2]) * vs;
             tris.clear();
             for (int x = 0; x < g; ++x) {
                 const size_t id = size_t(x) + size_t(y) * g + size_t(z) * g * g;
@@ -81,7 +66,7 @@ void derive_centre_classes(rpd_grid *gr) {
             if (degenerate) continue;
             std::sort(xs.begin(), xs.end());
             for (int x = 0; x < g; ++x) {
-                const double px = (x + kApexFrac[0]) * vs;
+                const double px = (x + kApexFracHost[0]) * vs;
                 const auto it = std::upper_bound(xs.begin(), xs.end(), px);
                 const bool near = (it != xs.end() && *it - px < 1e-9 * vs) ||
                                   (it != xs.begin() && px - *(it - 1) < 1e-9 * vs);
@@ -91,7 +76,6 @@ void derive_centre_classes(rpd_grid *gr) {
             }
         }
 }
-}  // namespace
 
 namespace {
 
@@ -432,7 +416,7 @@ extern "C" int rpd_grid_load(const char *path, rpd_grid **out) {
     rpd_grid_view(gr, &d);
     if (rpd_grid_volume(&d) < 0)
         for (size_t t = 0; t + 2 < gr->triangles.size(); t += 3) std::swap(gr->triangles[t], gr->triangles[t + 1]);
-    derive_centre_classes(gr);
+    rpd_derive_centre_classes(gr);
     *out = gr;
     return RPD_OK;
 }

@@ -7,8 +7,28 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "rpd.h"
+
+// Host domain grid behind the rpd_grid handle (the content of Grid,
+// Grid.h:14-70, plus the voxel-centre classes).
+struct rpd_grid {
+    int32_t size = 0;
+    std::vector<int8_t> in_domain;
+    std::vector<double> points;  // xyzw
+    std::vector<int32_t> triangles;
+    std::vector<int32_t> tri_list;
+    std::vector<int32_t> offsets;
+    std::vector<int8_t> centre;  // voxel-centre classes 0 out / 2 in / 1 unknown
+};
+
+// Fractions of a voxel at which the restriction puts its cone apex (the
+// kernels' apex_mode 1); the voxel-centre classes are those of these points.
+constexpr double kApexFracHost[3] = {0.530901699437495, 0.520710678118655, 0.536602540378444};
+
+// Voxel-centre classes of a grid from its triangle lists (rpd_grid.cpp).
+void rpd_derive_centre_classes(rpd_grid *gr);
 
 // Growable device buffer (hipMalloc'ed, freed with the owner).
 struct DevBuf {

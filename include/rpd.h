@@ -97,6 +97,21 @@ int rpd_voxel_mask(int shape, int g, const double *params, uint8_t *out_mask);
    Grid::getVolume() > 0; corner flags 2/0/1 = all/none/some of the 8 incident
    voxels kept; per-voxel sorted lists by exact closed-box overlap. */
 int rpd_grid_from_voxels(int g, const uint8_t *mask, rpd_grid **out);
+/* Grid of a closed triangle mesh, built on the GPU (replaces the mesh path of
+   utils/createGrid/main.cpp:150-294): points (3*num_points xyz) and
+   triangles (3*num_triangles point ids); points no triangle uses are dropped
+   and the rest renumbered in order (main.cpp:164).  normalize != 0 scales
+   the mesh into [4.998, 995.22]^3 (main.cpp:168-175); normalize == 0
+   requires every point in [0,1000]^3.  Per-voxel lists: the voxels of each
+   triangle's bounding range (main.cpp:180-210) that pass the
+   single-precision triangle/box overlap test (geometry.cpp:433-508), in
+   ascending triangle order.  Corner flags: 1 within 1e-3 of the surface
+   (main.cpp:276) or undecided, else 2 inside / 0 outside (the reference's
+   tetrahedralization, main.cpp:240-292, is not used: its 0 and 2 corners
+   get the same flags, some of its 1 corners get 0 or 2).  Triangles are
+   re-oriented as Grid::load does when the volume is negative. */
+int rpd_grid_from_mesh(int g, const double *points, int num_points, const int32_t *triangles, int num_triangles,
+                       int normalize, rpd_grid **out);
 /* Grid::load / Grid::save (Grid.cpp:186-279, 290-344), reference text format. */
 int rpd_grid_load(const char *path, rpd_grid **out);
 int rpd_grid_save(const rpd_grid *grid, const char *path);

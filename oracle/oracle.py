@@ -101,6 +101,19 @@ def lib():
         L.orc_grid_volume.restype = ctypes.c_double
         L.orc_grid_valid.argtypes = [ctypes.POINTER(OrcGrid)]
         L.orc_grid_moments.argtypes = [ctypes.POINTER(OrcGrid), ctypes.c_void_p]
+        L.orc_tri_box_overlap.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p]
+        L.orc_point_triangle_distance.argtypes = [ctypes.c_void_p] * 4
+        L.orc_point_triangle_distance.restype = ctypes.c_double
+        L.orc_winding_number.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.orc_winding_number.restype = ctypes.c_double
+        L.orc_mesh_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_mesh_lists.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_longlong]
+        L.orc_mesh_lists.restype = ctypes.c_longlong
+        L.orc_mesh_corner_flags.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -296,4 +309,60 @@ def grid_moments(grid):
     g = grid.orc()
     out = np.zeros(4, np.float64)
     lib().orc_grid_moments(ctypes.byref(g), _ptr(out))
+    return out
+
+
+# ---- mesh-to-grid generator (utils/createGrid/main.cpp:150-294) ----------
+
+def tri_box_overlap(centre, half, a, b, c):
+    """Single-precision triangle/box overlap of geometry.cpp:433-508."""
+    ctr = np.ascontiguousarray(centre, dtype=np.float32)
+    v = [np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, c)]
+    return bool(lib().orc_tri_box_overlap(_ptr(ctr), ctypes.c_float(half), _ptr(v[0]), _ptr(v[1]), _ptr(v[2])))
+
+
+def point_triangle_distance(p, a, b, c):
+    v = [np.ascontiguousarray(x, dtype=np.float64) for x in (p, a, b, c)]
+    return lib().orc_point_triangle_distance(*[_ptr(x) for x in v])
+
+
+def winding_number(p, xyz, tris):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    return lib().orc_winding_number(_ptr(p), _ptr(xyz), _ptr(tris), tris.size // 3)
+
+
+def mesh_points(points, tris, normalize=True):
+    """Kept points (renumbered) and triangles, normalised as main.cpp:164-175."""
+    points = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1)
+    xyz = np.zeros_like(points)
+    out_t = np.zeros_like(tris)
+    m = lib().orc_mesh_points(_ptr(points), points.shape[0], _ptr(tris), tris.size // 3, int(bool(normalize)),
+                              _ptr(xyz), _ptr(out_t))
+    if m < 0:
+        raise RuntimeError("orc_mesh_points failed %d" % m)
+    return xyz[:m].copy(), out_t
+
+
+def mesh_lists(xyz, tris, g):
+    """Per-voxel triangle lists of main.cpp:178-219: (offsets[g^3+1], list)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    nt = tris.size // 3
+    offs = np.zeros(g ** 3 + 1, np.int32)
+    total = lib().orc_mesh_lists(_ptr(xyz), _ptr(tris), nt, int(g), _ptr(offs), None, 0)
+    lst = np.zeros(max(total, 1), np.int32)
+    lib().orc_mesh_lists(_ptr(xyz), _ptr(tris), nt, int(g), _ptr(offs), _ptr(lst), total)
+    return offs, lst[:total].copy()
+
+
+def mesh_corner_flags(xyz, tris, g, eps=1e-3):
+    """Corner flags by brute force: 1 within eps of the surface, else 2
+    inside / 0 outside by the winding number."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    out = np.zeros(g ** 3, np.int8)
+    lib().orc_mesh_corner_flags(_ptr(xyz), _ptr(tris), tris.size // 3, int(g), float(eps), _ptr(out))
     return out

@@ -1483,3 +1483,223 @@ int orc_grid_valid(const orc_grid *g) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Mesh-to-grid generator (utils/createGrid/main.cpp:150-294).
+// ---------------------------------------------------------------------------
+namespace {
+
+// Single-precision triangle/box overlap, Akenine-Moller's separating axis
+// test as geometry.cpp:433-508 evaluates it: v = float(vertex) - centre,
+// edges e0 = v1-v0, e1 = v2-v1, e2 = v0-v2; nine edge axes with the
+// vertex pairs of the reference's AXISTEST macros, the three box axes, then
+// the triangle plane.  Built with -ffp-contract=off: every product and sum
+// rounds on its own, as in the reference's x86-64 build.
+bool orc_tri_box(const float ctr[3], float h, const double *T0, const double *T1, const double *T2) {
+    float v0[3], v1[3], v2[3];
+    for (int d = 0; d < 3; ++d) {
+        v0[d] = float(T0[d]) - ctr[d];
+        v1[d] = float(T1[d]) - ctr[d];
+        v2[d] = float(T2[d]) - ctr[d];
+    }
+    float e0[3], e1[3], e2[3];
+    for (int d = 0; d < 3; ++d) {
+        e0[d] = v1[d] - v0[d];
+        e1[d] = v2[d] - v1[d];
+        e2[d] = v0[d] - v2[d];
+    }
+    auto sep = [&](float p, float q, float rad) { return std::min(p, q) > rad || std::max(p, q) < -rad; };
+    // X tests: p = a*v[Y] - b*v[Z], a = e[Z], b = e[Y]
+    auto tx = [&](const float *e, const float *va, const float *vb) {
+        const float a = e[2], b = e[1];
+        return sep(a * va[1] - b * va[2], a * vb[1] - b * vb[2], std::fabs(a) * h + std::fabs(b) * h);
+    };
+    // Y tests: p = -a*v[X] + b*v[Z], a = e[Z], b = e[X]
+    auto ty = [&](const float *e, const float *va, const float *vb) {
+        const float a = e[2], b = e[0];
+        return sep(-a * va[0] + b * va[2], -a * vb[0] + b * vb[2], std::fabs(a) * h + std::fabs(b) * h);
+    };
+    // Z tests: p = a*v[X] - b*v[Y], a = e[Y], b = e[X]
+    auto tz = [&](const float *e, const float *va, const float *vb) {
+        const float a = e[1], b = e[0];
+        return sep(a * va[0] - b * va[1], a * vb[0] - b * vb[1], std::fabs(a) * h + std::fabs(b) * h);
+    };
+    if (tx(e0, v0, v2) || ty(e0, v0, v2) || tz(e0, v1, v2)) return false;
+    if (tx(e1, v0, v2) || ty(e1, v0, v2) || tz(e1, v0, v1)) return false;
+    if (tx(e2, v0, v1) || ty(e2, v0, v1) || tz(e2, v1, v2)) return false;
+    for (int d = 0; d < 3; ++d) {
+        const float mn = std::min(v0[d], std::min(v1[d], v2[d]));
+        const float mx = std::max(v0[d], std::max(v1[d], v2[d]));
+        if (mn > h || mx < -h) return false;
+    }
+    const float nx = e0[1] * e1[2] - e0[2] * e1[1];
+    const float ny = e0[2] * e1[0] - e0[0] * e1[2];
+    const float nz = e0[0] * e1[1] - e0[1] * e1[0];
+    const float dpl = -(nx * v0[0] + ny * v0[1] + nz * v0[2]);
+    const float ax = nx > 0.0f ? h : -h, ay = ny > 0.0f ? h : -h, az = nz > 0.0f ? h : -h;
+    if ((nx * -ax + ny * -ay + nz * -az) + dpl > 0.0f) return false;
+    return (nx * ax + ny * ay + nz * az) + dpl >= 0.0f;
+}
+
+// Distance from p to triangle abc: the plane distance when the projection
+// falls inside the triangle (barycentric signs), else the nearest of the
+// three edge segments.
+double orc_point_tri_dist(const double *p, const double *a, const double *b, const double *c) {
+    const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const double v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const double w[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2], vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double uv = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+    const double wu = w[0] * u[0] + w[1] * u[1] + w[2] * u[2], wv = w[0] * v[0] + w[1] * v[1] + w[2] * v[2];
+    const double den = uu * vv - uv * uv;
+    const double s = (vv * wu - uv * wv) / den, t = (uu * wv - uv * wu) / den;
+    if (s >= 0 && t >= 0 && s + t <= 1) {
+        double q[3];
+        for (int d = 0; d < 3; ++d) q[d] = a[d] + s * u[d] + t * v[d] - p[d];
+        return std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    }
+    auto seg = [&](const double *x, const double *y) {
+        const double e[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+        const double f[3] = {p[0] - x[0], p[1] - x[1], p[2] - x[2]};
+        double l = (e[0] * f[0] + e[1] * f[1] + e[2] * f[2]) / (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        l = std::min(1.0, std::max(0.0, l));
+        const double q[3] = {f[0] - l * e[0], f[1] - l * e[1], f[2] - l * e[2]};
+        return std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    };
+    return std::min(seg(a, b), std::min(seg(b, c), seg(c, a)));
+}
+
+// Generalised winding number of a closed triangle mesh at p: the sum of the
+// signed solid angles of its triangles (Van Oosterom & Strackee, IEEE TBME
+// 30(2), 1983) over 4 pi; +-1 inside, 0 outside.
+double orc_winding(const double *p, const double *pts, const int *tris, int nt) {
+    double sum = 0;
+    for (int t = 0; t < nt; ++t) {
+        double r[3][3], l[3];
+        for (int q = 0; q < 3; ++q) {
+            for (int d = 0; d < 3; ++d) r[q][d] = pts[3 * size_t(tris[3 * t + q]) + d] - p[d];
+            l[q] = std::sqrt(r[q][0] * r[q][0] + r[q][1] * r[q][1] + r[q][2] * r[q][2]);
+        }
+        const double det = r[0][0] * (r[1][1] * r[2][2] - r[1][2] * r[2][1]) -
+                           r[0][1] * (r[1][0] * r[2][2] - r[1][2] * r[2][0]) +
+                           r[0][2] * (r[1][0] * r[2][1] - r[1][1] * r[2][0]);
+        auto dt = [&](int i, int j) { return r[i][0] * r[j][0] + r[i][1] * r[j][1] + r[i][2] * r[j][2]; };
+        const double den = l[0] * l[1] * l[2] + dt(0, 1) * l[2] + dt(0, 2) * l[1] + dt(1, 2) * l[0];
+        sum += 2.0 * std::atan2(det, den);
+    }
+    return sum / (4.0 * M_PI);
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_tri_box_overlap(const float centre[3], float half, const double *a, const double *b, const double *c) {
+    return orc_tri_box(centre, half, a, b, c) ? 1 : 0;
+}
+
+double orc_point_triangle_distance(const double *p, const double *a, const double *b, const double *c) {
+    return orc_point_tri_dist(p, a, b, c);
+}
+
+double orc_winding_number(const double *p, const double *pts, const int *tris, int nt) {
+    return orc_winding(p, pts, tris, nt);
+}
+
+int orc_mesh_points(const double *points, int np, const int *tris, int nt, int normalize, double *out_xyz,
+                    int *out_tris) {
+    std::vector<int> remap(size_t(np), -1);
+    for (size_t i = 0; i < 3 * size_t(nt); ++i) {
+        if (tris[i] < 0 || tris[i] >= np) return -1;
+        remap[size_t(tris[i])] = 0;
+    }
+    int m = 0;
+    for (int i = 0; i < np; ++i)
+        if (remap[size_t(i)] == 0) {
+            remap[size_t(i)] = m;
+            for (int d = 0; d < 3; ++d) out_xyz[3 * size_t(m) + d] = points[3 * size_t(i) + d];
+            ++m;
+        }
+    for (size_t i = 0; i < 3 * size_t(nt); ++i) out_tris[i] = remap[size_t(tris[i])];
+    if (normalize) {
+        // main.cpp:168-175: bounding box of the kept points, scale 990.222 /
+        // its largest side, then shift by 4.998
+        double lo[3] = {out_xyz[0], out_xyz[1], out_xyz[2]}, hi[3] = {lo[0], lo[1], lo[2]};
+        for (int i = 0; i < m; ++i)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], out_xyz[3 * size_t(i) + d]);
+                hi[d] = std::max(hi[d], out_xyz[3 * size_t(i) + d]);
+            }
+        const double maxdim = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        if (!(maxdim > 0)) return -1;
+        for (int i = 0; i < m; ++i)
+            for (int d = 0; d < 3; ++d)
+                out_xyz[3 * size_t(i) + d] = (990.222 / maxdim) * (out_xyz[3 * size_t(i) + d] - lo[d]) + 4.998;
+    }
+    return m;
+}
+
+long long orc_mesh_lists(const double *xyz, const int *tris, int nt, int g, int *out_offsets, int *out_list,
+                         long long cap) {
+    // main.cpp:178-219: per triangle, the voxels of the scaled bounding box
+    // range that pass the overlap test; lists in increasing triangle order
+    const long long nv = (long long)g * g * g;
+    std::vector<std::vector<int>> lists(static_cast<size_t>(nv));
+    const float vsf = float(1000. / double(g));
+    const float half = 0.5f * vsf;
+    for (int t = 0; t < nt; ++t) {
+        const double *P[3] = {xyz + 3 * size_t(tris[3 * t]), xyz + 3 * size_t(tris[3 * t + 1]),
+                              xyz + 3 * size_t(tris[3 * t + 2])};
+        long long lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            const double mn = std::min(P[0][d], std::min(P[1][d], P[2][d]));
+            const double mx = std::max(P[0][d], std::max(P[1][d], P[2][d]));
+            lo[d] = (long long)std::floor(double(g) * mn / 1000.);
+            hi[d] = (long long)std::ceil(double(g) * mx / 1000.);
+        }
+        for (long long i = lo[0]; i <= hi[0]; ++i) {
+            if (i >= g) continue;
+            for (long long j = lo[1]; j <= hi[1]; ++j) {
+                if (j >= g) continue;
+                for (long long k = lo[2]; k <= hi[2]; ++k) {
+                    if (k >= g) continue;
+                    const float c[3] = {float(double(vsf) * (0.5 + double(i))), float(double(vsf) * (0.5 + double(j))),
+                                        float(double(vsf) * (0.5 + double(k)))};
+                    if (orc_tri_box(c, half, P[0], P[1], P[2])) lists[size_t(k * g * g + j * g + i)].push_back(t);
+                }
+            }
+        }
+    }
+    long long total = 0;
+    for (const auto &l : lists) total += (long long)l.size();
+    if (total > cap) return total;
+    long long o = 0;
+    for (long long v = 0; v < nv; ++v) {
+        out_offsets[v] = int(o);
+        for (int t : lists[size_t(v)]) out_list[o++] = t;
+    }
+    out_offsets[nv] = int(o);
+    return total;
+}
+
+void orc_mesh_corner_flags(const double *xyz, const int *tris, int nt, int g, double eps, signed char *out_flags) {
+    // the flag rule of the library's mesh grids: 1 within eps of the surface,
+    // else 2 inside / 0 outside, by brute force over every triangle
+    const long long nv = (long long)g * g * g;
+    const double vs = 1000.0 / double(g);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long long id = 0; id < nv; ++id) {
+        const double p[3] = {double(id % g) * vs, double((id / g) % g) * vs, double(id / ((long long)g * g)) * vs};
+        double dmin = 1e300;
+        for (int t = 0; t < nt && dmin > eps; ++t)
+            dmin = std::min(dmin, orc_point_tri_dist(p, xyz + 3 * size_t(tris[3 * t]), xyz + 3 * size_t(tris[3 * t + 1]),
+                                                    xyz + 3 * size_t(tris[3 * t + 2])));
+        if (dmin <= eps) {
+            out_flags[id] = 1;
+            continue;
+        }
+        out_flags[id] = std::fabs(orc_winding(p, xyz, tris, nt)) > 0.5 ? 2 : 0;
+    }
+}
+
+}  // extern "C"

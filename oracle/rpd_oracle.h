@@ -117,6 +117,24 @@ int orc_clip_cell(const double *xyzw, int n, int seed, const int *neigs, int K,
                   int security_mode, double wmin,
                   double *out4, int *nb_vertex, int *nb_plane);
 
+/* Mesh-to-grid generator (utils/createGrid/main.cpp:150-294).
+   orc_mesh_points: drops the points no triangle uses (main.cpp:164), then
+   normalize != 0 scales into [4.998, 995.22]^3 (main.cpp:168-175); writes
+   out_xyz[3*m] and out_tris[3*nt], returns m (<0 bad arguments).
+   orc_mesh_lists: per-voxel lists of main.cpp:178-219 with the overlap test
+   of geometry.cpp:433-508; returns the entry count and writes offsets (g^3+1)
+   and list only when the count fits cap.
+   orc_mesh_corner_flags: corner flags 1 within eps of the surface, else 2
+   inside / 0 outside by the winding number (brute force). */
+int orc_tri_box_overlap(const float centre[3], float half, const double *a, const double *b, const double *c);
+double orc_point_triangle_distance(const double *p, const double *a, const double *b, const double *c);
+double orc_winding_number(const double *p, const double *pts, const int *tris, int nt);
+int orc_mesh_points(const double *points, int np, const int *tris, int nt, int normalize, double *out_xyz,
+                    int *out_tris);
+long long orc_mesh_lists(const double *xyz, const int *tris, int nt, int g, int *out_offsets, int *out_list,
+                         long long cap);
+void orc_mesh_corner_flags(const double *xyz, const int *tris, int nt, int g, double eps, signed char *out_flags);
+
 /* Grid helpers restating Grid.cpp. */
 double orc_grid_volume(const orc_grid *grid);              /* Grid.cpp:30-35 */
 int orc_grid_valid(const orc_grid *grid);                  /* Grid.cpp:60-184 */
