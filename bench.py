@@ -177,6 +177,42 @@ def serving(rpd, ctx, n_req, log):
     return res
 
 
+def mesh_grid_bench(args):
+    """Mesh-to-grid generator (the mesh path of utils/createGrid): a torus
+    mesh R=3 r=1.1 of 800x400 quads (640,000 triangles) voxelized at
+    --mesh-division.  GPU: rpd_grid_from_mesh wall time, host arrays in and a
+    host grid out.  CPU leg: the oracle's per-voxel list loop (the
+    reference's main.cpp:178-219 without its tetrahedralization), one thread,
+    and a bit-exact comparison of the lists."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import __graft_entry__ as ge
+    from meshes import torus
+
+    rpd = ge.load_package()
+    g = args.mesh_division
+    P, T = torus(3.0, 1.1, 800, 400, rot_seed=12)
+    times = []
+    for s in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        grid = rpd.Grid.mesh(P, T, g)
+        dt = time.perf_counter() - t0
+        if s >= args.warmup:
+            times.append(dt)
+        log("bench: mesh grid step %d %.1f ms" % (s, 1e3 * dt))
+    out = {"workload": "mesh-grid", "triangles": int(T.shape[0]), "division": g, "entries": int(grid.tri_list.size),
+           "ms_min": 1e3 * min(times), "ms_median": 1e3 * float(np.median(times)),
+           "corner_flags": {str(k): int(v) for k, v in zip(*np.unique(grid.in_domain, return_counts=True))}}
+    if not args.no_cpu:
+        orc = ge.load_oracle()
+        xyz, tris = orc.mesh_points(P, T)
+        t0 = time.perf_counter()
+        offs, lst = orc.mesh_lists(xyz, tris, g)
+        out["cpu_lists_ms"] = 1e3 * (time.perf_counter() - t0)
+        out["cpu_cores"] = 1
+        out["lists_bit_exact"] = bool(np.array_equal(offs, grid.offsets) and np.array_equal(lst, grid.tri_list))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,7 +226,14 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counter scaling)")
+    ap.add_argument("--mesh-grid", action="store_true",
+                    help="only time the mesh-to-grid generator (torus mesh of 640k triangles, --mesh-division)")
+    ap.add_argument("--mesh-division", type=int, default=256)
     args = ap.parse_args()
+
+    if args.mesh_grid:
+        mesh_grid_bench(args)
+        return
 
     if args.copy_calibration:
         import torch

@@ -14,25 +14,41 @@
 #include <fstream>
 #include <set>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "rpd.h"
 #include "rpd_internal.h"
 
+namespace {
+void derive_centre_slices(rpd_grid *gr, int first, int stride);
+}  // namespace
+
 // Classes of the voxels' apex points for a grid read from a file: parity of
 // the surface crossings of the ray from the point towards +x, row by row from
 // the voxels' triangle lists (a triangle the ray meets lies in the list of a
 // voxel of the row).  A row whose ray passes within rounding of a triangle
 // edge, and a point within rounding of the surface, keep class 1 (unknown).
+// Rows are independent: z-slices are shared out over host threads.
 void rpd_derive_centre_classes(rpd_grid *gr) {
     const int g = gr->size;
     const size_t nv = gr->in_domain.size();
-    const double vs = 1000.0 / double(g);
     gr->centre.assign(nv, 1);
+    const int nthreads = std::max(1, std::min({int(std::thread::hardware_concurrency()), 16, g}));
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nthreads; ++w) pool.emplace_back(derive_centre_slices, gr, w, nthreads);
+    derive_centre_slices(gr, 0, nthreads);
+    for (auto &t : pool) t.join();
+}
+
+namespace {
+void derive_centre_slices(rpd_grid *gr, int first, int stride) {
+    const int g = gr->size;
+    const double vs = 1000.0 / double(g);
     std::vector<int> tris;
     std::vector<double> xs;
-    for (int z = 0; z < g; ++z)
+    for (int z = first; z < g; z += stride)
         for (int y = 0; y < g; ++y) {
             const double py = (y + kApexFracHost[1]) * vs, pz = (z + kApexFracHost[2]) * vs;
             tris.clear();
@@ -76,6 +92,7 @@ void rpd_derive_centre_classes(rpd_grid *gr) {
             }
         }
 }
+}  // namespace
 
 namespace {
 
