@@ -112,7 +112,8 @@ __device__ void tri_range(const double *P, int g, int lo[3], int hi[3]) {
 template <bool EMIT>
 __global__ void __launch_bounds__(256) mesh_sat_kernel(const double *__restrict__ pts, const int *__restrict__ tris,
                                                        int nt, int g, int *__restrict__ count,
-                                                       const int *__restrict__ start, uint64_t *__restrict__ keys,
+                                                       const long long *__restrict__ start,
+                                                       uint64_t *__restrict__ keys,
                                                        int *__restrict__ vcnt) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
@@ -133,7 +134,7 @@ __global__ void __launch_bounds__(256) mesh_sat_kernel(const double *__restrict_
     const size_t base = EMIT ? size_t(start[t]) : 0;
     // slots the count pass reserved (the same test decides both passes;
     // the bound only keeps every write inside the triangle's range)
-    const int room = EMIT ? start[t + 1] - start[t] : 0;
+    const long long room = EMIT ? start[t + 1] - start[t] : 0;
     for (int z = lo[2]; z <= hi[2]; ++z)
         for (int y = lo[1]; y <= hi[1]; ++y)
             for (int x = lo[0]; x <= hi[0]; ++x) {
@@ -346,15 +347,17 @@ extern "C" int rpd_grid_from_mesh(int g, const double *points, int num_points, c
     const int nt = num_triangles;
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return RPD_E_DEVICE;
-    DevBuf d_pts, d_tri, d_cnt, d_keys, d_keys2, d_vcnt, d_list, d_offs, d_tmp, d_centre, d_flags;
+    DevBuf d_pts, d_tri, d_cnt, d_start, d_keys, d_keys2, d_vcnt, d_list, d_offs, d_tmp, d_centre, d_flags;
     auto *gr = new rpd_grid();
     int rc = RPD_OK;
     auto run = [&]() -> int {
         int r;
         if ((r = d_pts.ensure(xyz.size() * 8)) || (r = d_tri.ensure(tri.size() * 4)) ||
-            (r = d_cnt.ensure(2 * (size_t(nt) + 1) * 4)) || (r = d_vcnt.ensure((nv + 1) * 4)))
+            (r = d_cnt.ensure((size_t(nt) + 1) * 4)) || (r = d_start.ensure((size_t(nt) + 1) * 8)) ||
+            (r = d_vcnt.ensure((nv + 1) * 4)))
             return r;
-        int *cnt = d_cnt.as<int>(), *start = cnt + nt + 1;
+        int *cnt = d_cnt.as<int>();
+        long long *start = d_start.as<long long>();
         if ((r = hip_ok(hipMemcpyAsync(d_pts.ptr, xyz.data(), xyz.size() * 8, hipMemcpyHostToDevice, st))) ||
             (r = hip_ok(hipMemcpyAsync(d_tri.ptr, tri.data(), tri.size() * 4, hipMemcpyHostToDevice, st))) ||
             (r = hip_ok(hipMemsetAsync(cnt, 0, (size_t(nt) + 1) * 4, st))) ||
@@ -364,19 +367,19 @@ extern "C" int rpd_grid_from_mesh(int g, const double *points, int num_points, c
         mesh_sat_kernel<false><<<(nt + tb - 1) / tb, tb, 0, st>>>(d_pts.as<double>(), d_tri.as<int>(), nt, g, cnt,
                                                                    nullptr, nullptr, nullptr);
         size_t scan_bytes = 0;
-        if ((r = hip_ok(rocprim::exclusive_scan(nullptr, scan_bytes, cnt, start, 0, size_t(nt) + 1,
-                                                rocprim::plus<int>(), st))) ||
+        // 64-bit starts: the entry total is checked against the int32
+        // offsets of the grid format below, not left to wrap
+        if ((r = hip_ok(rocprim::exclusive_scan(nullptr, scan_bytes, cnt, start, 0LL, size_t(nt) + 1,
+                                                rocprim::plus<long long>(), st))) ||
             (r = d_tmp.ensure(scan_bytes)) ||
-            (r = hip_ok(rocprim::exclusive_scan(d_tmp.ptr, scan_bytes, cnt, start, 0, size_t(nt) + 1,
-                                                rocprim::plus<int>(), st))))
+            (r = hip_ok(rocprim::exclusive_scan(d_tmp.ptr, scan_bytes, cnt, start, 0LL, size_t(nt) + 1,
+                                                rocprim::plus<long long>(), st))))
             return r;
-        int total = 0;
-        if ((r = hip_ok(hipMemcpyAsync(&total, start + nt, 4, hipMemcpyDeviceToHost, st))) ||
+        long long total = 0;
+        if ((r = hip_ok(hipMemcpyAsync(&total, start + nt, 8, hipMemcpyDeviceToHost, st))) ||
             (r = hip_ok(hipStreamSynchronize(st))))
             return r;
-        // the count pass sums per-triangle counts in int: a mesh whose
-        // lists exceed 2^31 entries would have wrapped
-        if (total < 0) return RPD_E_LIMIT;
+        if (total < 0 || total > 0x7fffffffLL) return RPD_E_LIMIT;
         const size_t m = size_t(total);
         unsigned vbits = 1;
         while ((size_t(1) << vbits) < nv) ++vbits;
