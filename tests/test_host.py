@@ -193,3 +193,43 @@ def test_cli_usage_and_errors(tmp_path):
     assert r.returncode == 1 and "--K=k" in r.stderr
     r = subprocess.run([cli, str(tmp_path / "missing.xyz")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "could not load" in r.stderr
+
+
+def test_mesh_grid_argument_checks(rpd):
+    # rejected before any device work: bad ids, non-finite points, a flat
+    # mesh, points outside the frame when not normalised, division limits
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    T = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]], np.int32)
+    cases = [(P, np.array([[0, 1, 4]], np.int32), 8, True),
+             (np.vstack([P[:3], [[np.nan, 0, 0]]]), T, 8, True),
+             (np.zeros((4, 3)), T, 8, True),
+             (P * 2000.0, T, 8, False),
+             (P - 1.0, T, 8, False),
+             (P, T, 0, True),
+             (P, T, 1025, True)]
+    for pts, tri, g, norm in cases:
+        with pytest.raises(rpd.RpdError, match="rpd_grid_from_mesh failed \\(-1\\)"):
+            rpd.Grid.mesh(pts, tri, g, normalize=norm)
+
+
+def test_read_mesh_formats(rpd, tmp_path):
+    obj = tmp_path / "m.obj"
+    obj.write_text("# quad and triangle\nv 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvt 0 0\n"
+                   "f 1/1 2/1 3/1 4/1\nf -4//1 -2//1 -1//1\n")
+    p, t = rpd.read_mesh(str(obj))
+    assert p.shape == (4, 3) and np.array_equal(t, [[0, 1, 2], [0, 2, 3], [0, 2, 3]])
+    off = tmp_path / "m.off"
+    off.write_text("OFF\n# comment\n4 2 0\n0 0 0\n1 0 0\n1 1 0\n0 1 0\n4 0 1 2 3\n3 3 2 1\n")
+    p, t = rpd.read_mesh(str(off))
+    assert p.shape == (4, 3) and np.array_equal(t, [[0, 1, 2], [0, 2, 3], [3, 2, 1]])
+
+
+def test_create_grid_cli_errors(tmp_path):
+    import subprocess
+    exe = os.path.join(ROOT, "gpu-restricted-power-diagrams_amd", "rpd_create_grid")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "meshfile" in r.stdout
+    r = subprocess.run([exe, str(tmp_path / "missing.obj")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "can not read" in r.stderr
+    r = subprocess.run([exe, "x.obj", "--bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "unknown option" in r.stderr
