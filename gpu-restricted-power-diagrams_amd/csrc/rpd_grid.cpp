@@ -154,38 +154,47 @@ extern "C" int rpd_grid_from_voxels(int g, const uint8_t *mask, rpd_grid **out) 
         if (i < 0 || j < 0 || k < 0 || i >= g || j >= g || k >= g) return false;
         return mask[size_t(i) + size_t(j) * g + size_t(k) * g * g] != 0;
     };
-    // corner flags
+    // corner flags, z-slices shared out over host threads
     gr->in_domain.resize(nv);
-    for (int k = 0; k < g; ++k)
-        for (int j = 0; j < g; ++j)
-            for (int i = 0; i < g; ++i) {
-                int c = 0;
-                for (int dz = -1; dz <= 0; ++dz)
-                    for (int dy = -1; dy <= 0; ++dy)
-                        for (int dx = -1; dx <= 0; ++dx) c += kept(i + dx, j + dy, k + dz);
-                gr->in_domain[size_t(i) + size_t(j) * g + size_t(k) * g * g] = int8_t(c == 8 ? 2 : (c == 0 ? 0 : 1));
-            }
+    auto flag_slices = [&](int first, int stride) {
+        for (int k = first; k < g; k += stride)
+            for (int j = 0; j < g; ++j)
+                for (int i = 0; i < g; ++i) {
+                    int c = 0;
+                    for (int dz = -1; dz <= 0; ++dz)
+                        for (int dy = -1; dy <= 0; ++dy)
+                            for (int dx = -1; dx <= 0; ++dx) c += kept(i + dx, j + dy, k + dz);
+                    gr->in_domain[size_t(i) + size_t(j) * g + size_t(k) * g * g] =
+                        int8_t(c == 8 ? 2 : (c == 0 ? 0 : 1));
+                }
+    };
+    {
+        const int nthreads = std::max(1, std::min({int(std::thread::hardware_concurrency()), 16, g}));
+        std::vector<std::thread> pool;
+        for (int w = 1; w < nthreads; ++w) pool.emplace_back(flag_slices, w, nthreads);
+        flag_slices(0, nthreads);
+        for (auto &t : pool) t.join();
+    }
     // lattice points, created on demand
     const double vs = 1000.0 / g;
     const int gp = g + 1;
-    std::unordered_map<long long, int32_t> pid;
+    // dense index of the (g+1)^3 lattice points, -1 until first used: ids
+    // follow the order in which the faces first touch the points
+    std::vector<int32_t> pid(size_t(gp) * gp * gp, -1);
     auto point = [&](int i, int j, int k) -> int32_t {
-        const long long key = (long long)i + (long long)j * gp + (long long)k * gp * gp;
-        auto it = pid.find(key);
-        if (it != pid.end()) return it->second;
-        const int32_t id = int32_t(gr->points.size() / 4);
+        int32_t &id = pid[size_t(i) + size_t(j) * gp + size_t(k) * gp * gp];
+        if (id >= 0) return id;
+        id = int32_t(gr->points.size() / 4);
         gr->points.push_back(i * vs);
         gr->points.push_back(j * vs);
         gr->points.push_back(k * vs);
         gr->points.push_back(0.0);
-        pid.emplace(key, id);
         return id;
     };
     // boundary faces: for each axis a and each lattice plane position, the face
     // between voxel (.., c-1, ..) and (.., c, ..) along a is a boundary face iff
     // exactly one side is kept.  Each face = unit square in the two other axes,
     // split along its diagonal into two triangles.
-    std::vector<std::vector<int32_t>> lists(nv);
     struct Tri {
         int a;        // normal axis
         int c;        // lattice plane index along a
@@ -238,6 +247,9 @@ extern "C" int rpd_grid_from_voxels(int g, const uint8_t *mask, rpd_grid **out) 
     // cut by its diagonal: half 0 keeps t_v <= t_u, half 1 keeps t_v >= t_u
     // (t = local coordinate in [0,1]).  A neighbour square overlaps iff some
     // corner of its intersection with the unit square satisfies that test.
+    // The overlaps are visited twice, in increasing triangle order: counts,
+    // then entries, so each list is sorted.
+    auto visit = [&](auto &&emit) {
     for (size_t t = 0; t < tris.size(); ++t) {
         const Tri &T = tris[t];
         const int b = (T.a + 1) % 3, cax = (T.a + 2) % 3;
@@ -261,20 +273,20 @@ extern "C" int rpd_grid_from_voxels(int g, const uint8_t *mask, rpd_grid **out) 
                     x[T.a] = ia;
                     x[b] = jb;
                     x[cax] = kc;
-                    lists[size_t(x[0]) + size_t(x[1]) * g + size_t(x[2]) * g * g].push_back(int32_t(t));
+                    emit(size_t(x[0]) + size_t(x[1]) * g + size_t(x[2]) * g * g, int32_t(t));
                 }
             }
         }
     }
+    };
     gr->centre.resize(nv);
     for (size_t i = 0; i < nv; ++i) gr->centre[i] = mask[i] ? 2 : 0;
-    gr->offsets.resize(nv + 1);
-    gr->offsets[0] = 0;
-    for (size_t i = 0; i < nv; ++i) {
-        // ids were appended in increasing order, so each list is sorted
-        gr->tri_list.insert(gr->tri_list.end(), lists[i].begin(), lists[i].end());
-        gr->offsets[i + 1] = int32_t(gr->tri_list.size());
-    }
+    gr->offsets.assign(nv + 1, 0);
+    visit([&](size_t v, int32_t) { ++gr->offsets[v + 1]; });
+    for (size_t i = 0; i < nv; ++i) gr->offsets[i + 1] += gr->offsets[i];
+    gr->tri_list.resize(size_t(gr->offsets[nv]));
+    std::vector<int32_t> cursor(gr->offsets.begin(), gr->offsets.end() - 1);
+    visit([&](size_t v, int32_t t) { gr->tri_list[size_t(cursor[v]++)] = t; });
     *out = gr;
     return RPD_OK;
 }
