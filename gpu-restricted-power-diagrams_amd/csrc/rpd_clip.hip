@@ -519,11 +519,12 @@ struct RecHeader {
 // ---------------------------------------------------------------------------
 // Minimum waves per SIMD the bisector kernel is compiled for (one vertex
 // slot per lane): its run time is set by latency, so occupancy pays more than
-// the few registers the compiler then spills (measured at 1M seeds: FP64 5;
-// FP32 forms 8, which they reach without spills).  Wider cells (S > 1,
+// the few registers the compiler then spills (measured at 1M seeds: FP64 7,
+// 72 VGPRs and 12 bytes of spill, 2.10 ms against 2.35 at 5 waves and 2.27 at
+// 8; FP32 forms 8, which they reach without spills).  Wider cells (S > 1,
 // re-queue passes) are left unconstrained.
 template <typename R, int S>
-constexpr int kClipWaves = S > 1 ? 1 : (sizeof(R) == 8 ? 5 : 8);
+constexpr int kClipWaves = S > 1 ? 1 : (sizeof(R) == 8 ? 7 : 8);
 
 template <typename R, typename RP, bool FILTER, int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kClipWaves<R, S>)))
