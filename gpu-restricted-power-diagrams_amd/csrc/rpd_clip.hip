@@ -663,6 +663,18 @@ void clip_cells_kernel(ClipArgs a) {
         if (a.security_mode == 1 || wgap <= 0) return dl * dl > 4.0 * Rs * Rs;
         return dl * (dl - 2.0 * Rs) >= wgap;
     };
+    // The exact stop only ever holds when no later bisector can cut, so the
+    // FP32 radius's margin merely delays it.  The reference stop can hold
+    // while a later bisector of a lighter seed still cuts; there a delayed
+    // stop changes the cell.  So in the mixed form a reference-stop test
+    // within the FP32 radius's margin (2R / rmarg < d <= 2R rmarg) that the
+    // loop reaches ends the cell with needs_exact_predicates: the FP64 pass
+    // decides it, as an uncertain plane-side test.
+    const bool ref_stop = FILTER && a.security_mode == 1;
+    auto stop_near = [&](double dl, double Rm) -> bool {
+        const double Rl = Rm / rmarg;
+        return dl * dl > 4.0 * Rl * Rl && !stop_at(dl, Rm);
+    };
     int i = 0;
     while (i < K) {
         const int base = i & ~63;
@@ -709,6 +721,13 @@ void clip_cells_kernel(ClipArgs a) {
         const int f_end = m_end ? lowest_bit(m_end) : 64;
         const int f_clip = m_clip ? lowest_bit(m_clip) : 64;
         const int f_stop = m_stop ? lowest_bit(m_stop) : 64;
+        if (ref_stop) {
+            const uint64_t m_near = ballot(in && cnb >= 0 && stop_near(cdl, Rcur));
+            if (m_near && lowest_bit(m_near) < min(f_end, min(f_clip, f_stop))) {
+                fail(RPD_NEEDS_EXACT_PREDICATES, base + lowest_bit(m_near) + 1);
+                return;
+            }
+        }
         if (f_end < f_clip && f_end < f_stop) {
             complete = true;  // every other seed has been used
             kused = base + f_end;
@@ -741,9 +760,16 @@ void clip_cells_kernel(ClipArgs a) {
                 return;
             }
         }
-        if (stop_at(rlane(cdl, f_clip), Rcur)) {
-            reached = true;
-            break;
+        {
+            const double dl = rlane(cdl, f_clip);
+            if (ref_stop && stop_near(dl, Rcur)) {
+                fail(RPD_NEEDS_EXACT_PREDICATES, kused);
+                return;
+            }
+            if (stop_at(dl, Rcur)) {
+                reached = true;
+                break;
+            }
         }
         i = base + f_clip + 1;
     }

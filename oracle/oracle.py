@@ -41,7 +41,8 @@ class OrcGrid(ctypes.Structure):
 class OrcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "K", "P", "V", "T", "precision", "vhs0", "vhs1", "apex_mode",
-        "fix_exact_requeue", "max_extra_steps", "threads", "single_pass", "security_mode", "cull", "H", "HC")]
+        "fix_exact_requeue", "max_extra_steps", "threads", "single_pass", "security_mode", "cull", "H", "HC",
+        "frame")]
 
 
 class OrcStats(ctypes.Structure):
@@ -168,11 +169,15 @@ def knn(xyzw, K, precision=1, method=0):
 
 
 def params(K=25, P=35, V=40, T=100, precision=1, vhs0=2, vhs1=2, apex_mode=1, fix_exact_requeue=1,
-           max_extra_steps=5, threads=0, single_pass=0, security_mode=0, cull=0, H=0, HC=1):
+           max_extra_steps=5, threads=0, single_pass=0, security_mode=0, cull=0, H=0, HC=1, frame=0):
+    """Oracle parameters.  precision 0 float build, 1 double (USE_DOUBLE), 2
+    long double (a higher-precision evaluation of the same algorithm); frame 0
+    the reference's [0,1000]^3 frame, 1 each cell in its seed's frame."""
     p = OrcParams()
     for k, v in dict(K=K, P=P, V=V, T=T, precision=precision, vhs0=vhs0, vhs1=vhs1, apex_mode=apex_mode,
                      fix_exact_requeue=fix_exact_requeue, max_extra_steps=max_extra_steps, threads=threads,
-                     single_pass=single_pass, security_mode=security_mode, cull=cull, H=H, HC=HC).items():
+                     single_pass=single_pass, security_mode=security_mode, cull=cull, H=H, HC=HC,
+                     frame=frame).items():
         setattr(p, k, int(v))
     return p
 

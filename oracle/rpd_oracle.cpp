@@ -192,6 +192,10 @@ struct Cell {
     using V3 = std::array<uint8_t, 3>;
     R4<R> seed;
     const R4<R> *pts = nullptr;
+    // Origin of the frame the cell is built in: (0,0,0) for the reference's
+    // [0,1000]^3 frame, the seed for the seed-local frame (w stays 0, so a
+    // shifted point keeps its weight).
+    R4<R> org = R4<R>{0, 0, 0, 0};
     int status = kSuccess;
     int P = 0, V = 0, vhs = 2;
     int security_mode = 1;  // 0 exact power stop, 1 reference stop
@@ -203,18 +207,22 @@ struct Cell {
     uint8_t next[256];
     Counters *ctr = nullptr;
 
-    // CCInit ConvexCell.cl:278-308
-    void init(const R4<R> *p_pts, int id, R cube_eps) {
+    // point vid in the cell's frame
+    R4<R> at(int vid) const { return sub4(pts[vid], org); }
+
+    // CCInit ConvexCell.cl:278-308; local: the frame's origin is the seed
+    void init(const R4<R> *p_pts, int id, R cube_eps, bool local = false) {
         pts = p_pts;
-        seed = p_pts[id];
+        org = local ? mk4<R>(p_pts[id].x, p_pts[id].y, p_pts[id].z, R(0)) : mk4<R>(0, 0, 0, 0);
+        seed = at(id);
         status = kSuccess;
         const R vmin = -cube_eps, vmax = R(1000) + cube_eps;
-        planes[0] = mk4<R>(1, 0, 0, -vmin);
-        planes[1] = mk4<R>(-1, 0, 0, vmax);
-        planes[2] = mk4<R>(0, 1, 0, -vmin);
-        planes[3] = mk4<R>(0, -1, 0, vmax);
-        planes[4] = mk4<R>(0, 0, 1, -vmin);
-        planes[5] = mk4<R>(0, 0, -1, vmax);
+        planes[0] = mk4<R>(1, 0, 0, org.x - vmin);
+        planes[1] = mk4<R>(-1, 0, 0, vmax - org.x);
+        planes[2] = mk4<R>(0, 1, 0, org.y - vmin);
+        planes[3] = mk4<R>(0, -1, 0, vmax - org.y);
+        planes[4] = mk4<R>(0, 0, 1, org.z - vmin);
+        planes[5] = mk4<R>(0, 0, -1, vmax - org.z);
         for (int i = 0; i < 6; ++i) pnb[i] = -1;
         nb_plane = 6;
         static const uint8_t cube[8][3] = {{2, 5, 0}, {5, 3, 0}, {1, 5, 2}, {5, 1, 3},
@@ -230,7 +238,7 @@ struct Cell {
             return -1;
         }
         ctr->n_planes++;
-        const R4<R> B = pts[vid];
+        const R4<R> B = at(vid);
         const R4<R> dir = sub4(seed, B);
         const R4<R> ave2 = add4(seed, B);
         R dt = dot3(ave2, dir) - (B.w - seed.w);
@@ -536,7 +544,8 @@ template <class R>
 CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, int P, int V, int T, int vhs,
                            const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
                            R4<R> *out, Counters *ctr, bool all_listed, double cull_r2 = 0, int H = 0, int HC = 1,
-                           R *hess = nullptr, int *hids = nullptr, std::vector<int> *face_nb = nullptr) {
+                           R *hess = nullptr, int *hids = nullptr, std::vector<int> *face_nb = nullptr,
+                           bool local_frame = false) {
     CellResult<R> res;
     // Hessian (CHESSIAN): per plane the accumulated area (HC = 1) or
     // area-weighted face centroid and area (HC = 4) over the pieces of the
@@ -560,7 +569,8 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
     cc.security_mode = security_mode;
     cc.wmin = wmin;
     cc.ctr = ctr;
-    cc.init(pts, seed, grid && grid->g > 0 ? R(0.1) : R(0));
+    cc.init(pts, seed, grid && grid->g > 0 ? R(0.1) : R(0), local_frame);
+    const R4<R> org = cc.org;
     bool complete = false;  // every other seed consumed
     int i = 0;
     for (; i < K; ++i) {
@@ -582,7 +592,7 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
             res.kused = i + 1;
             return res;
         }
-        if (cc.radius_reached(pts[z])) break;
+        if (cc.radius_reached(cc.at(z))) break;
     }
     res.kused = std::min(i + 1, K);
     if (complete) res.kused = i;
@@ -591,7 +601,7 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
     if (!complete && i == K && all_listed) complete = true;
     res.maxP = cc.nb_plane;
     res.maxV = cc.nb_vertex;
-    if (!complete && !cc.radius_reached(pts[neigs[K - 1]])) {
+    if (!complete && !cc.radius_reached(cc.at(neigs[K - 1]))) {
         res.status = kSecurityRadiusNotReached;
         return res;
     }
@@ -615,7 +625,7 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
         R bmin[3] = {1000, 1000, 1000}, bmax[3] = {0, 0, 0};
         ctr->n_bbox_vertex += cc.nb_vertex;
         for (int ve = 0; ve < cc.nb_vertex; ++ve) {
-            const R4<R> p = cc.vertex_coords(cc.verts[ve], true);
+            const R4<R> p = add4(cc.vertex_coords(cc.verts[ve], true), org);
             const R c[3] = {p.x, p.y, p.z};
             for (int d = 0; d < 3; ++d) {
                 if (bmin[d] > c[d]) bmin[d] = c[d];
@@ -732,6 +742,7 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
             }
         }
         add_volume = in_domain == 2;
+        Pt0 = sub4(Pt0, org);  // the apex in the cell's frame
     }
     if (add_volume) cc.export_bary_volume(out, R(1), vol_eps2, sp, HC);
     ctr->restricting = true;
@@ -745,9 +756,9 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
         std::vector<typename Cell<R>::V3> bk(cc.verts, cc.verts + cc.nb_vertex);
         for (size_t t = 0; t < tr.size(); ++t) {
             const int tid = tr[t];
-            R4<R> P1 = grid->pts[size_t(grid->tris[3 * tid + 0])];
-            R4<R> P2 = grid->pts[size_t(grid->tris[3 * tid + 1])];
-            R4<R> P3 = grid->pts[size_t(grid->tris[3 * tid + 2])];
+            R4<R> P1 = sub4(grid->pts[size_t(grid->tris[3 * tid + 0])], org);
+            R4<R> P2 = sub4(grid->pts[size_t(grid->tris[3 * tid + 1])], org);
+            R4<R> P3 = sub4(grid->pts[size_t(grid->tris[3 * tid + 2])], org);
             const bool go_out = det3r4(sub4(P1, Pt0), sub4(P2, Pt0), sub4(P3, Pt0)) > R(0);
             if (!go_out) std::swap(P2, P3);
             cc.clip_tet(Pt0, P1, P2, P3);
@@ -775,6 +786,18 @@ CellResult<R> compute_cell(const R4<R> *pts, int seed, const int *neigs, int K, 
         }
     }
     res.status = kSuccess;
+    if (local_frame) {
+        // moments about the seed -> moments in the [0,1000]^3 frame
+        out->x += org.x * out->w;
+        out->y += org.y * out->w;
+        out->z += org.z * out->w;
+        if (H > 0 && HC == 4)
+            for (int p = 0; p < cc.nb_plane; ++p) {
+                surf[size_t(4 * p + 0)] += org.x * surf[size_t(4 * p + 3)];
+                surf[size_t(4 * p + 1)] += org.y * surf[size_t(4 * p + 3)];
+                surf[size_t(4 * p + 2)] += org.z * surf[size_t(4 * p + 3)];
+            }
+    }
     if (H > 0) {
         // planes towards a seed whose face keeps more than 0.2 of area, in
         // plane order; more than H of them: hessian_overflow
@@ -1057,7 +1080,8 @@ void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, 
                const GridView<R> *grid, int apex_mode, bool out_domain, int security_mode, double wmin,
                std::vector<R4<R>> &out,
                std::vector<int> &status, std::vector<int> &kused, Counters &ctr, double &t_knn, double &t_clip,
-               const std::vector<double> &cull_r2, int H = 0, int HC = 1, R *hess = nullptr, int *hids = nullptr) {
+               const std::vector<double> &cull_r2, int H = 0, int HC = 1, R *hess = nullptr, int *hids = nullptr,
+               bool local_frame = false) {
     const long long nids = (long long)ids.size();
     if (nids == 0) return;
     std::vector<int> nb(size_t(nids) * size_t(K));
@@ -1076,7 +1100,7 @@ void clip_pass(const KnnGrid<R> &kg, const std::vector<int> &ids, int K, int P, 
                                               K >= int(kg.stored.size()) - 1,
                                               cull_r2.empty() ? 0.0 : cull_r2[size_t(s)], H, HC,
                                               H > 0 ? hess + size_t(s) * H * HC : nullptr,
-                                              H > 0 ? hids + size_t(s) * H : nullptr);
+                                              H > 0 ? hids + size_t(s) * H : nullptr, nullptr, local_frame);
             status[size_t(s)] = r.status;
             kused[size_t(s)] = r.kused;
         }
@@ -1099,7 +1123,10 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
     const KnnGrid<R> kg = build_grid(pts);
     GridView<R> gv = make_grid_view<R>(gr);
     const GridView<R> *grid = gr ? &gv : nullptr;
-    const bool dbl = sizeof(R) == sizeof(double);
+    // double and long double: the USE_DOUBLE pass policy; long double
+    // evaluates the predicates in its own precision (det4 of VHSAlgo 0)
+    const bool dbl = sizeof(R) >= sizeof(double);
+    const bool local = prm->frame == 1;
     const int sm = prm->security_mode;
     double wmin = std::numeric_limits<double>::infinity();
     for (const auto &p : pts)
@@ -1114,7 +1141,7 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
     Counters ctr;
     double t_knn = 0, t_clip = 0;
     int K = prm->K, P = prm->P, V = prm->V, T = prm->T;
-    int vhs = dbl ? 2 : prm->vhs0;
+    int vhs = sizeof(R) > sizeof(double) ? 0 : (dbl ? 2 : prm->vhs0);
     int passes = 0;
     std::vector<int> all;
     if (ids_orig) {
@@ -1166,12 +1193,12 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
 
     if (prm->single_pass) {
         clip_pass(kg, todo, K, P, V, T, vhs, grid, prm->apex_mode, grid && gv.has_border_vertices, sm, wmin, out,
-                  status, kused, ctr, t_knn, t_clip, cull_r2, H, HC, hess.data(), hids.data());
+                  status, kused, ctr, t_knn, t_clip, cull_r2, H, HC, hess.data(), hids.data(), local);
         passes = 1;
     } else {
         // first pass: T = 0 defers every boundary cell (voronoi.cpp:1294-1297)
         clip_pass(kg, todo, K, P, V, 0, vhs, grid, prm->apex_mode, false, sm, wmin, out, status, kused, ctr, t_knn,
-                  t_clip, cull_r2, H, HC, hess.data(), hids.data());
+                  t_clip, cull_r2, H, HC, hess.data(), hids.data(), local);
         passes = 1;
         auto collect = [&](const std::vector<int> &from, auto cond, std::vector<int> &ids0, std::vector<int> &ids1) {
             ids0.clear();
@@ -1186,12 +1213,12 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
         collect(todo, [](int s) { return s == kTriangleOverflow; }, ids0, ids1);
         if (!ids1.empty()) {  // tr_overflow pass (voronoi.cpp:1304-1310)
             clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, false, sm, wmin, out, status, kused, ctr,
-                      t_knn, t_clip, cull_r2, H, HC, hess.data(), hids.data());
+                      t_knn, t_clip, cull_r2, H, HC, hess.data(), hids.data(), local);
             passes++;
         }
         if (!ids0.empty()) {
             // extra passes (voronoi.cpp:1311-1376)
-            vhs = dbl ? 2 : prm->vhs1;
+            vhs = sizeof(R) > sizeof(double) ? 0 : (dbl ? 2 : prm->vhs1);
             const bool out_domain = grid && gv.has_border_vertices;
             std::vector<int> cur = ids0;
             collect(cur, [](int) { return true; }, ids0, ids1);
@@ -1211,7 +1238,7 @@ int solve_impl(const double *xyzw, int n, const orc_grid *gr, const orc_params *
                 if (P >= 256) P = 255;  // uchar plane ids (voronoi.cpp:867-870)
                 K = std::min(K, n - 1 > 0 ? n - 1 : 1);
                 clip_pass(kg, ids1, K, P, V, T, vhs, grid, prm->apex_mode, out_domain, sm, wmin, out, status, kused,
-                          ctr, t_knn, t_clip, cull_r2, H, HC, hess.data(), hids.data());
+                          ctr, t_knn, t_clip, cull_r2, H, HC, hess.data(), hids.data(), local);
                 passes++;
                 cur = ids0;
                 collect(cur, [](int s) { return s != kEmptyCell; }, ids0, ids1);
@@ -1284,6 +1311,7 @@ int orc_solve(const double *xyzw, int n, const orc_grid *grid, const orc_params 
     if (prm->P >= 256 || prm->K <= 0 || prm->V <= 0) return -2;
     orc_params p = *prm;
     p.K = std::min(p.K, n - 1);
+    if (p.precision == 2) return solve_impl<long double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st);
     return p.precision == 0 ? solve_impl<float>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st)
                             : solve_impl<double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st);
 }
@@ -1294,6 +1322,9 @@ int orc_solve_hessian(const double *xyzw, int n, const orc_grid *grid, const orc
     if (prm->P >= 256 || prm->K <= 0 || prm->V <= 0 || prm->H < 0 || (prm->HC != 1 && prm->HC != 4)) return -2;
     orc_params p = *prm;
     p.K = std::min(p.K, n - 1);
+    if (p.precision == 2)
+        return solve_impl<long double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, nullptr, 0, out_hess,
+                                       out_hids);
     return p.precision == 0
                ? solve_impl<float>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, nullptr, 0, out_hess, out_hids)
                : solve_impl<double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, nullptr, 0, out_hess,
@@ -1372,6 +1403,8 @@ int orc_solve_ids(const double *xyzw, int n, const int *ids_orig, int n_ids, con
     if (prm->P >= 256 || prm->K <= 0 || prm->V <= 0) return -2;
     orc_params p = *prm;
     p.K = std::min(p.K, n - 1);
+    if (p.precision == 2)
+        return solve_impl<long double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, ids_orig, n_ids);
     return p.precision == 0
                ? solve_impl<float>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, ids_orig, n_ids)
                : solve_impl<double>(xyzw, n, grid, &p, out_mg, out_status, out_kused, st, ids_orig, n_ids);

@@ -33,7 +33,11 @@ typedef struct {
 
 typedef struct {
     int K, P, V, T;              /* caps of the first pass (voronoi.h:199-205) */
-    int precision;               /* 0: float build, 1: double (USE_DOUBLE)     */
+    int precision;               /* 0: float build, 1: double (USE_DOUBLE),
+                                    2: long double (x87 80-bit, 64-bit
+                                    significand) with the double pass policy
+                                    and its own det4 predicate: a higher-
+                                    precision evaluation of the same cells    */
     int vhs0, vhs1;              /* float build: half-space algo first/extra   */
     int apex_mode;               /* 0: reference corner rule (ConvexCell.cl:950-
                                     1056), 1: centre of the first voxel of the
@@ -55,6 +59,10 @@ typedef struct {
     int H, HC;                   /* Hessian (CHESSIAN): up to H neighbours per
                                     seed, HC = 1 face area, 4 face centroid x
                                     area and area; H = 0: none                 */
+    int frame;                   /* 0: planes, vertices and moments in the
+                                    [0,1000]^3 frame, as the reference; 1: in
+                                    the frame of each cell's seed (the cell is
+                                    translated by -seed, its moments back)     */
 } orc_params;
 
 typedef struct {

@@ -51,3 +51,22 @@ def moment_errors(got, ref, n):
     v = 1e9 / n
     return (float(np.abs(got[:, 3] - ref[:, 3]).max() / v),
             float(np.abs(got[:, :3] - ref[:, :3]).max() / (v * 1000.0)))
+
+
+def cell_errors(got, ref, n, floor=0.01):
+    """Per-result errors over the cells whose reference volume exceeds floor
+    x the mean cell volume 1e9/n: (largest relative volume error, largest
+    barycentre displacement over the cell's size (cube root of its volume),
+    number of cells compared).  Below the floor a cell is a sliver of a
+    restricted cell, whose relative error grows as the absolute rounding of
+    its pieces (a few ulps of the whole cell's volume) over its own volume."""
+    v = 1e9 / n
+    big = ref[:, 3] > floor * v
+    if not big.any():
+        return 0.0, 0.0, 0
+    m = ref[big, 3]
+    rv = float((np.abs(got[big, 3] - m) / m).max())
+    bg = got[big, :3] / got[big, 3:4]
+    br = ref[big, :3] / ref[big, 3:4]
+    rb = float((np.abs(bg - br).max(1) / np.cbrt(m)).max())
+    return rv, rb, int(big.sum())

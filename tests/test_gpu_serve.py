@@ -73,3 +73,38 @@ def test_served_result_matches_oracle(rpd, orc, ctx, setup):
     ref, st, _, _ = orc.solve(x, oracle_grid(orc, grids[d]), K=50, P=32, V=48, T=100)
     ev, em = moment_errors(got, ref, x.shape[0])
     assert ev < 1e-6 and em < 1e-6
+
+
+def test_full_size_domains_alone_equals_in_mix(rpd, orc, ctx):
+    """The serving workload's domains at their real size (4 resident domains
+    at 96^3) with a request of 250,000 seeds among small ones: each result is
+    the same alone and in the mix, bit for bit, and a sample of the large
+    request's cells matches the oracle (mixed form, 1e-6 per cell)."""
+    from conftest import cell_errors
+    grids = [rpd.Grid.shape(code, 96, prm) for code, prm in DOMAINS]
+    doms = [ctx.domain(g) for g in grids]
+    sizes = [250000, 1200, 30000, 4000, 2500, 60000, 1000, 9000]
+    reqs = _requests(rpd, sizes, seed0=300)
+    srv = rpd.Server(ctx, max_inflight=4)
+    tickets = [srv.submit(doms[d], x, **CAPS) for d, x in reqs]
+    got = {}
+    while len(got) < len(tickets):
+        r = srv.poll(wait=True)
+        assert r is not None
+        got[r[0]] = r[1]
+    srv.close()
+    for t, (d, x) in zip(tickets, reqs):
+        ctx.set_seeds(x)
+        ctx.solve(doms[d], **CAPS)
+        assert np.array_equal(got[t], ctx.moments()), "ticket %d differs from its solve alone" % t
+    d, x = reqs[0]
+    n = x.shape[0]
+    ids = np.arange(0, n, 25, dtype=np.int32)
+    ref, _, _, _ = orc.solve_ids(x, ids, oracle_grid(orc, grids[d]), K=50, P=32, V=48, T=100, frame=1)
+    cv, cb, nc = cell_errors(got[tickets[0]][ids], ref, n)
+    ev, em = moment_errors(got[tickets[0]][ids], ref, n)
+    print("96^3 serving: sample of %d cells, per cell %.2e %.2e, of mean %.2e %.2e" % (nc, cv, cb, ev, em),
+          flush=True)
+    assert nc > 1000 and cv <= 1e-6 and cb <= 1e-6 and ev <= 1e-6 and em <= 1e-6
+    for dd in doms:
+        dd.close()

@@ -100,13 +100,19 @@ def test_equal_weights_is_voronoi(rpd, orc):
     assert np.array_equal(a, b)
 
 
+# the oracle's forms: FP64 in the reference's [0,1000]^3 frame, FP64 in each
+# seed's frame, long double in the reference's frame
+FORMS = [(1, 0), (1, 1), (2, 0)]
+
+
+@pytest.mark.parametrize("prec,frame", FORMS)
 @pytest.mark.parametrize("shape,g,params,n,seed", [
     ("sphere", 8, (0.5, 0.5, 0.5, 0.42), 40, 3),
     ("torus", 10, (0.5, 0.5, 0.5, 0.3, 0.15), 36, 7),
     ("cube_minus_sphere", 6, (0.5, 0.5, 0.5, 0.3), 30, 9),
     ("cube", 5, (), 30, 10),
 ])
-def test_restricted_cells_brute_force(rpd, orc, shape, g, params, n, seed):
+def test_restricted_cells_brute_force(rpd, orc, shape, g, params, n, seed, prec, frame):
     """Every restricted cell (volume and first moments) against brute-force
     clipping of the cell by all N-1 bisectors and by every kept voxel."""
     code = {"sphere": rpd.SHAPE_SPHERE, "torus": rpd.SHAPE_TORUS, "cube": rpd.SHAPE_CUBE,
@@ -114,7 +120,8 @@ def test_restricted_cells_brute_force(rpd, orc, shape, g, params, n, seed):
     x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=seed, wmax=0.25 * (1.0 / n) ** (2 / 3) * 1e6)
     grid = rpd.Grid.shape(code, g, params)
     mask = rpd.voxel_mask(code, g, params)
-    out, st, _, _ = orc.solve(x, oracle_grid(orc, grid), K=n + 4, P=200, V=250, T=4000)
+    out, st, _, _ = orc.solve(x, oracle_grid(orc, grid), K=n + 4, P=200, V=250, T=4000, precision=prec,
+                              frame=frame)
     ref = np.array([pt.restricted_cell(pt.power_cell(x, i), mask, g) for i in range(n)])
     assert np.all((st == 4) | (st == 6))
     v = 1e9 / n
@@ -122,10 +129,11 @@ def test_restricted_cells_brute_force(rpd, orc, shape, g, params, n, seed):
     assert np.abs(out[:, :3] - ref[:, :3]).max() / (v * 1000) < 1e-11
 
 
-def test_unrestricted_cells_brute_force(rpd, orc):
+@pytest.mark.parametrize("prec,frame", FORMS)
+def test_unrestricted_cells_brute_force(rpd, orc, prec, frame):
     n = 40
     x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=8, wmax=0.25 * (1.0 / n) ** (2 / 3) * 1e6)
-    out, st, _, _ = orc.solve(x, None, K=n + 4, P=200, V=250)
+    out, st, _, _ = orc.solve(x, None, K=n + 4, P=200, V=250, precision=prec, frame=frame)
     ref = np.array([pt.volume_moments(pt.power_cell(x, i)) for i in range(n)])
     v = 1e9 / n
     assert np.abs(out[:, 3] - ref[:, 3]).max() / v < 1e-12
