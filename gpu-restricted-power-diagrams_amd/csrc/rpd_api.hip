@@ -366,6 +366,10 @@ void rpd_default_params(rpd_params *p) {
 int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     if (!c || !g || !out || g->size <= 0) return set_err(c, RPD_E_ARG, "rpd_domain_create: bad arguments");
     if (g->size > 1024) return set_err(c, RPD_E_LIMIT, "rpd_domain_create: at most 1024 voxels per axis");
+    // the lists are read by index on the host and on the GPU: every range
+    // and id must lie inside its array
+    if (const char *why = rpd_grid_desc_check(g))
+        return set_err(c, RPD_E_ARG, "rpd_domain_create: malformed grid: %s", why);
     hipSetDevice(c->device);
     auto *d = new rpd_domain();
     d->device = c->device;

@@ -202,6 +202,23 @@ extern "C" int rpd_grid_from_voxels(int g, const uint8_t *mask, rpd_grid **out) 
         int half;     // 0: corners (0,0),(1,0),(1,1); 1: (0,0),(1,1),(0,1)
     };
     std::vector<Tri> tris;
+    int64_t nfaces = 0;  // boundary faces, counted first: ids are int32
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c <= g; ++c)
+            for (int v = 0; v < g; ++v)
+                for (int u = 0; u < g; ++u) {
+                    int lo[3], hi[3];
+                    lo[a] = c - 1;
+                    hi[a] = c;
+                    lo[(a + 1) % 3] = hi[(a + 1) % 3] = u;
+                    lo[(a + 2) % 3] = hi[(a + 2) % 3] = v;
+                    nfaces += kept(lo[0], lo[1], lo[2]) != kept(hi[0], hi[1], hi[2]);
+                }
+    if (6 * nfaces > int64_t(INT32_MAX)) {
+        delete gr;
+        return RPD_E_LIMIT;
+    }
+    tris.reserve(size_t(2 * nfaces));
     for (int a = 0; a < 3; ++a) {
         const int b = (a + 1) % 3, cax = (a + 2) % 3;
         for (int c = 0; c <= g; ++c)
@@ -281,9 +298,18 @@ extern "C" int rpd_grid_from_voxels(int g, const uint8_t *mask, rpd_grid **out) 
     };
     gr->centre.resize(nv);
     for (size_t i = 0; i < nv; ++i) gr->centre[i] = mask[i] ? 2 : 0;
+    // list entries are counted in 64 bits: the grid's ids and offsets are
+    // int32 (Grid.h), which a porous mask at g ~ 1000 can exceed
+    std::vector<int64_t> cnt(nv + 1, 0);
+    visit([&](size_t v, int32_t) { ++cnt[v + 1]; });
+    for (size_t i = 0; i < nv; ++i) cnt[i + 1] += cnt[i];
+    if (cnt[nv] > int64_t(INT32_MAX)) {
+        delete gr;
+        return RPD_E_LIMIT;
+    }
     gr->offsets.assign(nv + 1, 0);
-    visit([&](size_t v, int32_t) { ++gr->offsets[v + 1]; });
-    for (size_t i = 0; i < nv; ++i) gr->offsets[i + 1] += gr->offsets[i];
+    for (size_t i = 0; i <= nv; ++i) gr->offsets[i] = int32_t(cnt[i]);
+    std::vector<int64_t>().swap(cnt);
     gr->tri_list.resize(size_t(gr->offsets[nv]));
     std::vector<int32_t> cursor(gr->offsets.begin(), gr->offsets.end() - 1);
     visit([&](size_t v, int32_t t) { gr->tri_list[size_t(cursor[v]++)] = t; });
@@ -328,8 +354,27 @@ extern "C" void rpd_grid_moments(const rpd_grid_desc *g, double out[4]) {
     }
 }
 
+const char *rpd_grid_desc_check(const rpd_grid_desc *g) {
+    if (!g || g->size <= 0) return "no grid";
+    const int64_t nv = int64_t(g->size) * g->size * g->size;
+    if (g->num_points < 0 || g->num_triangles < 0 || g->num_tri_list < 0) return "negative count";
+    if (!g->in_domain || !g->offsets) return "missing flags or offsets";
+    if ((g->num_points > 0 && !g->points) || (g->num_triangles > 0 && !g->triangles) ||
+        (g->num_tri_list > 0 && !g->tri_list))
+        return "missing array";
+    for (int64_t i = 0; i < 3 * int64_t(g->num_triangles); ++i)
+        if (g->triangles[i] < 0 || g->triangles[i] >= g->num_points) return "triangle point id out of range";
+    for (int64_t i = 0; i < g->num_tri_list; ++i)
+        if (g->tri_list[i] < 0 || g->tri_list[i] >= g->num_triangles) return "list entry out of range";
+    if (g->offsets[0] != 0) return "offsets do not start at 0";
+    for (int64_t v = 0; v < nv; ++v)
+        if (g->offsets[v + 1] < g->offsets[v]) return "offsets decrease";
+    if (g->offsets[nv] > g->num_tri_list) return "offsets end past the list";
+    return nullptr;
+}
+
 extern "C" int rpd_grid_valid(const rpd_grid_desc *g) {
-    if (!g || g->size <= 0) return 0;
+    if (rpd_grid_desc_check(g)) return 0;
     const long long nv = (long long)g->size * g->size * g->size;
     for (long long i = 0; i < nv; ++i)
         if (g->in_domain[i] & 0xfc) return 0;
@@ -356,6 +401,21 @@ extern "C" int rpd_grid_valid(const rpd_grid_desc *g) {
             if (!edges.insert({g->triangles[3 * t + j], g->triangles[3 * t + (j + 1) % 3]}).second) return 0;
     for (const auto &e : edges)
         if (!edges.count({e.second, e.first})) return 0;
+    // no voxel whose 8 corners are all on the border (flag 1): its corner
+    // apex could not be placed.  The flag of voxel v is that of its first
+    // corner, so voxel v's corners are the flags of v + {0,1}^3; a voxel on
+    // a max face is skipped, its +1 corners taken to lie outside
+    // (Grid.cpp:147-167, which means this test; its loop counts the 8 voxels
+    // at the grid origin for every border voxel instead of those of v).
+    const long long gs = g->size;
+    for (long long z = 0; z + 1 < gs; ++z)
+        for (long long y = 0; y + 1 < gs; ++y)
+            for (long long x = 0; x + 1 < gs; ++x) {
+                bool all = true;
+                for (int c = 0; c < 8 && all; ++c)
+                    all = g->in_domain[(x + (c & 1)) + (y + ((c >> 1) & 1)) * gs + (z + (c >> 2)) * gs * gs] == 1;
+                if (all) return 0;
+            }
     for (long long o = 0; o < nv; ++o) {
         if (g->offsets[o] > g->offsets[o + 1]) return 0;
         for (int t = g->offsets[o]; t + 1 < g->offsets[o + 1]; ++t)
@@ -435,11 +495,14 @@ extern "C" int rpd_grid_load(const char *path, rpd_grid **out) {
     f >> no;
     if (!f.good() || no <= int(nv)) return fail();
     gr->offsets.resize(size_t(no));
+    int prev = 0;
     for (auto &o : gr->offsets) {
         f >> o;
-        if (o < 0 || o > ni) return fail();
+        // offsets are a non-decreasing prefix into the list entries
+        if (o < prev || o > ni) return fail();
+        prev = o;
     }
-    if (f.fail()) return fail();
+    if (f.fail() || gr->offsets[0] != 0) return fail();
     // orientation normalisation of Grid::load (Grid.cpp:270-275)
     rpd_grid_desc d;
     rpd_grid_view(gr, &d);

@@ -29,6 +29,11 @@ constexpr double kApexFracHost[3] = {0.530901699437495, 0.520710678118655, 0.536
 
 // Voxel-centre classes of a grid from its triangle lists (rpd_grid.cpp).
 void rpd_derive_centre_classes(rpd_grid *gr);
+// Structural check of a grid description (rpd_grid.cpp): array pointers
+// present, counts non-negative, triangles' point ids and list entries'
+// triangle ids in range, offsets starting at 0, non-decreasing and ending at
+// most at num_tri_list.  Returns nullptr when sound, else what is wrong.
+const char *rpd_grid_desc_check(const rpd_grid_desc *g);
 
 // Growable device buffer (hipMalloc'ed, freed with the owner).
 struct DevBuf {

@@ -95,7 +95,9 @@ int rpd_voxel_mask(int shape, int g, const double *params, uint8_t *out_mask);
    voxel domains): boundary faces between kept and non-kept voxels (grid
    exterior counts as not kept), 2 triangles each, oriented so that
    Grid::getVolume() > 0; corner flags 2/0/1 = all/none/some of the 8 incident
-   voxels kept; per-voxel sorted lists by exact closed-box overlap. */
+   voxels kept; per-voxel sorted lists by exact closed-box overlap.
+   RPD_E_LIMIT when the triangle point ids (3 per triangle) or the list
+   entries exceed INT32_MAX (the int32 arrays of Grid). */
 int rpd_grid_from_voxels(int g, const uint8_t *mask, rpd_grid **out);
 /* Grid of a closed triangle mesh, built on the GPU (replaces the mesh path of
    utils/createGrid/main.cpp:150-294): points (3*num_points xyz) and
@@ -118,7 +120,10 @@ int rpd_grid_save(const rpd_grid *grid, const char *path);
 /* Read-only view of a host grid (pointers stay valid until rpd_grid_free). */
 int rpd_grid_view(const rpd_grid *grid, rpd_grid_desc *out);
 void rpd_grid_free(rpd_grid *grid);
-/* Grid::valid (Grid.cpp:60-184) -> 1 valid, 0 invalid. */
+/* Grid::valid (Grid.cpp:60-184) -> 1 valid, 0 invalid.  Its rule that no
+   voxel has 8 border corners (Grid.cpp:147-167) is applied per voxel, to the
+   flags of v + {0,1}^3 (voxels on a max face skipped); the reference loop
+   counts the corners of voxel 0 for every border voxel instead. */
 int rpd_grid_valid(const rpd_grid_desc *grid);
 /* Grid::getVolume (Grid.cpp:30-35) and first moments (Grid.cpp:37-53). */
 double rpd_grid_volume(const rpd_grid_desc *grid);
@@ -200,7 +205,10 @@ const char *rpd_last_error(const rpd_ctx *ctx);
 void rpd_default_params(rpd_params *out);
 
 /* Replaces VoroAlgo::setGrid / GridAccess::init (voronoi.cpp:1079-1082,
-   78-144): copy the grid to HBM; it stays resident until rpd_domain_destroy. */
+   78-144): copy the grid to HBM; it stays resident until rpd_domain_destroy.
+   RPD_E_ARG for a malformed description: a missing array, a triangle point
+   id or list entry out of range, offsets not starting at 0, decreasing or
+   ending past num_tri_list. */
 int rpd_domain_create(rpd_ctx *ctx, const rpd_grid_desc *grid, rpd_domain **out);
 /* A domain may be destroyed before or after the context that created it; it
    may be used by any context of the same device. */

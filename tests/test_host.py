@@ -233,3 +233,91 @@ def test_create_grid_cli_errors(tmp_path):
     assert r.returncode == 1 and "can not read" in r.stderr
     r = subprocess.run([exe, "x.obj", "--bogus"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "unknown option" in r.stderr
+
+
+def _desc(rpd, size, in_domain, points, triangles, tri_list, offsets):
+    """A grid description over numpy arrays (kept alive by the caller)."""
+    d = rpd.GridDesc()
+    d.size = size
+    d.in_domain = in_domain.ctypes.data
+    d.points = points.ctypes.data
+    d.num_points = points.shape[0]
+    d.triangles = triangles.ctypes.data
+    d.num_triangles = triangles.size // 3
+    d.tri_list = tri_list.ctypes.data
+    d.num_tri_list = tri_list.size
+    d.offsets = offsets.ctypes.data
+    d.centre = None
+    return d
+
+
+def _valid(rpd, *arrays):
+    return bool(rpd.lib().rpd_grid_valid(ctypes.byref(_desc(rpd, *arrays))))
+
+
+def test_grid_valid_border_corner_rule(rpd, orc):
+    """Grid::valid's rule that no voxel has all 8 corners on the border
+    (Grid.cpp:147-167): a voxel's corners are the flags of v + {0,1}^3.  The
+    library tests it per voxel; the reference loop (restated by the oracle)
+    counts the corners of voxel 0 for every border voxel, so it only
+    rejects a grid whose voxel 0 has 8 border corners."""
+    grid = rpd.Grid.shape(rpd.SHAPE_SPHERE, 8, (0.5, 0.5, 0.5, 0.42))
+    g = grid.size
+    arrays = [grid.in_domain.copy(), grid.points, grid.triangles, grid.tri_list, grid.offsets]
+    assert _valid(rpd, g, *arrays)
+
+    def flags_with(block):
+        f = grid.in_domain.copy().reshape(g, g, g)
+        (x0, x1), (y0, y1), (z0, z1) = block
+        f[z0:z1, y0:y1, x0:x1] = 1
+        return f.reshape(-1)
+
+    def oracle_valid(flags):
+        return bool(orc.grid_valid(orc.Grid(g, flags, grid.points, grid.triangles, grid.tri_list, grid.offsets)))
+
+    # an interior voxel (3,3,3) whose 8 corners are border
+    f = flags_with(((3, 5), (3, 5), (3, 5)))
+    assert not _valid(rpd, g, f, *arrays[1:])
+    assert oracle_valid(f)  # the reference loop does not look at voxel (3,3,3)
+    # the same block against the max x face: the voxel there has no +x corners
+    f = flags_with(((g - 1, g), (3, 5), (3, 5)))
+    assert _valid(rpd, g, f, *arrays[1:]) and oracle_valid(f)
+    # voxel 0's corners all border: both reject
+    f = flags_with(((0, 2), (0, 2), (0, 2)))
+    assert not _valid(rpd, g, f, *arrays[1:]) and not oracle_valid(f)
+
+
+def test_malformed_grids_rejected(rpd, tmp_path):
+    """Offsets that decrease or run past the list, out-of-range list entries
+    or point ids: rpd_grid_valid says invalid (and rpd_domain_create, which
+    runs the same check, refuses them) and rpd_grid_load refuses the file."""
+    grid = rpd.Grid.shape(rpd.SHAPE_TORUS, 8, (0.5, 0.5, 0.5, 0.3, 0.15))
+    g = grid.size
+    base = [grid.in_domain, grid.points, grid.triangles, grid.tri_list, grid.offsets]
+    assert _valid(rpd, g, *base)
+    nz = int(np.nonzero(np.diff(grid.offsets))[0][0])
+    bad_offsets = grid.offsets.copy()
+    bad_offsets[nz + 2] = bad_offsets[nz + 1] - 1  # decreasing
+    past = grid.offsets.copy()
+    past[-1] = grid.tri_list.size + 5
+    bad_list = grid.tri_list.copy()
+    bad_list[0] = grid.triangles.size // 3
+    bad_tri = grid.triangles.copy()
+    bad_tri[0] = grid.points.shape[0]
+    for arrays in ([base[0], base[1], base[2], base[3], bad_offsets], [base[0], base[1], base[2], base[3], past],
+                   [base[0], base[1], base[2], bad_list, base[4]], [base[0], base[1], bad_tri, base[3], base[4]]):
+        assert not _valid(rpd, g, *arrays)
+    # a file with decreasing offsets: its last line holds the offsets
+    path = str(tmp_path / "torus.grid")
+    grid.save(path)
+    lines = open(path, "rb").read().split(b"\n")
+    assert lines[-1] == b"" and lines[-3] == b"%d" % (g ** 3 + 1)
+    lines[-2] = b" ".join(b"%d" % o for o in bad_offsets) + b" "
+    with open(path, "wb") as f:
+        f.write(b"\n".join(lines))
+    with pytest.raises(rpd.RpdError):
+        rpd.Grid.load(path)
+    lines[-2] = b" ".join(b"%d" % o for o in grid.offsets) + b" "
+    with open(path, "wb") as f:
+        f.write(b"\n".join(lines))
+    assert rpd.Grid.load(path).valid()
