@@ -61,10 +61,33 @@ class GridDesc(ctypes.Structure):
     ]
 
 
+# layout of rpd_read_debug_counters' output (rpd.h)
+DCOUNT_MAX = 0
+DCOUNT_HIST_K = 8
+DCOUNT_HIST_P = DCOUNT_HIST_K + 257
+DCOUNT_HIST_V = DCOUNT_HIST_P + 256
+DCOUNT_HIST_T = DCOUNT_HIST_V + 257
+DCOUNT_UNION_MAX = 4096
+DCOUNT_INTS = DCOUNT_HIST_T + DCOUNT_UNION_MAX + 1
+
+
+def read_device(ptr, dtype, count):
+    """Copies count elements of dtype from a device pointer (as returned by
+    the *_device entry points) to a new numpy array, through the HIP runtime
+    librpd is linked with."""
+    out = np.empty(int(count), dtype)
+    if count:
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [VP, VP, ctypes.c_size_t, ctypes.c_int]
+        if hip.hipMemcpy(_p(out), VP(ptr), out.nbytes, 2) != 0:  # hipMemcpyDeviceToHost
+            raise RpdError("hipMemcpy from the device failed")
+    return out
+
+
 class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in
                 ("K", "P", "V", "T", "precision", "max_extra_steps", "apex_mode", "security_mode", "cull", "H",
-                 "HC")] + [("reserved", ctypes.c_int32 * 1)]
+                 "HC", "debug_counters")]
 
 
 class Stats(ctypes.Structure):
@@ -160,6 +183,9 @@ def lib():
         "rpd_knn": (ctypes.c_int, [VP, I32, I32, VP]),
         "rpd_compact": (ctypes.c_int, [VP, ctypes.c_double, ctypes.POINTER(I64), VP, VP]),
         "rpd_compact_used": (ctypes.c_int, [VP, ctypes.c_double, I32, ctypes.POINTER(I64), VP, VP]),
+        "rpd_compact_used_device": (ctypes.c_int, [VP, ctypes.c_double, I32, ctypes.POINTER(I64),
+                                                   ctypes.POINTER(VP), ctypes.POINTER(VP)]),
+        "rpd_read_debug_counters": (ctypes.c_int, [VP, I32, VP]),
         "rpd_server_create": (ctypes.c_int, [VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(VP)]),
         "rpd_server_destroy": (None, [VP]),
         "rpd_server_submit": (ctypes.c_int, [VP, VP, VP, I64, ctypes.POINTER(Params), ctypes.POINTER(I64)]),
@@ -367,11 +393,12 @@ class Context:
         self.n = int(n)
 
     def solve(self, domain=None, K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, apex_mode=1,
-              security_mode=0, cull=1, H=0, HC=1):
+              security_mode=0, cull=1, H=0, HC=1, debug_counters=0):
         """rpd_solve; H > 0 also computes the Hessian (up to H neighbours per
-        seed, HC = 1 face area or 4 face centroid times area and area)."""
+        seed, HC = 1 face area or 4 face centroid times area and area);
+        debug_counters keeps the per-pass counters (debug_counters())."""
         p = Params(int(K), int(P), int(V), int(T), int(precision), int(max_extra_steps), int(apex_mode),
-                   int(security_mode), int(cull), int(H), int(HC))
+                   int(security_mode), int(cull), int(H), int(HC), int(debug_counters))
         st = Stats()
         self.check(lib().rpd_solve(self._h, domain._h if domain is not None else None, ctypes.byref(p),
                                    ctypes.byref(st)), "rpd_solve")
@@ -423,6 +450,25 @@ class Context:
         self.check(lib().rpd_compact(self._h, float(keep_volume), ctypes.byref(cnt), _p(out), _p(perm)),
                    "rpd_compact")
         return out, perm
+
+    def debug_counters(self, pass_index):
+        """rpd_read_debug_counters: {"max": [K, P, V, T, H], "K": hist[257],
+        "P": hist[256], "V": hist[257], "T": hist[4097]} of one pass."""
+        out = np.zeros(DCOUNT_INTS, np.int32)
+        self.check(lib().rpd_read_debug_counters(self._h, int(pass_index), _p(out)), "rpd_read_debug_counters")
+        return {"max": out[DCOUNT_MAX:DCOUNT_MAX + 5].copy(), "K": out[DCOUNT_HIST_K:DCOUNT_HIST_P].copy(),
+                "P": out[DCOUNT_HIST_P:DCOUNT_HIST_V].copy(), "V": out[DCOUNT_HIST_V:DCOUNT_HIST_T].copy(),
+                "T": out[DCOUNT_HIST_T:].copy()}
+
+    def compact_used_device(self, keep_volume, keep_distance):
+        """rpd_compact_used_device: the compaction left in HBM.  Returns
+        (count, device pointer of out[count, 4] doubles, device pointer of
+        perm_to_user[count] int32), valid until the next compaction / solve."""
+        cnt = I64(0)
+        po, pp = VP(), VP()
+        self.check(lib().rpd_compact_used_device(self._h, float(keep_volume), int(keep_distance), ctypes.byref(cnt),
+                                                 ctypes.byref(po), ctypes.byref(pp)), "rpd_compact_used_device")
+        return cnt.value, po.value, pp.value
 
     def compact_used(self, keep_volume, keep_distance):
         """rpd_compact_used: the used cells (m >= keep_volume) and keep_distance
@@ -563,33 +609,42 @@ class VoroAlgo:
                              self.max_extra_steps, self.apex_mode, self.security_mode, H=self.m_H, HC=self.m_HC)
         if stats is not None:
             stats.append(st)
-        if self.m_keepDistance < 0 or self._dom is None:
-            self._bary = self._ctx.moments()
+        full = self._ctx.moments()
+        self._compacted = not (self.m_keepDistance < 0 or self._dom is None)
+        if not self._compacted:
+            self._bary = full
             self._perm = np.arange(self._bary.shape[0], dtype=np.int32)
         else:
             self._bary, self._perm = self._ctx.compact_used(self.m_keepVolume, self.m_keepDistance)
         self._hess = self._hess_ids = None
         if self.m_H > 0:
-            # compact_result (voronoi.cpp:594-631): the output cells' entries,
-            # neighbours renumbered to output indices, dropped when not output
             vals, ids = self._ctx.hessian()
             H = self.m_H
-            pos = np.full(self._ctx.n, -1, np.int64)
-            pos[self._perm] = np.arange(self._perm.size)
-            hv = np.zeros((self._perm.size, H, self.m_HC))
-            hi = np.full((self._perm.size, H), -1, np.int32)
-            for o, u in enumerate(self._perm):
-                w = 0
-                for h in range(H):
-                    j = ids[u, h]
-                    if j < 0:
-                        break
-                    if pos[j] < 0:
+            if not self._compacted:
+                self._hess, self._hess_ids = vals.reshape(-1), ids.reshape(-1)
+            else:
+                # compact_result (voronoi.cpp:594-631): the output cells'
+                # entries, neighbours renumbered to output positions, dropped
+                # when not output or without volume; a cell without volume has
+                # no entries
+                pos = np.full(self._ctx.n, -1, np.int64)
+                pos[self._perm] = np.arange(self._perm.size)
+                hv = np.zeros((self._perm.size, H, self.m_HC))
+                hi = np.full((self._perm.size, H), -1, np.int32)
+                for o, u in enumerate(self._perm):
+                    if full[u, 3] < self.m_keepVolume:
                         continue
-                    hi[o, w] = pos[j]
-                    hv[o, w] = vals[u, h]
-                    w += 1
-            self._hess, self._hess_ids = hv.reshape(-1), hi.reshape(-1)
+                    w = 0
+                    for h in range(H):
+                        j = ids[u, h]
+                        if j < 0:
+                            break
+                        if pos[j] < 0 or full[j, 3] < self.m_keepVolume:
+                            continue
+                        hi[o, w] = pos[j]
+                        hv[o, w] = vals[u, h]
+                        w += 1
+                self._hess, self._hess_ids = hv.reshape(-1), hi.reshape(-1)
         ok = True if saveData is None else bool(saveData(self))
         return ok
 
@@ -601,6 +656,11 @@ class VoroAlgo:
 
     def getPermutationToUser(self):
         return self._perm
+
+    def getPermutationToInputPoints(self):
+        """Output position -> index of the input points (the user's order);
+        empty when it is the identity (voronoi.h:183-187)."""
+        return self._perm if getattr(self, "_compacted", False) else np.zeros(0, np.int32)
 
     def getHessians(self):
         """m_HC * m_H values per output cell (voronoi.cpp:1179-1212); empty with m_H = 0."""

@@ -40,6 +40,8 @@
 #include <thread>
 #include <vector>
 
+#include <rocprim/device/device_scan.hpp>
+
 #include "rpd_internal.h"
 
 // A domain keeps no pointer to the context that made it: it may outlive that
@@ -90,6 +92,15 @@ struct rpd_ctx {
     // used-cell marks (per stored id) and the scratch outputs of the
     // bordering passes; caps and stop test of the last solve
     DevBuf used, bout, bstatus;
+    // compacted output (compressResult): per original index keep flag and
+    // ring, their exclusive scan, the kept cells' values and original
+    // indices, rocPRIM scan scratch
+    DevBuf ckeep, cring, cpos, cout, cperm, ctmp;
+    // debug counters of the last solve that asked for them: RPD_MAX_PASSES
+    // blocks of kDcStride ints
+    DevBuf dcnt;
+    bool dcnt_ok = false;
+    int64_t ccount = 0;  // cells in cout / cperm after the last compaction
     int last_K = 0, last_P = 0, last_V = 0, last_sm = 0, last_steps = 0;
     // pass work buffers: up to three passes may run at once, one per stream
     PassBufs pb[4];
@@ -217,6 +228,44 @@ __global__ __launch_bounds__(256) void used_init_kernel(const double4 *__restric
                                                         int64_t n, double keep, int *__restrict__ used) {
     const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (s < n) used[s] = out[perm[s]].w >= keep ? 1 : 0;
+}
+
+// Compaction marks, per original index i: keep[i] = 1 when the cell is kept
+// (ring[i] = its mark, used == nullptr: every cell, mark 1), 0 otherwise;
+// keep[n] = 0 so that the scan's last entry is the count.
+__global__ __launch_bounds__(256) void compact_mark_kernel(const int *__restrict__ used, const int *__restrict__ perm,
+                                                           int64_t n, int *__restrict__ keep,
+                                                           int *__restrict__ ring) {
+    const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (s < n) {
+        const int u = used ? used[s] : 1;
+        const int o = perm[s];
+        keep[o] = u != 0 ? 1 : 0;
+        ring[o] = u;
+    } else if (s == n) {
+        keep[n] = 0;
+    }
+}
+
+// compact_result (voronoi.cpp:582-691) in ascending original index: the kept
+// cells' moments and original indices at their scanned positions.  Marks are
+// those of the bordering passes: 1 for the used cells and their face
+// neighbours, d + 1 for the neighbours ring d adds; a cell marked last (the
+// keep distance) whose volume is below keep gets w = NO_CELL_LIMITS (the
+// reference's MAX_DIST test); last < 0: no flags.
+__global__ __launch_bounds__(256) void compact_scatter_kernel(const double4 *__restrict__ out,
+                                                              const int *__restrict__ keepf,
+                                                              const int *__restrict__ ring,
+                                                              const int *__restrict__ pos, int64_t n, double keep,
+                                                              int last, double4 *__restrict__ cout,
+                                                              int *__restrict__ cperm) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i >= n || !keepf[i]) return;
+    double4 v = out[i];
+    if (last >= 0 && v.w < keep && ring[i] == last) v.w = RPD_NO_CELL_LIMITS;
+    const int p = pos[i];
+    cout[p] = v;
+    cperm[p] = int(i);
 }
 
 // The cells ring k expands (find_bordering, voronoi.cpp:693-741): ring 0 the
@@ -360,7 +409,7 @@ void rpd_default_params(rpd_params *p) {
     p->cull = 1;
     p->H = 0;  // no Hessian (VoroAlgo::m_H, voronoi.cpp:1048)
     p->HC = 1;
-    p->reserved[0] = 0;
+    p->debug_counters = 0;
 }
 
 int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
@@ -592,6 +641,17 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         HIPCHK(c, hipMemsetAsync(c->hids.ptr, 0xff, size_t(n) * prm->H * sizeof(int), c->stream));
     }
 
+    // debug counters: one zeroed block per pass (RPD_MAX_PASSES - 1 and
+    // later passes share the last), read back by rpd_read_debug_counters
+    c->dcnt_ok = false;
+    if (prm->debug_counters) {
+        RPDCHK(c, c->dcnt.ensure(size_t(RPD_MAX_PASSES) * kDcStride * sizeof(int)), "alloc debug counters");
+        HIPCHK(c, hipMemsetAsync(c->dcnt.ptr, 0, size_t(RPD_MAX_PASSES) * kDcStride * sizeof(int), c->stream));
+    }
+    auto dcnt_of = [&](int index) -> int * {
+        return prm->debug_counters ? c->dcnt.as<int>() + size_t(std::min(index, RPD_MAX_PASSES - 1)) * kDcStride
+                                   : nullptr;
+    };
     const DomainDev *dd = dom ? &dom->dev : nullptr;
     double wmin = 0;
     bool border_seeds = true;
@@ -635,6 +695,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     // go to failure list fl_out; fl_cnt_host receives the list's counters
     // as they stand once the bisector kernel is done.
     struct Pass {
+        int index = 0;  // pass number (0 the first), for the debug counters
         int slot = 0;  // stream, events and buffers
         int form = 0;
         int64_t n_ids = 0;
@@ -705,6 +766,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
             L.H = prm->H;
             L.hc = prm->HC;
         }
+        L.dcnt = dcnt_of(ps.index);
         return rpd_launch_cells(s, ps.form, L, c->pin + sl, false);
     };
     // the finish kernel of a pass whose bisector kernel (and queue length) is done
@@ -831,6 +893,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         ch.pz = Pass{};
         ch.pr.slot = ch.sl;
         ch.pz.slot = ch.sl2;
+        ch.pr.index = ch.pz.index = 1 + ch.k;
         ch.pr.form = ch.pz.form = RPD_PREC_FP64;
         ch.pr.n_ids = ch.n_res;
         ch.pz.n_ids = ch.n_scr;
@@ -984,6 +1047,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
             L.rslot = c->rslot.as<int>();
             L.lastnb = c->lastnb.as<int>();
         }
+        L.dcnt = dcnt_of(0);
         RPDCHK(c, rpd_launch_cells(s, form0, L, c->pin + 0, true), "pass 1 cells");
         p1.nq = c->pin[0];
         L.fl_cnt = c->fl[1].cnt.as<int>();
@@ -1059,6 +1123,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     c->last_sm = prm->security_mode;
     c->last_steps = prm->max_extra_steps;
     c->solved = true;
+    c->dcnt_ok = prm->debug_counters != 0;
     if (stats) *stats = st;
     return RPD_OK;
 }
@@ -1117,44 +1182,44 @@ int rpd_read_perm(rpd_ctx *c, int32_t *out) {
     return RPD_OK;
 }
 
-int rpd_compact(rpd_ctx *c, double keep_volume, int64_t *out_count, double *out4, int32_t *perm_to_user) {
-    if (!c || !out_count) return set_err(c, RPD_E_ARG, "rpd_compact");
-    if (!c->solved) return set_err(c, RPD_E_STATE, "rpd_compact: no solve");
-    std::vector<double> m(size_t(c->n) * 4);
-    int r = rpd_read_moments(c, m.data());
-    if (r != RPD_OK) return r;
-    int64_t k = 0;
-    for (int64_t i = 0; i < c->n; ++i) {
-        if (!(m[4 * i + 3] >= keep_volume)) continue;
-        if (out4)
-            for (int j = 0; j < 4; ++j) out4[4 * k + j] = m[4 * i + j];
-        if (perm_to_user) perm_to_user[k] = int32_t(i);
-        ++k;
-    }
-    *out_count = k;
-    return RPD_OK;
-}
+}  // extern "C"
 
-int rpd_compact_used(rpd_ctx *c, double keep_volume, int32_t keep_distance, int64_t *out_count, double *out4,
-                     int32_t *perm_to_user) {
-    if (!c || !out_count) return set_err(c, RPD_E_ARG, "rpd_compact_used");
+namespace {
+
+// Used cells, bordering rings and the compacted output on the device
+// (compressResult and findBordering, voronoi.cpp:513-751): fills c->cout,
+// c->cperm and c->ccount.
+int compact_device(rpd_ctx *c, double keep_volume, int32_t keep_distance) {
     if (!c->solved) return set_err(c, RPD_E_STATE, "rpd_compact_used: no solve");
     if (keep_distance > 254) return set_err(c, RPD_E_ARG, "rpd_compact_used: keep_distance > 254");
     hipSetDevice(c->device);
     const int64_t n = c->n;
-    std::vector<double> m(size_t(n) * 4);
-    int r = rpd_read_moments(c, m.data());
-    if (r != RPD_OK) return r;
-    if (keep_distance < 0) {  // every cell, in the per-seed layout
-        for (int64_t i = 0; i < n; ++i) {
-            if (out4)
-                for (int j = 0; j < 4; ++j) out4[4 * i + j] = m[4 * i + j];
-            if (perm_to_user) perm_to_user[i] = int32_t(i);
-        }
-        *out_count = n;
-        return RPD_OK;
-    }
     hipStream_t s = c->stream;
+    RPDCHK(c, c->ckeep.ensure(size_t(n + 1) * 4), "alloc compaction");
+    RPDCHK(c, c->cring.ensure(size_t(n) * 4), "alloc compaction");
+    RPDCHK(c, c->cpos.ensure(size_t(n + 1) * 4), "alloc compaction");
+    RPDCHK(c, c->cout.ensure(size_t(n) * 32), "alloc compaction");
+    RPDCHK(c, c->cperm.ensure(size_t(n) * 4), "alloc compaction");
+    size_t scan_bytes = 0;
+    HIPCHK(c, rocprim::exclusive_scan(nullptr, scan_bytes, c->ckeep.as<int>(), c->cpos.as<int>(), 0, size_t(n + 1),
+                                      rocprim::plus<int>(), s));
+    RPDCHK(c, c->ctmp.ensure(scan_bytes), "alloc compaction");
+    const unsigned nb1 = unsigned((n + 256) / 256);  // n + 1 threads: the mark kernel also ends the array
+    auto finish = [&](const int *used, int last) -> int {
+        compact_mark_kernel<<<nb1, 256, 0, s>>>(used, c->perm.as<int>(), n, c->ckeep.as<int>(), c->cring.as<int>());
+        HIPCHK(c, rocprim::exclusive_scan(c->ctmp.ptr, scan_bytes, c->ckeep.as<int>(), c->cpos.as<int>(), 0,
+                                          size_t(n + 1), rocprim::plus<int>(), s));
+        compact_scatter_kernel<<<nb1, 256, 0, s>>>(c->out.as<double4>(), c->ckeep.as<int>(), c->cring.as<int>(),
+                                                  c->cpos.as<int>(), n, keep_volume, last, c->cout.as<double4>(),
+                                                  c->cperm.as<int>());
+        HIPCHK(c, hipMemcpyAsync(c->pin + kPinCounters - 1, c->cpos.as<int>() + n, sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        c->ccount = c->pin[kPinCounters - 1];
+        return RPD_OK;
+    };
+    if (keep_distance < 0) return finish(nullptr, -1);  // every cell
+
     RPDCHK(c, c->used.ensure(size_t(n) * 4), "alloc used");
     RPDCHK(c, c->ids1.ensure(size_t(n) * 4), "alloc ids");
     const unsigned nb = unsigned((n + 255) / 256);
@@ -1231,24 +1296,52 @@ int rpd_compact_used(rpd_ctx *c, double keep_volume, int32_t keep_distance, int6
     // compaction (compute_permutation / compact_result, voronoi.cpp:513-691) in
     // ascending original index; a cell of the last ring without volume is
     // flagged with w = NO_CELL_LIMITS
-    std::vector<int> used(static_cast<size_t>(n)), perm(static_cast<size_t>(n));
-    HIPCHK(c, hipMemcpyAsync(used.data(), c->used.ptr, size_t(n) * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(perm.data(), c->perm.ptr, size_t(n) * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
-    std::vector<int> mark(static_cast<size_t>(n), 0);
-    for (int64_t sid = 0; sid < n; ++sid) mark[size_t(perm[size_t(sid)])] = used[size_t(sid)];
-    int64_t k = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        if (mark[size_t(i)] == 0) continue;
-        if (out4) {
-            for (int j = 0; j < 4; ++j) out4[4 * k + j] = m[4 * i + j];
-            if (m[4 * i + 3] < keep_volume && mark[size_t(i)] == keep_distance) out4[4 * k + 3] = RPD_NO_CELL_LIMITS;
-        }
-        if (perm_to_user) perm_to_user[k] = int32_t(i);
-        ++k;
-    }
+    return finish(c->used.as<int>(), keep_distance);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rpd_read_debug_counters(rpd_ctx *c, int32_t pass, int32_t *out) {
+    if (!c || !out || pass < 0 || pass >= RPD_MAX_PASSES) return set_err(c, RPD_E_ARG, "rpd_read_debug_counters");
+    if (!c->solved || !c->dcnt_ok) return set_err(c, RPD_E_STATE, "rpd_read_debug_counters: no solve with counters");
+    hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpyAsync(out, c->dcnt.as<int>() + size_t(pass) * kDcStride, kDcStride * sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPD_OK;
+}
+
+int rpd_compact_used_device(rpd_ctx *c, double keep_volume, int32_t keep_distance, int64_t *out_count,
+                            const double **out4, const int32_t **perm_to_user) {
+    if (!c || !out_count) return set_err(c, RPD_E_ARG, "rpd_compact_used_device");
+    const int r = compact_device(c, keep_volume, keep_distance);
+    if (r != RPD_OK) return r;
+    *out_count = c->ccount;
+    if (out4) *out4 = c->cout.as<double>();
+    if (perm_to_user) *perm_to_user = c->cperm.as<int32_t>();
+    return RPD_OK;
+}
+
+int rpd_compact_used(rpd_ctx *c, double keep_volume, int32_t keep_distance, int64_t *out_count, double *out4,
+                     int32_t *perm_to_user) {
+    if (!c || !out_count) return set_err(c, RPD_E_ARG, "rpd_compact_used");
+    const int r = compact_device(c, keep_volume, keep_distance);
+    if (r != RPD_OK) return r;
+    const int64_t k = c->ccount;
+    if (k > 0 && out4)
+        HIPCHK(c, hipMemcpyAsync(out4, c->cout.ptr, size_t(k) * 32, hipMemcpyDeviceToHost, c->stream));
+    if (k > 0 && perm_to_user)
+        HIPCHK(c, hipMemcpyAsync(perm_to_user, c->cperm.ptr, size_t(k) * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     *out_count = k;
     return RPD_OK;
+}
+
+int rpd_compact(rpd_ctx *c, double keep_volume, int64_t *out_count, double *out4, int32_t *perm_to_user) {
+    // the used cells alone: keep distance 0 (no ring, no flags)
+    return rpd_compact_used(c, keep_volume, 0, out_count, out4, perm_to_user);
 }
 
 }  // extern "C"

@@ -112,6 +112,7 @@ struct ClipArgs {
     double *hess;
     int *hids;
     int H, hc;
+    int *dcnt;  // debug counters of this pass (kDcStride ints), nullable
 };
 
 // Appends a failed cell (stored id sid) to the failure list: overflows
@@ -777,6 +778,23 @@ void clip_cells_kernel(ClipArgs a) {
     // exact once the list is used up (the reference reads past the seeds there)
     if (!reached && !complete && int64_t(kbase) + K >= a.n_seeds - 1) complete = true;
 
+    // debug counters (ConvexCell.cl:887-911): K used where the security
+    // radius was reached, P and V once the neighbour loop is done
+    if (a.dcnt && lane == 0) {
+        if (reached) {
+            const int ku = min(kbase + kused, 256);
+            atomicMax(a.dcnt + kDcMax + 0, ku);
+            atomicAdd(a.dcnt + kDcHistK + ku, 1);
+        }
+        int nv = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) nv += __popcll(c.valid[s]);
+        atomicMax(a.dcnt + kDcMax + 1, c.nb_plane);
+        atomicAdd(a.dcnt + kDcHistP + min(c.nb_plane, 255), 1);
+        atomicMax(a.dcnt + kDcMax + 2, nv);
+        atomicAdd(a.dcnt + kDcHistV + min(nv, 256), 1);
+    }
+
     // A finished cell is queued for restriction + integration.  A cell whose
     // list ran out keeps its state when there is room, so that the next pass
     // goes on from here with the neighbours after the last one used.
@@ -1218,6 +1236,10 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
             fail(RPD_TRIANGLE_OVERFLOW);
             return;
         }
+        if (a.dcnt && sub == 0 && lane == 0) {  // ConvexCell.cl:981-982
+            atomicMax(a.dcnt + kDcMax + 3, nu);
+            atomicAdd(a.dcnt + kDcHistT + min(nu, kDcUnionMax), 1);
+        }
         wsync();
         ntri = nu;
         // apex
@@ -1565,6 +1587,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
             st_out = RPD_HESSIAN_OVERFLOW;
         else if (cnt < H && lane == 0)
             a.hids[size_t(orig) * size_t(H) + size_t(cnt)] = -1;
+        if (a.dcnt && lane == 0) atomicMax(a.dcnt + kDcMax + 4, min(cnt, H));  // ConvexCell.cl:1150
     }
     // local first moments -> global frame
     finish(st_out, acc_x + acc_m * S0.x, acc_y + acc_m * S0.y, acc_z + acc_m * S0.z, acc_m);
@@ -1673,6 +1696,7 @@ static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     a.H = L.hess && L.hids ? L.H : 0;
     a.hc = L.hc == 4 ? 4 : 1;
     a.exact_final = form == RPD_PREC_FP64 ? 1 : 0;
+    a.dcnt = L.dcnt;
     return RPD_OK;
 }
 

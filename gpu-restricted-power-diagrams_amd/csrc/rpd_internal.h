@@ -133,9 +133,23 @@ struct ClipLaunch {
     double *hess;
     int *hids;
     int H, hc;
+    int *dcnt;             // debug counters of the pass (kDcStride ints, nullable)
 };
 
 constexpr int kFailCounters = 16;
+
+// Debug counters of one pass (the reference's debugCounter0..4,
+// ConvexCell.cl:15-39, 887-911, 981-982, 1128-1150): [kDcMax + 0..4] largest
+// K used, P and V after the neighbour loop, triangle union, Hessian entries;
+// then histograms of K used (0..256), P (0..255), V (0..256) and the union
+// size (0..kDcUnionMax, larger unions in the last bin).
+constexpr int kDcMax = RPD_DCOUNT_MAX;
+constexpr int kDcHistK = RPD_DCOUNT_HIST_K;
+constexpr int kDcHistP = RPD_DCOUNT_HIST_P;
+constexpr int kDcHistV = RPD_DCOUNT_HIST_V;
+constexpr int kDcHistT = RPD_DCOUNT_HIST_T;
+constexpr int kDcUnionMax = RPD_DCOUNT_UNION_MAX;
+constexpr int kDcStride = RPD_DCOUNT_INTS;
 
 // Cells whose restriction is provably empty, before the kNN (rpd_cull.hip).
 struct CoverArgs {

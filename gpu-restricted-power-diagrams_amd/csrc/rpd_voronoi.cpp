@@ -28,7 +28,11 @@ namespace {
 struct Options {
     int K = 20, P = 25, V = 30, T = 100;  // Options.cpp:20-28
     double keepVolume = 0.001;            // USE_DOUBLE default (Options.cpp:36-40)
-    int precision = RPD_PREC_MIXED;
+    int precision = -1;                   // -1: from --cVoroIsInHalfSpaceAlgo
+    int vhs = 1;                          // half-space algorithm of the first step (Options.cpp:55, float build)
+    int keepDistLast = 0;                 // --keepDist2 (Options.cpp:42): the final keep distance
+    int maxK = 100000;                    // --maxK (accepted)
+    bool counterMax = false, counterHisto = false, counterStatus = false;
     int H = 50, HC = 1;                   // Options.cpp:190-191
     bool hessian = false;                 // --hessian (Options.cpp:77, 269)
     bool help = false, points4D = false, loadBinary = false, checkGrid = false, invariants = false, debug = false;
@@ -54,7 +58,12 @@ void usage(const char *prog) {
               << "\t--H=h: set the max number of Hessian values in a row[default 50]\n"
               << "\t--HC=C: set the number of Hessian components(1 or 4)[default 1]\n"
               << "\t--hessian: compute the hessian value\n"
-              << "\t--precision=fp64|mixed|fp32: clip precision form[default mixed]\n"
+              << "\t--precision=fp64|mixed|fp32: clip precision form[default: from --cVoroIsInHalfSpaceAlgo]\n"
+              << "\t--cVoroIsInHalfSpaceAlgo=algo: 0 FP32, 1 FP32 with check (mixed), 2 FP64[default 1]\n"
+              << "\t--keepDist=d, --keepDist2=d: rings of bordering cells kept in the output[default 0]\n"
+              << "\t--maxK=k: accepted (the kNN is one launch)\n"
+              << "\t--cVoroDebugCounterMax, --cVoroDebugCounterHisto, --cVoroDebugCounterStatus: per-pass\n"
+              << "\t    counters of K, P, V, T (maxima, histograms) and statuses in GTime[\"stat\"]\n"
               << "\t--output=xyz|mg|geogram|volume: outputs (gridGPU.xyz, gridGPU_mg.xyzw,\n"
               << "\t    gridGPU.geogram_ascii, volumes on stderr)\n"
               << "\t--printInvariants: displays the final volume and barycenter\n"
@@ -96,7 +105,14 @@ bool parse(int argc, char **argv, Options &o, std::string &file) {
         } else if (a == "--grid") o.grid = val();
         else if (a == "--saveBinaryPoints") o.saveBinary = val();
         else if (a == "--output") o.outputs.push_back(val());
-        else if (a == "--precision") {
+        else if (a == "--cVoroIsInHalfSpaceAlgo") o.vhs = std::atoi(val().c_str());
+        else if (a == "--keepDist" || a == "--keepDist2") o.keepDistLast = std::atoi(val().c_str());
+        else if (a == "--maxK") o.maxK = std::atoi(val().c_str());
+        else if (a == "--cVoroDebugCounterMax") o.counterMax = true;
+        else if (a == "--cVoroDebugCounterHisto") o.counterHisto = true;
+        else if (a == "--cVoroDebugCounterStatus") o.counterStatus = true;
+        else if (a == "--enableProfiling") {
+        } else if (a == "--precision") {
             const std::string p = val();
             o.precision = p == "fp64" ? RPD_PREC_FP64 : (p == "fp32" ? RPD_PREC_FP32 : RPD_PREC_MIXED);
         } else if (a.size() > 1 && a[0] == '-') {
@@ -184,20 +200,33 @@ int main(int argc, char **argv) {
     if (hasGrid && o.checkGrid && !grid.valid()) std::cerr << argv[0] << ": the grid is invalid." << std::endl;
 
     std::vector<double> bary;
+    std::vector<int> permutation;
     std::vector<rpd::VoroAlgoStat> stats;
     try {
+        // computeVoroDiagram (test_voronoi.cpp:30-62) with
+        // Options::initVoroAlgoParameters (Options.cpp:253-280)
         rpd::VoroAlgo algo(0, o.debug);
         algo.m_K = o.K;
+        algo.m_maxKComputed = o.maxK;
+        algo.m_vhsAlgo[0] = o.vhs;
+        algo.m_vhsAlgo[1] = 2;
+        algo.m_precision = o.precision;
         algo.m_P = o.P;
         algo.m_V = o.V;
         algo.m_T = o.T;
-        algo.m_precision = o.precision;
-        algo.m_H = o.hessian ? o.H : 0;  // Options.cpp:268-270
+        algo.m_H = o.hessian ? o.H : 0;
         algo.m_HC = o.HC;
+        algo.m_keepDistance = o.keepDistLast;  // the final keep distance
+        algo.m_keepVolume = o.keepVolume;
+        algo.m_enableDCountersHistoK = algo.m_enableDCountersHistoP = algo.m_enableDCountersHistoT =
+            algo.m_enableDCountersHistoV = o.counterHisto;
+        algo.m_enableDCountersMax = o.counterMax;
+        algo.m_enableDCountersStatus = o.counterStatus;
         algo.setInputPoints(pts);
         if (hasGrid) algo.setGrid(grid);
         if (!algo.launch([&](rpd::VoroAlgo const &a) {
                 bary = a.getBarycenters();
+                permutation = a.getPermutationToUser();
                 return true;
             }, stats)) {
             std::cerr << argv[0] << ": solve failed: " << algo.lastError() << std::endl;
@@ -211,15 +240,8 @@ int main(int argc, char **argv) {
     // GTime["stat"] (test_voronoi.cpp:52-60, VoroAlgoStat::pythonPrint)
     std::cout << "GTime[\"stat\"] = [";
     for (size_t i = 0; i < stats.size(); ++i) {
-        const auto &s = stats[i];
-        std::cout << "{\"name\" : \"" << s.m_name << "\", \"knn\":" << s.m_times[0] << ", \"CC\":" << s.m_times[1]
-                  << ", \"time\":" << s.m_times[2] << ", ";
-        for (auto const &it : s.m_idToHistoMap) {
-            std::cout << "\"counter" << it.first << "\":[";
-            for (size_t k = 0; k < it.second.size(); ++k) std::cout << it.second[k] << (k + 1 < it.second.size() ? "," : "");
-            std::cout << "], ";
-        }
-        std::cout << "\"nbcells\":" << s.m_num << "}" << (i + 1 < stats.size() ? ",\n" : "");
+        stats[i].pythonPrint(std::cout);
+        if (i + 1 < stats.size()) std::cout << ",\n";
     }
     std::cout << "]\n";
 
@@ -272,16 +294,20 @@ int main(int argc, char **argv) {
             f << "[ATTR]\n\"GEO::Mesh::vertices\"\n\"w\"\n\"double\"\n8\n1\n";
             for (size_t i = 0; i < n; ++i) f << bary[4 * i + 3] << "\n";
         } else if (s == "mg") {
-            // per original seed (test_voronoi.cpp:186-211)
+            // per original seed, through the output permutation (test_voronoi.cpp:186-211)
             std::ofstream f("gridGPU_mg.xyzw");
-            f << n << "\n";
-            for (size_t i = 0; i < n; ++i) {
-                if (bary[4 * i + 3] <= o.keepVolume) {
+            const size_t np = pts.size() / 4;
+            std::vector<long long> pos(np, -1);
+            for (size_t k = 0; k < permutation.size(); ++k) pos[size_t(permutation[k])] = (long long)k;
+            f << np << "\n";
+            for (size_t i = 0; i < np; ++i) {
+                const long long q = pos[i];
+                if (q < 0 || bary[4 * size_t(q) + 3] <= o.keepVolume) {
                     f << "0 0 0 0\n";
                     continue;
                 }
-                f << std::setprecision(20) << bary[4 * i] << " " << bary[4 * i + 1] << " " << bary[4 * i + 2] << " "
-                  << bary[4 * i + 3] << "\n";
+                const double *b = &bary[4 * size_t(q)];
+                f << std::setprecision(20) << b[0] << " " << b[1] << " " << b[2] << " " << b[3] << "\n";
             }
         } else if (s == "volume") {
             if (!o.invariants) volume_report();

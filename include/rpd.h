@@ -168,7 +168,10 @@ typedef struct {
     int32_t HC;                   /* values per Hessian entry (m_HC): 1 face
                                      area, 4 face centroid times area (x, y, z)
                                      and area                                 */
-    int32_t reserved[1];
+    int32_t debug_counters;       /* non-zero: keep the debug counters of each
+                                     pass (VoroAlgo::m_enableDCounters*,
+                                     voronoi.h:93-102), read with
+                                     rpd_read_debug_counters; 0 (default): none */
 } rpd_params;
 
 /* Per-solve statistics (VoroAlgoStat, voronoi.h:60-103). */
@@ -227,6 +230,31 @@ int rpd_set_seeds_device(rpd_ctx *ctx, const double *d_xyzw, int64_t n);
    integrate with the status re-queue.  dom may be NULL (domain [0,1000]^3). */
 int rpd_solve(rpd_ctx *ctx, rpd_domain *dom, const rpd_params *prm, rpd_stats *stats);
 
+/* Debug counters of pass `pass` (0 the first, k the k-th re-queue pass; the
+   two re-queue chains add into the same k) of the last solve run with
+   debug_counters set (the reference's debugCounter0..4, ConvexCell.cl:15-39,
+   887-911, 981-982, 1128-1150), RPD_DCOUNT_INTS int32 in out:
+   [RPD_DCOUNT_MAX + 0..4]  largest K used where the security radius was
+                            reached, P and V after the neighbour loop,
+                            triangle union, Hessian entries of a cell;
+   [RPD_DCOUNT_HIST_K + k]  cells that reached their radius at neighbour k
+                            (k <= 256; counting from the first pass of a
+                            resumed cell);
+   [RPD_DCOUNT_HIST_P + p]  cells with p planes after the neighbour loop;
+   [RPD_DCOUNT_HIST_V + v]  cells with v vertices after it;
+   [RPD_DCOUNT_HIST_T + t]  cells whose triangle union had t entries (the
+                            last bin also counts larger unions).
+   Cells the domain cull ends (cull = 1) count only where they reached the
+   loop's end.  RPD_E_STATE without such a solve. */
+#define RPD_DCOUNT_MAX 0
+#define RPD_DCOUNT_HIST_K 8
+#define RPD_DCOUNT_HIST_P (RPD_DCOUNT_HIST_K + 257)
+#define RPD_DCOUNT_HIST_V (RPD_DCOUNT_HIST_P + 256)
+#define RPD_DCOUNT_HIST_T (RPD_DCOUNT_HIST_V + 257)
+#define RPD_DCOUNT_UNION_MAX 4096
+#define RPD_DCOUNT_INTS (RPD_DCOUNT_HIST_T + RPD_DCOUNT_UNION_MAX + 1)
+int rpd_read_debug_counters(rpd_ctx *ctx, int32_t pass, int32_t *out);
+
 /* Outputs, valid until the next rpd_set_seeds/rpd_solve on this context. */
 /* Replaces getBarycenters (voronoi.cpp:1173-1183) in the per-original-seed
    "mg" layout of test_voronoi.cpp:186-211: 4n doubles (m*x, m*y, m*z, m). */
@@ -255,7 +283,8 @@ int rpd_hessian_device(rpd_ctx *ctx, const double **vals, const int32_t **ids, i
 int rpd_knn(rpd_ctx *ctx, int32_t K, int32_t knn_precision, int32_t *out_nk);
 
 /* Compaction of used cells (compressResult, voronoi.cpp:513-691) in a
-   deterministic order: cells with m >= keep_volume, ascending original index.
+   deterministic order, on the device (a scan of the keep flags, then a
+   scatter): cells with m >= keep_volume, ascending original index.
    out_count receives the number of used cells; out4 (4*count) and
    out_perm_to_user (count) may be NULL to query the count. */
 int rpd_compact(rpd_ctx *ctx, double keep_volume, int64_t *out_count, double *out4, int32_t *out_perm_to_user);
@@ -270,6 +299,13 @@ int rpd_compact(rpd_ctx *ctx, double keep_volume, int64_t *out_count, double *ou
    out4 and out_perm_to_user may be NULL to query the count. */
 int rpd_compact_used(rpd_ctx *ctx, double keep_volume, int32_t keep_distance, int64_t *out_count, double *out4,
                      int32_t *out_perm_to_user);
+/* The same compaction left in HBM (getBarycentersCL and
+   getPermutationToUserCL after compressResult, voronoi.h:150-160,
+   voronoi.cpp:1140-1160): *out4 receives a device pointer to 4*count
+   doubles, *out_perm_to_user one to count original indices; both stay valid
+   until the next compaction, rpd_set_seeds or rpd_solve on this context. */
+int rpd_compact_used_device(rpd_ctx *ctx, double keep_volume, int32_t keep_distance, int64_t *out_count,
+                            const double **out4, const int32_t **out_perm_to_user);
 
 /* ---- serving --------------------------------------------------------------
    A request loop for streams of solves with different seed counts and
