@@ -637,8 +637,9 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     if (hess) {
         RPDCHK(c, c->hess.ensure(size_t(n) * prm->H * prm->HC * sizeof(double)), "alloc hessian");
         RPDCHK(c, c->hids.ensure(size_t(n) * prm->H * sizeof(int)), "alloc hessian");
-        // no entries (-1) until a cell writes its own
+        // no entries (-1, values 0) until a cell writes its own
         HIPCHK(c, hipMemsetAsync(c->hids.ptr, 0xff, size_t(n) * prm->H * sizeof(int), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->hess.ptr, 0, size_t(n) * prm->H * prm->HC * sizeof(double), c->stream));
     }
 
     // debug counters: one zeroed block per pass (RPD_MAX_PASSES - 1 and
@@ -811,7 +812,13 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         Caps nc = cap;
         if (h[RPD_VERTEX_OVERFLOW]) nc.V = std::min(256, int(cap.V * 1.5));
         if (h[RPD_PLANE_OVERFLOW]) nc.P = std::min(255, int(cap.P * 1.5));
-        if (h[RPD_SECURITY_RADIUS_NOT_REACHED]) nc.K = int(std::min<int64_t>(n - 1, std::min(256, 2 * cap.K)));
+        // K doubles, by at most 256 a pass: a resumed cell lists only the
+        // neighbours after the ones it used (at most 256, the kNN's limit),
+        // so a cell may go on past 256 neighbours in all (a cell bounded by
+        // walls reaches twice as far as its bisectors); a recomputed cell
+        // lists at most 256
+        if (h[RPD_SECURITY_RADIUS_NOT_REACHED])
+            nc.K = int(std::min<int64_t>(n - 1, int64_t(cap.K) + std::min(cap.K, 256)));
         if (h[RPD_TRIANGLE_OVERFLOW]) nc.T = std::max(int(cap.T * 1.5) + 1, cnt[13]);
         nc.T = std::min(nc.T, rpd_clip_max_T(RPD_PREC_FP64, nc.P, nc.V, hess));
         return nc;
@@ -900,7 +907,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         ch.pr.caps[0] = nc.K - ch.cap.K;  // the neighbours after the ones used
         // recomputed cells need the grown K only when some of them ran out
         // of neighbours without a kept state (the record buffer was full)
-        ch.pz.caps[0] = cnt[2 + RPD_SECURITY_RADIUS_NOT_REACHED] > n_sec ? nc.K : ch.cap.K;
+        ch.pz.caps[0] = std::min(256, cnt[2 + RPD_SECURITY_RADIUS_NOT_REACHED] > n_sec ? nc.K : ch.cap.K);
         ch.pr.caps[1] = ch.pz.caps[1] = nc.P;
         ch.pr.caps[2] = ch.pz.caps[2] = nc.V;
         ch.pr.caps[3] = ch.pz.caps[3] = nc.T;

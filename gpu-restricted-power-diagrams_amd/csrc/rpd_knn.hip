@@ -251,18 +251,19 @@ __device__ __forceinline__ R dist2_unfused(const typename Vec4T<R>::T &a, const 
 }
 
 constexpr int kKnnWavesPerBlock = 4;
+// candidate slots of one wave: the sorted-array indices of the seeds of up to
+// 64 row segments, expanded in pieces of this many
+constexpr int kCandBuf = 256;
 
 template <typename R, int S>
 __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> a) {
     using KeyT = Key<R>;
-    __shared__ int seg_off_all[kKnnWavesPerBlock][64];
-    __shared__ int seg_a_all[kKnnWavesPerBlock][64];
+    __shared__ int cand_all[kKnnWavesPerBlock][kCandBuf];
     const int lane = lane_id();
     const int wv = uni(int(threadIdx.x >> 6));  // wave-uniform: lets ids, seeds and pointers live in SGPRs
     const int64_t qi = int64_t(blockIdx.x) * kKnnWavesPerBlock + wv;
     if (qi >= a.n_ids) return;  // wave-uniform
-    int *seg_off = seg_off_all[wv];
-    int *seg_a = seg_a_all[wv];
+    int *cand = cand_all[wv];
     const int q = a.ids ? a.ids[qi] : int(qi);
     const typename Vec4T<R>::T p = a.pts[q];
     const int dim = a.dim;
@@ -389,32 +390,30 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
             const int len = sb - sa;
             const int off = wave_excl_scan(len);
             const int total = uni(rlane(off + len, 63));
-            seg_off[lane] = off;
-            seg_a[lane] = sa;
-            wave_lds_sync();
-            for (int base = 0; base < total; base += 64) {
-                const int g = base + lane;
-                KeyT c = KeyT::inf();
-                if (g < total) {
-                    // last segment whose offset is <= g (empty segments never win)
-                    int lo = 0, hi = 63;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (seg_off[mid] <= g)
-                            lo = mid;
-                        else
-                            hi = mid - 1;
+            // The segments' seeds, in segment order, are candidates off ..
+            // off + len - 1 of the chunk.  Each lane writes the indices of
+            // its segment's part of the current piece into LDS, then every
+            // batch of 64 reads one index per lane.
+            for (int pbase = 0; pbase < total; pbase += kCandBuf) {
+                const int lo = max(off, pbase), hi = min(off + len, pbase + kCandBuf);
+                for (int gi = lo; gi < hi; ++gi) cand[gi - pbase] = sa + (gi - off);
+                wave_lds_sync();
+                const int nhere = min(total - pbase, kCandBuf);
+                for (int base = 0; base < nhere; base += 64) {
+                    const int g = base + lane;
+                    KeyT c = KeyT::inf();
+                    if (g < nhere) {
+                        const int idx = cand[g];
+                        if (idx != q) {
+                            const typename Vec4T<R>::T cp = a.pts[idx];
+                            const R d2 = dist2_unfused<R>(p, cp);
+                            c = KeyT::make(d2, uint32_t(a.perm[idx]), uint32_t(idx));
+                        }
                     }
-                    const int idx = seg_a[lo] + (g - seg_off[lo]);
-                    if (idx != q) {
-                        const typename Vec4T<R>::T cp = a.pts[idx];
-                        const R d2 = dist2_unfused<R>(p, cp);
-                        c = KeyT::make(d2, uint32_t(a.perm[idx]), uint32_t(idx));
-                    }
+                    merge_batch(c);
                 }
-                merge_batch(c);
+                wave_lds_sync();  // the next piece overwrites cand
             }
-            wave_lds_sync();
         }
         // stop once the K-th key is closer than anything in the next shell
         if (!kth.is_inf()) {

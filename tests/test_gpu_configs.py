@@ -127,7 +127,7 @@ def test_1m_power_torus_per_cell(rpd, orc, ctx):
     vol = grid.volume()
     assert abs(f64[:, 3].sum() - vol) / vol < 1e-11
     assert abs(mix[:, 3].sum() - vol) / vol < 1e-11
-    assert (s64 == 4).all() and (smx == 4).all()
+    assert set(np.unique(s64)) <= {4, 6} and np.array_equal(s64, smx)
     # the cells near the domain, where the volume is: every cell that has one
     # and a stride over the rest
     near = np.nonzero(f64[:, 3] != 0)[0]

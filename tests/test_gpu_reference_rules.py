@@ -56,15 +56,17 @@ def test_reference_apex_on_mesh_domains(rpd, orc, ctx, mesh, prec, tol):
 
 @pytest.mark.parametrize("shape,g,prm", [(0, 32, (0.5, 0.5, 0.5, 0.45)), (1, 40, (0.5, 0.5, 0.5, 0.3, 0.12))])
 def test_reference_apex_on_voxel_domains(rpd, orc, ctx, shape, g, prm):
-    """FP64 form.  The cells that the reference rule completes (success,
-    finite moments) in the oracle and on the GPU agree per cell; the two
-    fail on the same cells but for a handful whose flat cones the two
-    arithmetics resolve differently."""
+    """FP64 form.  The reference rule fails on a few percent of the cells of
+    a voxel domain, in the oracle and on the GPU; which cells fail depends
+    on how each arithmetic resolves the flat cones.  The cells both
+    complete (success, finite moments) agree per cell, and the cells the
+    GPU completes equal the oracle's with apex_mode 1."""
     n = 20000
     h = (1.0 / n) ** (1 / 3)
     x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=2, wmax=0.25 * h * h * 1e6)
     grid = rpd.Grid.shape(shape, g, prm)
     ref, ref_st, _, _ = orc.solve(x, oracle_grid(orc, grid), K=64, P=40, V=60, T=100, apex_mode=0, frame=1)
+    ref1, ref1_st, _, _ = orc.solve(x, oracle_grid(orc, grid), K=64, P=40, V=60, T=100, apex_mode=1, frame=1)
     dom = ctx.domain(grid)
     ctx.set_seeds(x)
     st = ctx.solve(dom, K=64, P=40, V=60, T=100, precision=rpd.PREC_FP64, apex_mode=0, cull=0)
@@ -78,13 +80,19 @@ def test_reference_apex_on_voxel_domains(rpd, orc, ctx, shape, g, prm):
     print("apex 0 on voxels", shape, g, "oracle fails", fail_ref, "gpu fails", fail_got, "differ", only,
           st["status_hist"], flush=True)
     # the reference rule fails on a few percent of the cells of a voxel domain
-    assert 0 < fail_ref < n // 10
-    assert only <= max(5, fail_ref // 20)
+    assert 0 < fail_ref < n // 10 and 0 < fail_got < n // 10
     cv, cb, nc = cell_errors(got[both], ref[both], n)
     ev, em = moment_errors(got[both], ref[both], n)
-    print("  completed cells: %d, per cell %.2e %.2e, of mean %.2e %.2e" % (nc, cv, cb, ev, em), flush=True)
+    print("  completed by both: %d, per cell %.2e %.2e, of mean %.2e %.2e" % (nc, cv, cb, ev, em), flush=True)
     assert ev <= 1e-12 and em <= 1e-12
     assert cv <= 1e-12 and cb <= 1e-12
+    # what the GPU completes is the diagram (any valid apex gives it)
+    assert (ref1_st == 4).all() or set(np.unique(ref1_st)) <= {4, 6}
+    c1v, c1b, n1 = cell_errors(got[ok_got], ref1[ok_got], n)
+    e1v, e1m = moment_errors(got[ok_got], ref1[ok_got], n)
+    print("  GPU-completed vs apex_mode 1: %d, per cell %.2e %.2e, of mean %.2e %.2e" % (n1, c1v, c1b, e1v, e1m),
+          flush=True)
+    assert e1v <= 1e-12 and e1m <= 1e-12 and c1v <= 1e-12 and c1b <= 1e-12
 
 
 def _walls(rpd, n, seed, frac):

@@ -50,7 +50,7 @@ def test_sphere_domain(rpd, orc, ctx, prec, tol):
     print("sphere", prec, st, ev, em, cv, cb, nc, flush=True)
     assert (status == 4).all()
     assert ev <= tol and em <= tol
-    assert cv <= tol and cb <= tol and nc > n // 2
+    assert cv <= tol and cb <= tol and nc > n // 3
     vol = grid.volume()
     assert abs(got[:, 3].sum() - vol) / vol < (1e-12 if prec == 0 else 1e-7)
 
