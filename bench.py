@@ -78,6 +78,9 @@ def measured_traffic(kernel):
 
 
 def rel_errors(got, ref, n):
+    """Errors of moments got against ref: of the mean cell volume (all cells),
+    and per cell on the cells above 1% of the mean cell volume (relative
+    volume, barycentre displacement over the cell's size)."""
     v = 1e9 / n
     ev = float(np.abs(got[:, 3] - ref[:, 3]).max() / v)
     em = float(np.abs(got[:, :3] - ref[:, :3]).max() / (v * 1000.0))
@@ -90,7 +93,18 @@ def rel_errors(got, ref, n):
     else:
         rv = rb = 0.0
     return {"vol_over_mean_cell": ev, "moment_over_mean_cell_x1000": em, "vol_rel_cells_gt_1pct": rv,
-            "bary_over_cell_size": rb}
+            "bary_over_cell_size": rb, "cells_gt_1pct": int(big.sum())}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def serving(rpd, ctx, n_req, log):
@@ -290,6 +304,22 @@ def main():
     err_mixed_vs_fp64 = rel_errors(res["mixed"]["out"], res["fp64"]["out"], n)
     log("bench: mixed vs fp64", err_mixed_vs_fp64)
 
+    # host buffers on both sides (rpd_set_seeds from host memory, moments
+    # read back to host memory): the PCIe-inclusive rate of the mixed form
+    host_out = np.empty((n, 4), np.float64)
+    pcie_ms = []
+    for i in range(max(1, min(args.steps, 10))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.set_seeds(seeds)
+        ctx.solve(dom, K=K, P=P, V=V, T=T, precision=rpd.PREC_MIXED)
+        host_out[:] = ctx.moments()
+        pcie_ms.append((time.perf_counter() - t0) * 1e3)
+    pcie = {"cells_per_s": n / (float(np.median(pcie_ms)) * 1e-3), "ms_per_step": float(np.median(pcie_ms)),
+            "what": "seeds copied from host memory, solve, moments copied to host memory (median of %d)" %
+                    len(pcie_ms)}
+    log("bench: PCIe-inclusive", pcie)
+
     # ---- CPU oracle on a bounded sample of the same workload ----------------
     cpu = None
     flop = None
@@ -301,19 +331,41 @@ def main():
         stride = max(1, n // args.cpu_sample)
         ids = np.arange(0, n, stride, dtype=np.int32)[: args.cpu_sample]
         avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        threads = max(1, min(avail, int(os.environ.get("OMP_NUM_THREADS", "16"))))
-        t0 = time.perf_counter()
-        ref, ref_st, _, ost = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads)
-        t1 = time.perf_counter()
-        cpu = {"value": float(ids.size / (t1 - t0)), "unit": "cells/s", "cores": int(threads), "kind": "port",
+        threads = max(1, min(avail, int(os.environ.get("OMP_NUM_THREADS", str(avail)))))
+
+        def timed(sel, **kw):
+            t0 = time.perf_counter()
+            out = orc.solve_ids(seeds, sel, og, K=K, P=P, V=V, T=T, **kw)
+            return out, time.perf_counter() - t0
+
+        # the reference's algorithm (every cell clipped, reference frame), FP64
+        (ref, ref_st, _, ost), dt = timed(ids, threads=threads)
+        # the same on one core, on every 16th cell of the sample
+        ids1 = ids[::16].copy()
+        _, dt1 = timed(ids1, threads=1)
+        # the same with the library's culls (cells provably outside the domain
+        # get volume 0 without being clipped): the algorithmic part of the gain
+        _, dtc = timed(ids, threads=threads, cull=1)
+        model = cpu_model()
+        cpu = {"value": float(ids.size / dt), "unit": "cells/s", "cores": int(threads), "kind": "port",
                "sample": "%d cells (every %d-th seed) of the 1M-seed torus workload, all seeds as neighbours: "
                          "binning + kNN + clip + every re-queue pass of the FP64 (USE_DOUBLE) oracle, the "
-                         "reference's algorithm (every cell clipped), on %d threads; "
-                         "%.1f s" % (ids.size, stride, threads, t1 - t0)}
+                         "reference's algorithm (every cell clipped), on %d threads of %s; "
+                         "%.1f s" % (ids.size, stride, threads, model, dt),
+               "cpu_model": model, "host_threads_available": int(avail), "host_cpus": int(os.cpu_count() or 0),
+               "one_core": {"value": float(ids1.size / dt1), "cells": int(ids1.size), "seconds": dt1},
+               "with_culls": {"value": float(ids.size / dtc), "cores": int(threads), "seconds": dtc}}
         log("bench: cpu oracle", cpu)
         _, _, _, ost1 = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads, single_pass=1, cull=1)
         flop = {"mixed": kernel_flops(ost1, ids.size, True), "fp64": kernel_flops(ost1, ids.size, False)}
-        err_vs_oracle = {k: rel_errors(res[k]["out"][ids], ref, n) for k in ("mixed", "fp64")}
+        # parity: the FP64 oracle evaluated in each seed's frame, as the kernels
+        # clip and integrate (the reference-frame oracle's own rounding error
+        # reaches 1e-12 per cell, reported beside it)
+        ref_seed, _, _, _ = orc.solve_ids(seeds, ids, og, K=K, P=P, V=V, T=T, threads=threads, frame=1)
+        err_vs_oracle = {k: rel_errors(res[k]["out"][ids], ref_seed, n) for k in ("mixed", "fp64")}
+        err_vs_oracle["reference_frame_fp64"] = rel_errors(res["fp64"]["out"][ids], ref, n)
+        err_vs_oracle["oracle"] = ("FP64 oracle in each seed's frame; reference_frame_fp64: the FP64 form against "
+                                   "the oracle in the reference's [0,1000]^3 frame")
         log("bench: vs oracle", err_vs_oracle)
 
     # ---- rooflines: the dominant kernel (pass-1 finish kernel, mixed form),
@@ -365,6 +417,7 @@ def main():
         "fp64_ms_per_step": ms_fp64,
         "mixed_over_fp64": ms_fp64 / ms_mixed,
         "err_mixed_vs_fp64": err_mixed_vs_fp64,
+        "pcie_inclusive": pcie,
         "err_vs_oracle_sample": err_vs_oracle,
         "fallback_fraction": res["mixed"]["stats"][-1]["fallback_cells"] / n,
         "status_hist_mixed": res["mixed"]["stats"][-1]["status_hist"],
