@@ -122,7 +122,31 @@ struct rpd_ctx {
     double wmin = 0, wall = 0;
 };
 
+// ---- working memory of rpd_knn_device_noalloc -----------------------------
+// A file-scope device array: every GPU the library runs on holds its own copy
+// of it until the process ends.  Host code must never pass `scratch` itself
+// to a kernel or a copy (in host code the name is a stand-in, not a device
+// address): the address of each device's copy is resolved once, with that
+// device current, by rpd_create, and kept in g_scratch_ptr.
+constexpr size_t kNoAllocScratchBytes = size_t(24) << 20;
+constexpr int64_t kNoAllocMaxSeeds = int64_t(1) << 18;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
 namespace {
+
+// Resolves the current device's copy of `scratch` once (device current).
+int resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return RPD_E_DEVICE;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return RPD_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return RPD_E_DEVICE;
+    g_scratch_ptr[device] = p;
+    return RPD_OK;
+}
 
 int set_err(rpd_ctx *c, int code, const char *fmt, ...) {
     if (c) {
@@ -341,6 +365,7 @@ int rpd_create(int device, rpd_ctx **out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RPD_E_DEVICE;
     if (device < 0 || device >= ndev) return RPD_E_ARG;
     if (hipSetDevice(device) != hipSuccess) return RPD_E_DEVICE;
+    if (resolve_scratch(device) != RPD_OK) return RPD_E_DEVICE;
     auto *c = new rpd_ctx();
     c->device = device;
     for (auto &s : c->streams)
@@ -566,6 +591,41 @@ int rpd_knn(rpd_ctx *c, int32_t K, int32_t prec, int32_t *out_nk) {
                           c->n, K, c->knn.as<int>()),
            "knn launch");
     HIPCHK(c, hipMemcpyAsync(out_nk, c->knn.ptr, size_t(c->n) * K * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RPD_OK;
+}
+
+int rpd_knn_device_noalloc(rpd_ctx *c, const double *d_xyzw, int64_t n, int32_t K, int32_t prec,
+                           int32_t *d_perm, int32_t *d_out_nk) {
+    if (!c || !d_xyzw || !d_perm || !d_out_nk || n < 2 || K <= 0 || (prec != 0 && prec != 1))
+        return set_err(c, RPD_E_ARG, "rpd_knn_device_noalloc: bad arguments");
+    if (K > 256) return set_err(c, RPD_E_LIMIT, "rpd_knn_device_noalloc: K > 256");
+    if (n > kNoAllocMaxSeeds)
+        return set_err(c, RPD_E_LIMIT, "rpd_knn_device_noalloc: at most %lld seeds", (long long)kNoAllocMaxSeeds);
+    HIPCHK(c, hipSetDevice(c->device));
+    char *base = static_cast<char *>(g_scratch_ptr[c->device]);
+    if (!base) return set_err(c, RPD_E_STATE, "rpd_knn_device_noalloc: scratch not resolved");
+    const int dim = rpd_knn_dim(n);
+    const int64_t nc = int64_t(dim) * dim * dim;
+    // scratch: counters (dim^3 + 1) | sorted seeds (FP32 or FP64) | binning
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t off_stored = up(size_t(nc + 1) * 4);
+    const size_t off_bin = off_stored + up(size_t(n) * (prec == 0 ? 16 : 32));
+    const size_t bin_bytes = rpd_bin_scratch_bytes(c->stream, n, dim);
+    if (bin_bytes == 0) return set_err(c, RPD_E_DEVICE, "rpd_knn_device_noalloc: scratch size query");
+    if (off_bin + bin_bytes > kNoAllocScratchBytes)
+        return set_err(c, RPD_E_LIMIT, "rpd_knn_device_noalloc: %zu scratch bytes needed, %zu available",
+                       off_bin + bin_bytes, kNoAllocScratchBytes);
+    int *counters = reinterpret_cast<int *>(base);
+    void *stored = base + off_stored;
+    // the crowded-cell flag comes back through the context's pinned words
+    RPDCHK(c,
+           rpd_bin_seeds_in(c->stream, reinterpret_cast<const double4 *>(d_xyzw), n, prec, dim, base + off_bin,
+                            kNoAllocScratchBytes - off_bin, c->pin + kPinCounters - 2, d_perm, counters, stored,
+                            nullptr),
+           "rpd_knn_device_noalloc: binning");
+    RPDCHK(c, rpd_launch_knn(c->stream, prec, stored, d_perm, counters, dim, nullptr, n, K, d_out_nk),
+           "rpd_knn_device_noalloc: knn launch");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return RPD_OK;
 }

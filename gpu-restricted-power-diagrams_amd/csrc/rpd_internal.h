@@ -71,6 +71,12 @@ struct DevBuf {
 int rpd_knn_dim(int64_t n);
 int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch, int *perm,
                   int *counters, void *stored, double4 *stored64);
+// The same binning in caller-given scratch (scratch_bytes of it; RPD_E_LIMIT
+// when rpd_bin_scratch_bytes needs more); flag_host receives the
+// crowded-cell flag (pinned host memory for an asynchronous copy).
+int rpd_bin_seeds_in(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, void *scratch,
+                     size_t scratch_bytes, int *flag_host, int *perm, int *counters, void *stored, double4 *stored64);
+size_t rpd_bin_scratch_bytes(hipStream_t st, int64_t n, int dim);
 // K nearest of each query (stored ids ids[0..n_ids), nullptr: all), or, with
 // lower set, the K after the neighbour lower[q] (stored-id indexed).
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,

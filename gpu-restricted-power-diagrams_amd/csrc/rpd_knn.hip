@@ -451,11 +451,38 @@ int rpd_knn_dim(int64_t n) {
 // a counting sort whose cells are then sorted by index, or, when a cell holds
 // more than kSortRun seeds, a radix sort of (cell, index).  Waits for the
 // stream.
+// Bytes of binning scratch for n seeds on a dim^3 grid (0 on a HIP error).
+// Layout: cell_in, cell_out, rank / orig (n ints each), counts (dim^3 + 1),
+// the crowded-cell flag, rocPRIM temporary storage.
+size_t rpd_bin_scratch_bytes(hipStream_t st, int64_t n, int dim) {
+    const int64_t nc = int64_t(dim) * dim * dim;
+    size_t sort_bytes = 0, scan_bytes = 0;
+    int *null_i = nullptr;
+    unsigned end_bit = 1;
+    while ((int64_t(1) << end_bit) < nc) ++end_bit;
+    if (rocprim::radix_sort_pairs(nullptr, sort_bytes, null_i, null_i, null_i, null_i, n, 0u, end_bit, st) !=
+        hipSuccess)
+        return 0;
+    if (rocprim::exclusive_scan(nullptr, scan_bytes, null_i, null_i, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
+        hipSuccess)
+        return 0;
+    const size_t off_flag = (3 * size_t(n) * 4 + size_t(nc + 1) * 4 + 255) & ~size_t(255);
+    return off_flag + 256 + std::max(sort_bytes, scan_bytes) + 256;
+}
+
 int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch, int *perm,
                   int *counters, void *stored, double4 *stored64) {
+    const size_t need = rpd_bin_scratch_bytes(st, n, dim);
+    if (need == 0) return RPD_E_DEVICE;
+    if (scratch.ensure(need) != RPD_OK) return RPD_E_NOMEM;
+    int flag_host = 0;
+    return rpd_bin_seeds_in(st, seeds, n, prec, dim, scratch.ptr, need, &flag_host, perm, counters, stored, stored64);
+}
+
+int rpd_bin_seeds_in(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, void *scratch_ptr,
+                     size_t scratch_bytes, int *flag_host, int *perm, int *counters, void *stored, double4 *stored64) {
     const int64_t nc = int64_t(dim) * dim * dim;
     const size_t bytes_cell = size_t(n) * 4;
-    // scratch layout: cell_in, cell_out, rank / orig, counts(nc+1), flag, rocprim temp
     size_t sort_bytes = 0, scan_bytes = 0;
     int *null_i = nullptr;
     unsigned end_bit = 1;
@@ -470,8 +497,8 @@ int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int
     const size_t off_flag = (off_counts + size_t(nc + 1) * 4 + 255) & ~size_t(255);
     const size_t off_tmp = off_flag + 256;
     const size_t need = off_tmp + std::max(sort_bytes, scan_bytes) + 256;
-    if (scratch.ensure(need) != RPD_OK) return RPD_E_NOMEM;
-    char *base = static_cast<char *>(scratch.ptr);
+    if (need > scratch_bytes) return RPD_E_LIMIT;
+    char *base = static_cast<char *>(scratch_ptr);
     int *cell_in = reinterpret_cast<int *>(base);
     int *cell_out = reinterpret_cast<int *>(base + off_cell_out);
     int *rank = reinterpret_cast<int *>(base + off_rank);
@@ -501,11 +528,10 @@ int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int
     };
     // gathered at once; a crowded cell (rare) sorts and gathers again
     gather();
-    int big_h = 0;
-    if (hipMemcpyAsync(&big_h, big, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+    if (hipMemcpyAsync(flag_host, big, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
         return RPD_E_DEVICE;
-    if (big_h) {
+    if (*flag_host) {
         iota_kernel<<<nb, tb, 0, st>>>(rank, n);
         size_t sb = sort_bytes;
         if (rocprim::radix_sort_pairs(tmp, sb, cell_in, cell_out, rank, perm, n, 0u, end_bit, st) != hipSuccess)

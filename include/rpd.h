@@ -282,6 +282,21 @@ int rpd_hessian_device(rpd_ctx *ctx, const double **vals, const int32_t **ids, i
    knn_precision 0: FP32 seed copy, 1: FP64. out: n*K int32, stored ids. */
 int rpd_knn(rpd_ctx *ctx, int32_t K, int32_t knn_precision, int32_t *out_nk);
 
+/* rpd_knn over device buffers, allocating nothing.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The same binning and kNN kernels as rpd_knn, in the same
+   order: d_perm[n] receives the binning permutation (stored -> original,
+   as rpd_read_perm), d_out_nk[n*K] the K nearest of every seed in stored
+   order, ascending (d^2, original index), -1 padded.  d_xyzw: n seeds in
+   HBM (FP64 xyzw, as rpd_set_seeds_device).  The working memory is a
+   24 MiB device array of the library, one per GPU, held until the process
+   ends: n <= 262144 and K <= 256, else RPD_E_LIMIT.  Every call on one GPU
+   shares that GPU's array: calls that overlap on one GPU race.  Does not
+   change the context's seeds or results; waits for its stream. */
+int rpd_knn_device_noalloc(rpd_ctx *ctx, const double *d_xyzw, int64_t n, int32_t K, int32_t knn_precision,
+                           int32_t *d_perm, int32_t *d_out_nk);
+
 /* Compaction of used cells (compressResult, voronoi.cpp:513-691) in a
    deterministic order, on the device (a scan of the keep flags, then a
    scatter): cells with m >= keep_volume, ascending original index.
