@@ -135,6 +135,20 @@ struct Key<float> {
     // key of lane - 1 (lane 0: inf)
     __device__ Key up_shfl() const { return Key{shr_lane1(k, ~0ull), shr_lane1(s, 0xffffffffu)}; }
     __device__ Key lane(int l) const { return Key{rlane(k, l), rlane(s, l)}; }
+    // the ordering key as 32-bit words, most significant first (radix select)
+    static constexpr int kWords = 2;
+    __device__ uint32_t word(int w) const { return w == 0 ? uint32_t(k >> 32) : uint32_t(k); }
+    // candidate buffer in LDS: ordering keys and stored indices
+    struct Buf {
+        uint64_t *k;
+        uint32_t *s;
+    };
+    __device__ void put(const Buf &b, int i) const {
+        b.k[i] = k;
+        b.s[i] = s;
+    }
+    __device__ static Key get(const Buf &b, int i) { return Key{b.k[i], b.s[i]}; }
+    static constexpr int kBufBytes = 12;  // per entry
 };
 
 template <>
@@ -155,6 +169,19 @@ struct Key<double> {
         return Key{shr_lane1(d, ~0ull), shr_lane1(o, 0xffffffffu), shr_lane1(s, 0xffffffffu)};
     }
     __device__ Key lane(int l) const { return Key{rlane(d, l), rlane(o, l), rlane(s, l)}; }
+    static constexpr int kWords = 3;
+    __device__ uint32_t word(int w) const { return w == 0 ? uint32_t(d >> 32) : (w == 1 ? uint32_t(d) : o); }
+    struct Buf {
+        uint64_t *d;
+        uint32_t *o, *s;
+    };
+    __device__ void put(const Buf &b, int i) const {
+        b.d[i] = d;
+        b.o[i] = o;
+        b.s[i] = s;
+    }
+    __device__ static Key get(const Buf &b, int i) { return Key{b.d[i], b.o[i], b.s[i]}; }
+    static constexpr int kBufBytes = 16;
 };
 
 template <typename K>
@@ -254,16 +281,108 @@ constexpr int kKnnWavesPerBlock = 4;
 // candidate slots of one wave: the sorted-array indices of the seeds of up to
 // 64 row segments, expanded in pieces of this many
 constexpr int kCandBuf = 256;
+// kept candidates of one wave (keys below the running K-th key): room for
+// the K survivors of a selection plus one batch
+template <int S>
+constexpr int kKeep = S == 1 ? 256 : (S == 2 ? 384 : 512);
+
+// Buffer index of the key of rank `rank` (0-based) among the M (distinct)
+// keys of the buffer, by a radix select over 8-bit digits of the key words,
+// most significant first: a 256-bin histogram of the digit of the keys still
+// in play, its prefix sums across the lanes, the bin that holds the rank.
+// Stops as soon as the chosen bin holds one key.  Lane l owns buffer entries
+// l, l + 64, ... (SL of them); hist: 256 words of LDS.
+template <typename KeyT, int SL>
+__device__ int select_rank(const typename KeyT::Buf &buf, int M, int rank, uint32_t *hist) {
+    const int lane = lane_id();
+    uint32_t w[SL][KeyT::kWords];
+    uint32_t alive = 0;
+#pragma unroll
+    for (int j = 0; j < SL; ++j) {
+        const int e = lane + 64 * j;
+        if (e < M) {
+            const KeyT k = KeyT::get(buf, e);
+#pragma unroll
+            for (int q = 0; q < KeyT::kWords; ++q) w[j][q] = k.word(q);
+            alive |= 1u << j;
+        } else {
+#pragma unroll
+            for (int q = 0; q < KeyT::kWords; ++q) w[j][q] = 0;
+        }
+    }
+    uint4 *h4 = reinterpret_cast<uint4 *>(hist);
+#pragma unroll 1
+    for (int pass = 0; pass < 4 * KeyT::kWords; ++pass) {
+        const int q = pass >> 2, sh = 24 - 8 * (pass & 3);
+        h4[lane] = make_uint4(0u, 0u, 0u, 0u);
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < SL; ++j)
+            if ((alive >> j) & 1) atomicAdd(&hist[(w[j][q] >> sh) & 255u], 1u);
+        wave_lds_sync();
+        const uint4 h = h4[lane];
+        const int sum = int(h.x + h.y + h.z + h.w);
+        const int incl = wave_incl_scan_op(sum, 0, [](int a, int b) { return a + b; });
+        const int excl = incl - sum;
+        const int L = lowest_bit(ballot(excl <= rank && rank < incl));
+        // the bin of lane L's four that holds the rank
+        int before = rlane(excl, L);
+        const int c0 = int(rlane(h.x, L)), c1 = int(rlane(h.y, L)), c2 = int(rlane(h.z, L)), c3 = int(rlane(h.w, L));
+        int bin = 4 * L, cnt = c0;
+        if (rank >= before + c0) {
+            before += c0;
+            bin += 1;
+            cnt = c1;
+            if (rank >= before + c1) {
+                before += c1;
+                bin += 1;
+                cnt = c2;
+                if (rank >= before + c2) {
+                    before += c2;
+                    bin += 1;
+                    cnt = c3;
+                }
+            }
+        }
+        rank -= before;
+#pragma unroll
+        for (int j = 0; j < SL; ++j)
+            if (((w[j][q] >> sh) & 255u) != uint32_t(bin)) alive &= ~(1u << j);
+        wave_lds_sync();  // the histogram is cleared again by the next pass
+        if (cnt == 1) break;
+    }
+    // the one key left (keys are distinct, so one is left after every digit)
+    const int L = lowest_bit(ballot(alive != 0));
+    const int jj = __ffs(int(rlane(int(alive), L))) - 1;
+    return L + 64 * jj;
+}
 
 template <typename R, int S>
 __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> a) {
     using KeyT = Key<R>;
+    constexpr int KEEP = kKeep<S>;
+    constexpr int SL = KEEP / 64;
     __shared__ int cand_all[kKnnWavesPerBlock][kCandBuf];
+    __shared__ __attribute__((aligned(16))) uint32_t hist_all[kKnnWavesPerBlock][256];
+    __shared__ __attribute__((aligned(16))) char keep_all[kKnnWavesPerBlock][KEEP * KeyT::kBufBytes];
     const int lane = lane_id();
     const int wv = uni(int(threadIdx.x >> 6));  // wave-uniform: lets ids, seeds and pointers live in SGPRs
     const int64_t qi = int64_t(blockIdx.x) * kKnnWavesPerBlock + wv;
     if (qi >= a.n_ids) return;  // wave-uniform
     int *cand = cand_all[wv];
+    uint32_t *hist = hist_all[wv];
+    typename KeyT::Buf buf;
+    {
+        char *kb = keep_all[wv];
+        if constexpr (sizeof(R) == 4) {
+            buf.k = reinterpret_cast<uint64_t *>(kb);
+            buf.s = reinterpret_cast<uint32_t *>(kb + 8 * KEEP);
+        } else {
+            buf.d = reinterpret_cast<uint64_t *>(kb);
+            buf.o = reinterpret_cast<uint32_t *>(kb + 8 * KEEP);
+            buf.s = reinterpret_cast<uint32_t *>(kb + 12 * KEEP);
+        }
+    }
     const int q = a.ids ? a.ids[qi] : int(qi);
     const typename Vec4T<R>::T p = a.pts[q];
     const int dim = a.dim;
@@ -271,9 +390,10 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
     const int K = a.K;
     const double cs = 1000.0 / double(dim);
 
-    KeyT top[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) top[s] = KeyT::inf();
+    // Candidates whose key is below the running K-th key are kept in the
+    // buffer (M of them); once M reaches K, a selection finds the exact K-th
+    // key of the buffer and the buffer keeps only the K keys up to it.
+    int M = 0;
     KeyT kth = KeyT::inf();
     const bool has_low = a.lower != nullptr;
     KeyT klow = KeyT::inf();
@@ -281,82 +401,73 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
         const int L = a.lower[q];
         klow = KeyT::make(dist2_unfused<R>(p, a.pts[L]), uint32_t(a.perm[L]), uint32_t(L));
     }
-    const int kl = (K - 1) & 63, ks = (K - 1) >> 6;
-
-    auto refresh_kth = [&]() {
-        KeyT t = top[0];
+    // exact K-th key of the buffer, and the buffer cut to the K keys <= it
+    auto shrink = [&]() {
+        const int sel = select_rank<KeyT, SL>(buf, M, K - 1, hist);
+        kth = KeyT::get(buf, sel);
+        KeyT v[SL];
+        bool kp[SL];
 #pragma unroll
-        for (int s = 1; s < S; ++s)
-            if (s == ks) t = top[s];
-        kth = t.lane(kl);
-    };
-
-    auto insert_one = [&](const KeyT &c) {
-        int pos = 0;
-#pragma unroll
-        for (int s = 0; s < S; ++s) pos += __popcll(ballot(top[s] < c));
-        KeyT prev_last = KeyT::inf();
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            KeyT up = top[s].up_shfl();
-            if (lane == 0) up = prev_last;
-            prev_last = top[s].lane(63);
-            const int e = s * 64 + lane;
-            if (e == pos)
-                top[s] = c;
-            else if (e > pos)
-                top[s] = up;
+        for (int j = 0; j < SL; ++j) {
+            const int e = lane + 64 * j;
+            kp[j] = e < M;
+            v[j] = kp[j] ? KeyT::get(buf, e) : KeyT::inf();
+            kp[j] = kp[j] && !(kth < v[j]);
         }
+        wave_lds_sync();  // every entry is read before any is moved
+        int base = 0;
+#pragma unroll
+        for (int j = 0; j < SL; ++j) {
+            const uint64_t m = ballot(kp[j]);
+            if (kp[j]) v[j].put(buf, base + mbcnt(m));
+            base += __popcll(m);
+        }
+        M = base;  // == K
+        wave_lds_sync();
     };
-
-    auto merge_batch = [&](KeyT c) {
-        if (!(c < kth) || (has_low && !(klow < c))) c = KeyT::inf();
-        uint64_t m = ballot(!c.is_inf());
+    auto add_batch = [&](const KeyT &c) {
+        const bool keep = !c.is_inf() && c < kth && (!has_low || klow < c);
+        const uint64_t m = ballot(keep);
         if (m == 0) return;
-        if (__popcll(m) <= 6) {
-            while (m) {
-                const int l = lowest_bit(m);
-                m &= m - 1;
-                insert_one(c.lane(l));
-            }
-        } else {
-            KeyT cv[S];
-            cv[0] = c;
-#pragma unroll
-            for (int s = 1; s < S; ++s) cv[s] = KeyT::inf();
-            bitonic_sort<S>(cv, true);
-#pragma unroll
-            for (int s = 0; s < S; ++s) top[s] = kmin(top[s], cv[s]);
-            bitonic_merge_asc<S>(top);
-        }
-        refresh_kth();
+        if (M + 64 > KEEP) shrink();  // M >= K here: KEEP >= K + 64
+        const bool keep2 = keep && c < kth;
+        const uint64_t m2 = ballot(keep2);
+        if (keep2) c.put(buf, M + mbcnt(m2));
+        M += __popcll(m2);
+        wave_lds_sync();
     };
 
     // Cells whose box lies farther from the query than the current K-th
     // distance cannot hold a closer candidate, so from the second shell on,
     // rows and the x range of a row are cut to the ball of that radius (the
     // K-th key only decreases).  The bound is widened by a relative 1e-5 and
-    // the boxes by kCullPad, far above the rounding of d^2 and of the cell
-    // index, so no candidate at or below the K-th key is cut.
-    constexpr double kCullPad = 0.01;
-    const double px = double(p.x), py = double(p.y), pz = double(p.z);
-    auto box_gap = [&](int c, int cq, double v) -> double {
-        if (c < cq) return fmax(0.0, v - double(c + 1) * cs - kCullPad);
-        if (c > cq) return fmax(0.0, double(c) * cs - v - kCullPad);
-        return 0.0;
+    // the boxes by kCullPad, far above the rounding of d^2, of the FP32 box
+    // arithmetic (coordinates below 1000: ulp 6e-5) and of the cell index, so
+    // no candidate at or below the K-th key is cut.
+    constexpr float kCullPad = 0.01f;
+    const float px = float(p.x), py = float(p.y), pz = float(p.z);
+    const float csf = float(cs), inv_cs = float(1.0 / cs);
+    auto box_gap = [&](int c, int cq, float v) -> float {
+        if (c < cq) return fmaxf(0.0f, v - float(c + 1) * csf - kCullPad);
+        if (c > cq) return fmaxf(0.0f, float(c) * csf - v - kCullPad);
+        return 0.0f;
     };
     for (int r = 1;; ++r) {
         const int side = 2 * r + 1;
         const int nrows = side * side;
         const int nseg = 2 * nrows;
         const bool cull = r >= 2 && !kth.is_inf();
-        const double k2 = cull ? kth.dist2() * (1.0 + 1e-5) : 0.0;
+        const float k2 = cull ? float(kth.dist2() * (1.0 + 1e-5)) * (1.0f + 1e-6f) : 0.0f;
+        // row t of the shell: (t % side, t / side); t < 2 side^2, and the
+        // rounding of (t + 1/2) / side in FP32 stays far from an integer
+        const float inv_side = 1.0f / float(side);
         for (int chunk = 0; chunk < nseg; chunk += 64) {
             const int u = chunk + lane;
             int sa = 0, sb = 0;
             if (u < nseg) {
                 const int t = u >> 1, part = u & 1;
-                const int dy = t % side - r, dz = t / side - r;
+                const int tz = int((float(t) + 0.5f) * inv_side);
+                const int dy = t - tz * side - r, dz = tz - r;
                 const int y = cj + dy, z = ck + dz;
                 if (y >= 0 && y < dim && z >= 0 && z < dim) {
                     // the first shell (r == 1) is the whole 3x3x3 block
@@ -369,15 +480,15 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
                         xlo = xhi = part == 0 ? ci - r : ci + r;
                     }
                     if (cull) {
-                        const double gy = box_gap(y, cj, py), gz = box_gap(z, ck, pz);
-                        const double rem = k2 - gy * gy - gz * gz;
-                        if (rem < 0.0) {
+                        const float gy = box_gap(y, cj, py), gz = box_gap(z, ck, pz);
+                        const float rem = k2 - gy * gy - gz * gz;
+                        if (rem < 0.0f) {
                             xlo = 1;
                             xhi = 0;
                         } else {
-                            const double w = sqrt(rem) + kCullPad;
-                            xlo = max(xlo, int(floor((px - w) / cs)));
-                            xhi = min(xhi, int(floor((px + w) / cs)));
+                            const float w = sqrtf(rem) + kCullPad;
+                            xlo = max(xlo, int(floorf((px - w) * inv_cs)));
+                            xhi = min(xhi, int(floorf((px + w) * inv_cs)));
                         }
                     }
                     if (xlo >= 0 && xhi < dim && xlo <= xhi) {
@@ -410,11 +521,13 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
                             c = KeyT::make(d2, uint32_t(a.perm[idx]), uint32_t(idx));
                         }
                     }
-                    merge_batch(c);
+                    add_batch(c);
                 }
                 wave_lds_sync();  // the next piece overwrites cand
             }
         }
+        // the exact K-th key of the candidates so far
+        if (M > K || (M == K && kth.is_inf())) shrink();
         // stop once the K-th key is closer than anything in the next shell
         if (!kth.is_inf()) {
             const double lb = double(r) * cs * (1.0 - 1e-4);
@@ -423,6 +536,14 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
         const int rn = r + 1;
         if (ci - rn < 0 && ci + rn >= dim && cj - rn < 0 && cj + rn >= dim && ck - rn < 0 && ck + rn >= dim) break;
     }
+    // the (at most K) kept keys in ascending order: one sort
+    KeyT top[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int e = s * 64 + lane;
+        top[s] = e < M ? KeyT::get(buf, e) : KeyT::inf();
+    }
+    bitonic_sort<S>(top, false);
     int *out = a.out + qi * int64_t(K);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
