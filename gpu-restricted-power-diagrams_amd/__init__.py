@@ -133,7 +133,7 @@ class Stats(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_inflight", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("max_batch", ctypes.c_int32)]
 
 
 _lib = None
@@ -174,6 +174,9 @@ def lib():
         "rpd_set_seeds_device": (ctypes.c_int, [VP, VP, I64]),
         "rpd_solve": (ctypes.c_int, [VP, VP, ctypes.POINTER(Params), ctypes.POINTER(Stats)]),
         "rpd_read_moments": (ctypes.c_int, [VP, VP]),
+        "rpd_solve_batch": (ctypes.c_int, [VP, I32, VP, VP, VP, ctypes.POINTER(Params), ctypes.POINTER(Stats)]),
+        "rpd_batch_range": (ctypes.c_int, [VP, I32, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+        "rpd_read_moments_range": (ctypes.c_int, [VP, I64, I64, VP]),
         "rpd_read_status": (ctypes.c_int, [VP, VP]),
         "rpd_read_perm": (ctypes.c_int, [VP, VP]),
         "rpd_moments_device": (ctypes.c_int, [VP, ctypes.POINTER(VP)]),
@@ -405,6 +408,34 @@ class Context:
         self._hess_shape = (int(H), int(HC)) if H > 0 else None
         return st.as_dict()
 
+    def solve_batch(self, requests, K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, apex_mode=1,
+                    security_mode=0, cull=1, debug_counters=0):
+        """rpd_solve_batch over requests [(domain or None, xyzw[n, 4]), ...]:
+        one packed solve; returns (stats, [moments[n_r, 4] per request]).
+        Afterwards moments() / status() hold every request's cells back to
+        back."""
+        xs = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 4) for _, x in requests]
+        nr = len(xs)
+        doms = (VP * nr)(*[d._h.value if d is not None else None for d, _ in requests])
+        ptrs = (VP * nr)(*[x.ctypes.data for x in xs])
+        ns = (I64 * nr)(*[x.shape[0] for x in xs])
+        p = Params(int(K), int(P), int(V), int(T), int(precision), int(max_extra_steps), int(apex_mode),
+                   int(security_mode), int(cull), 0, 1, int(debug_counters))
+        st = Stats()
+        self.check(lib().rpd_solve_batch(self._h, nr, doms, ptrs, ns, ctypes.byref(p), ctypes.byref(st)),
+                   "rpd_solve_batch")
+        self.n = int(sum(x.shape[0] for x in xs))
+        self._hess_shape = None
+        outs = []
+        for r in range(nr):
+            first, count = I64(0), I64(0)
+            self.check(lib().rpd_batch_range(self._h, r, ctypes.byref(first), ctypes.byref(count)), "rpd_batch_range")
+            o = np.zeros((count.value, 4), np.float64)
+            self.check(lib().rpd_read_moments_range(self._h, first.value, count.value, _p(o)),
+                       "rpd_read_moments_range")
+            outs.append(o)
+        return st.as_dict(), outs
+
     def hessian(self):
         """(vals[n, H, HC], ids[n, H]) of the last solve with H > 0: per
         original seed the neighbours (original indices, -1 after the last)
@@ -490,11 +521,13 @@ def params(K=25, P=35, V=40, T=100, precision=PREC_MIXED, max_extra_steps=5, ape
 
 class Server:
     """Request loop for streams of solves (rpd_server_*): submit requests of
-    any size against resident domains, poll finished ones with their latency."""
+    any size against resident domains, poll finished ones with their latency.
+    Queued requests run as packed batches (one launch per kernel for all of
+    them), max_inflight batches at once, at most max_batch requests each."""
 
-    def __init__(self, ctx, hbm_budget_bytes=0, max_inflight=4):
+    def __init__(self, ctx, hbm_budget_bytes=0, max_inflight=2, max_batch=256):
         h = ctypes.c_void_p()
-        cfg = ServerConfig(int(hbm_budget_bytes), int(max_inflight), 0)
+        cfg = ServerConfig(int(hbm_budget_bytes), int(max_inflight), int(max_batch))
         _check(lib().rpd_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)), None, "rpd_server_create")
         self._h = h
         self._n = {}

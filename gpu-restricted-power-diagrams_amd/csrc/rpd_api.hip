@@ -102,6 +102,19 @@ struct rpd_ctx {
     bool dcnt_ok = false;
     int64_t ccount = 0;  // cells in cout / cperm after the last compaction
     int last_K = 0, last_P = 0, last_V = 0, last_sm = 0, last_steps = 0;
+    // packed batch set by rpd_solve_batch (empty: the seeds are one
+    // diagram): per request its domain, stored-id range, weight minima, kNN
+    // grid size and counters offset; on the device the per-request fields
+    // of the clip kernels and of the kNN, and the request of each stored id
+    struct BatchReq {
+        rpd_domain *dom = nullptr;
+        int64_t base = 0, n = 0;
+        double wmin = 0, wall = 0;
+        int dim = 0;
+        int64_t coff = 0;
+    };
+    std::vector<BatchReq> batch;
+    DevBuf req_desc, knn_req, req_of;
     // pass work buffers: up to three passes may run at once, one per stream
     PassBufs pb[4];
     // pass 1 writes lists 0 (bisector kernel) and 1 (finish kernel); the
@@ -569,6 +582,7 @@ static int set_seeds_common(rpd_ctx *c, const double *src, int64_t n, hipMemcpyK
     c->bin_prec = -1;
     c->wmin_ok = false;
     c->solved = false;
+    c->batch.clear();
     return RPD_OK;
 }
 
@@ -583,6 +597,7 @@ int rpd_set_seeds_device(rpd_ctx *c, const double *d_xyzw, int64_t n) {
 int rpd_knn(rpd_ctx *c, int32_t K, int32_t prec, int32_t *out_nk) {
     if (!c || c->n < 2 || K <= 0 || !out_nk || (prec != 0 && prec != 1)) return set_err(c, RPD_E_ARG, "rpd_knn");
     if (K > 256) return set_err(c, RPD_E_LIMIT, "rpd_knn: K > 256");
+    if (!c->batch.empty()) return set_err(c, RPD_E_STATE, "rpd_knn: the seeds are a packed batch");
     hipSetDevice(c->device);
     RPDCHK(c, ensure_binning(c, prec), "binning");
     RPDCHK(c, c->knn.ensure(size_t(c->n) * K * 4), "alloc knn");
@@ -630,8 +645,14 @@ int rpd_knn_device_noalloc(rpd_ctx *c, const double *d_xyzw, int64_t n, int32_t 
     return RPD_OK;
 }
 
-int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *stats) {
+// The solve of the context's seeds: one diagram (against dom), or the packed
+// batch rpd_solve_batch has set up (dom null, each request's domain in its
+// descriptor).
+static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *stats) {
     if (!c || !prm) return set_err(c, RPD_E_ARG, "rpd_solve: bad arguments");
+    const bool batch = !c->batch.empty();
+    if (batch && (dom || prm->H > 0))
+        return set_err(c, RPD_E_ARG, "rpd_solve_batch: no Hessian, domains per request");
     if (c->n < 2) return set_err(c, RPD_E_STATE, "rpd_solve: no seeds");
     if (prm->precision < 0 || prm->precision > 2) return set_err(c, RPD_E_ARG, "rpd_solve: bad precision");
     if (prm->K <= 0 || prm->K > 256 || prm->P < 7 || prm->P > 255 || prm->V < 8 || prm->V > 256 || prm->T < 0)
@@ -660,14 +681,18 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
 
     HIPCHK(c, hipEventRecord(c->ev[0][0], c->stream));
     // the weight minima of new seeds, read after the binning's wait
-    const bool new_wmin = prm->security_mode == 0 && !c->wmin_ok;
+    const bool new_wmin = !batch && prm->security_mode == 0 && !c->wmin_ok;
     if (new_wmin) {
         RPDCHK(c, c->wred.ensure(2 * kWminBlocks * sizeof(double)), "alloc wred");
         wmin_kernel<<<kWminBlocks, 256, 0, c->stream>>>(c->seeds.as<double4>(), n, c->wred.as<double>());
         HIPCHK(c, hipMemcpyAsync(c->pin_red, c->wred.ptr, 2 * kWminBlocks * sizeof(double), hipMemcpyDeviceToHost,
                                  c->stream));
     }
-    RPDCHK(c, ensure_binning(c, kprec), "binning");
+    if (batch) {
+        if (c->bin_prec != kprec) return set_err(c, RPD_E_STATE, "rpd_solve_batch: binned in another precision");
+    } else {
+        RPDCHK(c, ensure_binning(c, kprec), "binning");
+    }
     HIPCHK(c, hipEventRecord(c->ev[0][1], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     st.ms_bin = elapsed_ms(c->ev[0][0], c->ev[0][1]);
@@ -720,29 +745,50 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         wmin = c->wmin < INFINITY ? c->wmin : 0.0;
         border_seeds = c->wall <= RPD_NO_CELL_LIMITS;
     }
-    // cells provably outside the domain skip pass 1 (exact stop, no walls)
+    // cells provably outside the domain skip pass 1 (exact stop, no walls);
+    // in a batch, per request
     const int *ids1 = nullptr;
     int64_t n1 = n;
-    if (dd && prm->cull && prm->security_mode == 0 && !border_seeds) {
-        RPDCHK(c, c->ids1.ensure(size_t(n) * 4), "alloc ids");
+    std::vector<CoverArgs> cov;
+    auto cover_of = [&](const DomainDev *d, const int *counters, int dim, int64_t base, int64_t cnt, double wm,
+                        bool walls) {
         CoverArgs ca{};
         ca.stored64 = c->stored64.as<double4>();
         ca.perm = c->perm.as<int>();
-        ca.counters = c->counters.as<int>();
-        ca.dim = c->dim;
-        ca.n = n;
-        ca.dvox = dd->dvox;
-        ca.n_dvox = dd->n_dvox;
-        ca.dist = dd->dist;
-        ca.g = dd->g;
-        ca.vs = dd->vs;
-        ca.wmin = wmin;
+        ca.counters = counters;
+        ca.dim = dim;
+        ca.base = base;
+        ca.n = cnt;
+        ca.apply = d && prm->cull && prm->security_mode == 0 && !walls;
+        if (d) {
+            ca.dvox = d->dvox;
+            ca.n_dvox = d->n_dvox;
+            ca.dist = d->dist;
+            ca.g = d->g;
+            ca.vs = d->vs;
+        }
+        ca.wmin = wm;
         ca.out = c->out.as<double4>();
         ca.status = c->status.as<int>();
         ca.kused = c->kused.as<int>();
+        return ca;
+    };
+    if (batch) {
+        for (const auto &q : c->batch)
+            cov.push_back(cover_of(q.dom ? &q.dom->dev : nullptr, c->counters.as<int>() + q.coff, q.dim, q.base, q.n,
+                                   prm->security_mode == 0 && q.wmin < INFINITY ? q.wmin : 0.0,
+                                   prm->security_mode != 0 || q.wall <= RPD_NO_CELL_LIMITS));
+    } else if (dd) {
+        cov.push_back(cover_of(dd, c->counters.as<int>(), c->dim, 0, n, wmin, border_seeds));
+    }
+    bool any_cover = false;
+    for (const auto &ca : cov) any_cover = any_cover || ca.apply;
+    if (any_cover) {
+        RPDCHK(c, c->ids1.ensure(size_t(n) * 4), "alloc ids");
         int kept = 0;
         HIPCHK(c, hipEventRecord(c->ev[1][0], c->stream));
-        RPDCHK(c, rpd_cover_skip(c->stream, ca, c->cull_scratch, c->ids1.as<int>(), &kept), "domain cover");
+        RPDCHK(c, rpd_cover_skip(c->stream, cov.data(), int(cov.size()), n, c->cull_scratch, c->ids1.as<int>(), &kept),
+               "domain cover");
         HIPCHK(c, hipEventRecord(c->ev[1][1], c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev[1][1]));
         st.ms_cover = elapsed_ms(c->ev[1][0], c->ev[1][1]);
@@ -750,6 +796,9 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         n1 = kept;
         st.skipped_cells = n - kept;
     }
+    const ReqDesc *reqs = batch ? c->req_desc.as<ReqDesc>() : nullptr;
+    const KnnReq *kreqs = batch ? c->knn_req.as<KnnReq>() : nullptr;
+    const uint16_t *req_of = batch ? c->req_of.as<uint16_t>() : nullptr;
 
     // One pass = kNN + bisector kernel (which waits for the queue length)
     // on a stream, then the finish kernel queued behind them.  Failed cells
@@ -783,7 +832,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         if (r != RPD_OK) return r;
         if (hipEventRecord(c->ev[sl][0], s) != hipSuccess) return RPD_E_DEVICE;
         r = rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids, ps.n_ids,
-                           K_, b.knn.as<int>(), rz.in ? c->lastnb.as<int>() : nullptr);
+                           K_, b.knn.as<int>(), rz.in ? c->lastnb.as<int>() : nullptr, kreqs, req_of);
         if (r != RPD_OK) return r;
         if (hipEventRecord(c->ev[sl][1], s) != hipSuccess) return RPD_E_DEVICE;
         L = ClipLaunch{};
@@ -828,6 +877,8 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
             L.hc = prm->HC;
         }
         L.dcnt = dcnt_of(ps.index);
+        L.reqs = reqs;
+        L.req_of = req_of;
         return rpd_launch_cells(s, ps.form, L, c->pin + sl, false);
     };
     // the finish kernel of a pass whose bisector kernel (and queue length) is done
@@ -1071,7 +1122,7 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
         HIPCHK(c, hipEventRecord(c->ev[0][0], s));
         RPDCHK(c,
                rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids1, n1, K,
-                              b.knn.as<int>()),
+                              b.knn.as<int>(), nullptr, kreqs, req_of),
                "pass 1 knn");
         HIPCHK(c, hipEventRecord(c->ev[0][1], s));
         ClipLaunch &L = L1;
@@ -1115,6 +1166,8 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
             L.lastnb = c->lastnb.as<int>();
         }
         L.dcnt = dcnt_of(0);
+        L.reqs = reqs;
+        L.req_of = req_of;
         RPDCHK(c, rpd_launch_cells(s, form0, L, c->pin + 0, true), "pass 1 cells");
         p1.nq = c->pin[0];
         L.fl_cnt = c->fl[1].cnt.as<int>();
@@ -1192,6 +1245,133 @@ int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *sta
     c->solved = true;
     c->dcnt_ok = prm->debug_counters != 0;
     if (stats) *stats = st;
+    return RPD_OK;
+}
+
+int rpd_solve(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_stats *stats) {
+    if (c && !c->batch.empty())
+        return set_err(c, RPD_E_STATE, "rpd_solve: the seeds are a packed batch (set seeds, or rpd_solve_batch)");
+    return solve_impl(c, dom, prm, stats);
+}
+
+int rpd_solve_batch(rpd_ctx *c, int32_t nreq, rpd_domain *const *doms, const double *const *xyzw, const int64_t *ns,
+                    const rpd_params *prm, rpd_stats *stats) {
+    if (!c || !xyzw || !ns || !prm || nreq <= 0) return set_err(c, RPD_E_ARG, "rpd_solve_batch: bad arguments");
+    if (nreq > RPD_BATCH_MAX_REQUESTS)
+        return set_err(c, RPD_E_LIMIT, "rpd_solve_batch: at most %d requests", RPD_BATCH_MAX_REQUESTS);
+    if (prm->H > 0) return set_err(c, RPD_E_ARG, "rpd_solve_batch: no Hessian in a batch");
+    if (prm->precision < 0 || prm->precision > 2) return set_err(c, RPD_E_ARG, "rpd_solve_batch: bad precision");
+    int64_t total = 0;
+    for (int r = 0; r < nreq; ++r) {
+        if (!xyzw[r] || ns[r] < 2) return set_err(c, RPD_E_ARG, "rpd_solve_batch: request %d needs 2 seeds", r);
+        if (doms && doms[r] && doms[r]->device != c->device)
+            return set_err(c, RPD_E_ARG, "rpd_solve_batch: domain of request %d on another device", r);
+        total += ns[r];
+        if (total > INT32_MAX / 2) return set_err(c, RPD_E_LIMIT, "rpd_solve_batch: too many seeds");
+    }
+    hipSetDevice(c->device);
+    const int kprec = prm->precision == RPD_PREC_FP64 ? 1 : 0;
+    c->batch.clear();
+    c->n = 0;
+    c->bin_prec = -1;
+    c->wmin_ok = false;
+    c->solved = false;
+    std::vector<rpd_ctx::BatchReq> reqs(static_cast<size_t>(nreq));
+    int64_t base = 0, coff = 0;
+    for (int r = 0; r < nreq; ++r) {
+        auto &q = reqs[size_t(r)];
+        q.dom = doms ? doms[r] : nullptr;
+        q.base = base;
+        q.n = ns[r];
+        q.dim = rpd_knn_dim(q.n);
+        q.coff = coff;
+        // weight minima over the non-border seeds and over all, on the host
+        q.wmin = q.wall = INFINITY;
+        const double *x = xyzw[r];
+        for (int64_t i = 0; i < q.n; ++i) {
+            const double w = x[4 * i + 3];
+            if (w > RPD_NO_CELL_LIMITS) q.wmin = std::min(q.wmin, w);
+            q.wall = std::min(q.wall, w);
+        }
+        base += q.n;
+        coff += int64_t(q.dim) * q.dim * q.dim + 1;
+        if (coff > INT32_MAX) return set_err(c, RPD_E_LIMIT, "rpd_solve_batch: kNN grids too large");
+    }
+    const size_t vsz = kprec == 0 ? 16 : 32;
+    RPDCHK(c, c->seeds.ensure(size_t(total) * 32), "alloc seeds");
+    RPDCHK(c, c->perm.ensure(size_t(total) * 4), "alloc perm");
+    RPDCHK(c, c->counters.ensure(size_t(coff) * 4), "alloc counters");
+    RPDCHK(c, c->stored.ensure(size_t(total) * vsz), "alloc stored");
+    RPDCHK(c, c->stored64.ensure(size_t(total) * 32), "alloc stored64");
+    RPDCHK(c, c->req_of.ensure(size_t(total) * 2), "alloc request map");
+    RPDCHK(c, c->req_desc.ensure(size_t(nreq) * sizeof(ReqDesc)), "alloc request descriptors");
+    RPDCHK(c, c->knn_req.ensure(size_t(nreq) * sizeof(KnnReq)), "alloc request descriptors");
+    hipStream_t s = c->stream;
+    std::vector<ReqDesc> desc(static_cast<size_t>(nreq));
+    std::vector<KnnReq> kdesc(static_cast<size_t>(nreq));
+    for (int r = 0; r < nreq; ++r) {
+        const auto &q = reqs[size_t(r)];
+        double4 *sd = c->seeds.as<double4>() + q.base;
+        HIPCHK(c, hipMemcpyAsync(sd, xyzw[r], size_t(q.n) * 32, hipMemcpyHostToDevice, s));
+        // the request's own binning (as alone), then its ids made global
+        int *perm = c->perm.as<int>() + q.base;
+        int *cnt = c->counters.as<int>() + q.coff;
+        const int64_t nc = int64_t(q.dim) * q.dim * q.dim;
+        RPDCHK(c,
+               rpd_bin_seeds(s, sd, q.n, kprec, q.dim, c->bin_scratch, perm, cnt,
+                             static_cast<char *>(c->stored.ptr) + size_t(q.base) * vsz,
+                             c->stored64.as<double4>() + q.base),
+               "rpd_solve_batch: binning");
+        RPDCHK(c, rpd_add_base(s, perm, q.n, int(q.base)), "rpd_solve_batch: binning");
+        RPDCHK(c, rpd_add_base(s, cnt, nc + 1, int(q.base)), "rpd_solve_batch: binning");
+        HIPCHK(c, hipMemsetD16Async(reinterpret_cast<hipDeviceptr_t>(c->req_of.as<uint16_t>() + q.base),
+                                    uint16_t(r), size_t(q.n), s));
+        ReqDesc &d = desc[size_t(r)];
+        d = ReqDesc{};
+        d.n_seeds = q.n;
+        d.wmin = prm->security_mode == 0 && q.wmin < INFINITY ? q.wmin : 0.0;
+        if (q.dom) {
+            const DomainDev &dv = q.dom->dev;
+            d.g = dv.g;
+            d.vs = dv.vs;
+            d.flags = dv.flags;
+            d.centre = dv.centre;
+            d.gpts = dv.pts;
+            d.tris = dv.tris;
+            d.tlist = dv.tlist;
+            d.toffs = dv.toffs;
+            d.tvoffs = dv.tvoffs;
+            d.tvox = dv.tvox;
+            d.tinfo = dv.tinfo;
+            d.dist = prm->cull ? dv.dist : nullptr;
+        }
+        kdesc[size_t(r)] = KnnReq{cnt, q.dim};
+    }
+    HIPCHK(c, hipMemcpyAsync(c->req_desc.ptr, desc.data(), desc.size() * sizeof(ReqDesc), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->knn_req.ptr, kdesc.data(), kdesc.size() * sizeof(KnnReq), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipStreamSynchronize(s));  // the host vectors above are pageable
+    c->n = total;
+    c->bin_prec = kprec;
+    c->dim = 0;
+    c->batch = std::move(reqs);
+    return solve_impl(c, nullptr, prm, stats);
+}
+
+int rpd_batch_range(const rpd_ctx *c, int32_t r, int64_t *first, int64_t *count) {
+    if (!c || !first || !count || r < 0 || size_t(r) >= c->batch.size()) return RPD_E_ARG;
+    *first = c->batch[size_t(r)].base;
+    *count = c->batch[size_t(r)].n;
+    return RPD_OK;
+}
+
+int rpd_read_moments_range(rpd_ctx *c, int64_t first, int64_t count, double *out4) {
+    if (!c || !out4 || first < 0 || count < 0 || first + count > c->n)
+        return set_err(c, RPD_E_ARG, "rpd_read_moments_range: bad range");
+    if (!c->solved) return set_err(c, RPD_E_STATE, "rpd_read_moments_range: no solve");
+    hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpyAsync(out4, c->out.as<double4>() + first, size_t(count) * 32, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return RPD_OK;
 }
 
@@ -1286,6 +1466,8 @@ int compact_device(rpd_ctx *c, double keep_volume, int32_t keep_distance) {
         return RPD_OK;
     };
     if (keep_distance < 0) return finish(nullptr, -1);  // every cell
+    if (keep_distance > 0 && !c->batch.empty())
+        return set_err(c, RPD_E_STATE, "rpd_compact_used: no bordering rings for a packed batch");
 
     RPDCHK(c, c->used.ensure(size_t(n) * 4), "alloc used");
     RPDCHK(c, c->ids1.ensure(size_t(n) * 4), "alloc ids");

@@ -113,7 +113,33 @@ struct ClipArgs {
     int *hids;
     int H, hc;
     int *dcnt;  // debug counters of this pass (kDcStride ints), nullable
+    // packed batch (nullable): the fields of request req_of[sid] replace
+    // n_seeds, wmin and the domain's
+    const ReqDesc *reqs;
+    const uint16_t *req_of;
 };
+
+// The cell of stored id sid reads its request's seed count, weight minimum
+// and domain (packed batch; wave-uniform, so the fields stay in SGPRs).  The
+// kernels take it as a template parameter, so that a single diagram's
+// kernels keep every field in the kernel arguments.
+__device__ __forceinline__ void bind_request(ClipArgs &a, int sid) {
+    const ReqDesc &d = a.reqs[uni(int(a.req_of[sid]))];
+    a.n_seeds = d.n_seeds;
+    a.wmin = d.wmin;
+    a.g = d.g;
+    a.vs = d.vs;
+    a.flags = d.flags;
+    a.centre = d.centre;
+    a.gpts = d.gpts;
+    a.tris = d.tris;
+    a.tlist = d.tlist;
+    a.toffs = d.toffs;
+    a.tvoffs = d.tvoffs;
+    a.tvox = d.tvox;
+    a.tinfo = d.tinfo;
+    a.dist = d.dist;
+}
 
 // Appends a failed cell (stored id sid) to the failure list: overflows
 // (vertex, plane, K, triangle) always; needs_exact_predicates,
@@ -174,12 +200,12 @@ __device__ __forceinline__ void vertex_from_planes(const Pl<R> &p1, const Pl<R> 
     Z = -p1.x * m24 + p1.y * m14 - p1.w * m12;
     W = p1.x * m23 - p1.y * m13 + p1.z * m12;
     if (FILTER) {
-        const float a12 = fabsf(float(p2.x * p3.y)) + fabsf(float(p2.y * p3.x));
-        const float a13 = fabsf(float(p2.x * p3.z)) + fabsf(float(p2.z * p3.x));
-        const float a14 = fabsf(float(p2.x * p3.w)) + fabsf(float(p2.w * p3.x));
-        const float a23 = fabsf(float(p2.y * p3.z)) + fabsf(float(p2.z * p3.y));
-        const float a24 = fabsf(float(p2.y * p3.w)) + fabsf(float(p2.w * p3.y));
-        const float a34 = fabsf(float(p2.z * p3.w)) + fabsf(float(p2.w * p3.z));
+        const float x2 = fabsf(float(p2.x)), y2 = fabsf(float(p2.y)), z2 = fabsf(float(p2.z)),
+                    w2 = fabsf(float(p2.w));
+        const float x3 = fabsf(float(p3.x)), y3 = fabsf(float(p3.y)), z3 = fabsf(float(p3.z)),
+                    w3 = fabsf(float(p3.w));
+        const float a12 = fmaf(x2, y3, y2 * x3), a13 = fmaf(x2, z3, z2 * x3), a14 = fmaf(x2, w3, w2 * x3);
+        const float a23 = fmaf(y2, z3, z2 * y3), a24 = fmaf(y2, w3, w2 * y3), a34 = fmaf(z2, w3, w2 * z3);
         const float ax = fabsf(float(p1.x)), ay = fabsf(float(p1.y)), az = fabsf(float(p1.z)),
                     aw = fabsf(float(p1.w));
         const float MX = aw * a23 + ay * a34 + az * a24;
@@ -527,7 +553,7 @@ struct RecHeader {
 template <typename R, int S>
 constexpr int kClipWaves = S > 1 ? 1 : (sizeof(R) == 8 ? 7 : 8);
 
-template <typename R, typename RP, bool FILTER, int S>
+template <typename R, typename RP, bool FILTER, int S, bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kClipWaves<R, S>)))
 void clip_cells_kernel(ClipArgs a) {
     extern __shared__ __attribute__((aligned(16))) char lds_raw[];
@@ -549,7 +575,8 @@ void clip_cells_kernel(ClipArgs a) {
     c.V = a.V;
     c.lane = lane;
 
-    const int sid = a.ids ? a.ids[t] : int(t);
+    const int sid = uni(a.ids ? a.ids[t] : int(t));
+    if constexpr (BATCH) bind_request(a, sid);
     const double4 S0 = a.seeds[sid];
     const int orig = a.perm[sid];
     const int K = a.K, P = a.P;
@@ -896,7 +923,7 @@ __device__ __forceinline__ uint32_t box_neighbours(bool xl, bool xh, bool yl, bo
 // either (at most 3 more planes, at most 2 more vertices per clip), so the
 // outcome is the reference's.
 // ---------------------------------------------------------------------------
-template <typename R, typename RP, bool FILTER, int S, bool HESS>
+template <typename R, typename RP, bool FILTER, int S, bool HESS, bool BATCH>
 // 4 waves per SIMD for S = 1 (128 VGPRs): measured at 1M seeds against 5 and
 // 6, whose spills cost more than the extra residency gives.  HESS: the
 // Hessian variant (one wave per cell).
@@ -928,6 +955,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     const char *rec = a.rec + size_t(slot) * size_t(a.rec_stride);
     const RecHeader hd = *reinterpret_cast<const RecHeader *>(rec);
     const int sid = uni(hd.sid);
+    if constexpr (BATCH) bind_request(a, sid);
     const int kused = uni(hd.kused);
     const int nvalid = uni(hd.nvalid);
     c.nb_plane = uni(hd.nb_plane);
@@ -1645,6 +1673,7 @@ static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     if (L.P < 7 || L.P > 255 || L.V < 8 || L.K < 1) return RPD_E_ARG;
     if (L.V > 256) return RPD_E_LIMIT;
     if (!L.rec || !L.qcount) return RPD_E_ARG;
+    if (L.reqs && (!L.req_of || (L.hess && L.H > 0))) return RPD_E_ARG;
     a = ClipArgs{};
     a.seeds = L.seeds;
     a.perm = L.perm;
@@ -1697,6 +1726,8 @@ static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     a.hc = L.hc == 4 ? 4 : 1;
     a.exact_final = form == RPD_PREC_FP64 ? 1 : 0;
     a.dcnt = L.dcnt;
+    a.reqs = L.reqs;
+    a.req_of = L.req_of;
     return RPD_OK;
 }
 
@@ -1717,14 +1748,21 @@ int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out,
         const int wpb = 1;
         const unsigned blocks = unsigned((L.n_ids + wpb - 1) / wpb);
         const size_t shm = stride * wpb;
-#define RPD_CLIP_A(R, RP, F)                                                          \
+#define RPD_CLIP_A1(R, RP, F, B)                                                      \
     do {                                                                              \
         if (S == 1)                                                                   \
-            clip_cells_kernel<R, RP, F, 1><<<blocks, dim3(64 * wpb), shm, st>>>(a);   \
+            clip_cells_kernel<R, RP, F, 1, B><<<blocks, dim3(64 * wpb), shm, st>>>(a);\
         else if (S == 2)                                                              \
-            clip_cells_kernel<R, RP, F, 2><<<blocks, dim3(64 * wpb), shm, st>>>(a);   \
+            clip_cells_kernel<R, RP, F, 2, B><<<blocks, dim3(64 * wpb), shm, st>>>(a);\
         else                                                                          \
-            clip_cells_kernel<R, RP, F, 4><<<blocks, dim3(64 * wpb), shm, st>>>(a);   \
+            clip_cells_kernel<R, RP, F, 4, B><<<blocks, dim3(64 * wpb), shm, st>>>(a);\
+    } while (0)
+#define RPD_CLIP_A(R, RP, F)              \
+    do {                                  \
+        if (a.reqs)                       \
+            RPD_CLIP_A1(R, RP, F, true);  \
+        else                              \
+            RPD_CLIP_A1(R, RP, F, false); \
     } while (0)
         if (form == RPD_PREC_FP64)
             RPD_CLIP_A(double, double, false);
@@ -1733,6 +1771,7 @@ int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out,
         else
             RPD_CLIP_A(float, float, false);
 #undef RPD_CLIP_A
+#undef RPD_CLIP_A1
         if (hipGetLastError() != hipSuccess) return RPD_E_DEVICE;
     }
     if (L.ev_mid && hipEventRecord(L.ev_mid, st) != hipSuccess) return RPD_E_DEVICE;
@@ -1775,14 +1814,22 @@ int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq, int
         const int cpb = 1;
         const unsigned blocks = unsigned((nq + cpb - 1) / cpb);
         const size_t shm = size_t(cpb) * cstride + size_t(wpb) * wstride;
-#define RPD_CLIP_B(R, RP, F, HS)                                                                  \
-    do {                                                                                          \
-        if (S == 1)                                                                               \
-            finish_cells_kernel<R, RP, F, 1, HS><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
-        else if (S == 2)                                                                          \
-            finish_cells_kernel<R, RP, F, 2, HS><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
-        else                                                                                      \
-            finish_cells_kernel<R, RP, F, 4, HS><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+#define RPD_CLIP_B1(R, RP, F, HS, B)                                                                 \
+    do {                                                                                             \
+        if (S == 1)                                                                                  \
+            finish_cells_kernel<R, RP, F, 1, HS, B><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+        else if (S == 2)                                                                             \
+            finish_cells_kernel<R, RP, F, 2, HS, B><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+        else                                                                                         \
+            finish_cells_kernel<R, RP, F, 4, HS, B><<<blocks, dim3(64 * wpb), shm, st>>>(a, nq, W);  \
+    } while (0)
+// a packed batch has no Hessian
+#define RPD_CLIP_B(R, RP, F, HS)                  \
+    do {                                          \
+        if (!HS && a.reqs)                        \
+            RPD_CLIP_B1(R, RP, F, false, true);   \
+        else                                      \
+            RPD_CLIP_B1(R, RP, F, HS, false);     \
     } while (0)
         if (form == RPD_PREC_FP64) {
             if (hess)
@@ -1801,6 +1848,7 @@ int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq, int
                 RPD_CLIP_B(float, float, false, false);
         }
 #undef RPD_CLIP_B
+#undef RPD_CLIP_B1
     }
     return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }

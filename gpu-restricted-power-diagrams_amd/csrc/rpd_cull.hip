@@ -95,29 +95,41 @@ __global__ __launch_bounds__(256) void skip_kernel(const double4 *__restrict__ s
 
 using namespace rpd;
 
-int rpd_cover_skip(hipStream_t st, const CoverArgs &c, DevBuf &scratch, int *ids_out, int *n_out) {
+int rpd_cover_skip(hipStream_t st, const CoverArgs *reqs, int nreq, int64_t n_total, DevBuf &scratch, int *ids_out,
+                   int *n_out) {
     *n_out = 0;
-    if (c.n <= 0) return RPD_OK;
-    const int64_t n = c.n;
-    // scratch: M bits | keep flags | selected count | rocprim temp
+    if (n_total <= 0 || nreq <= 0) return RPD_OK;
+    const int64_t n = n_total;
+    // scratch: M bits per request | keep flags | selected count | rocprim temp
     size_t sel_bytes = 0;
     const uint8_t *null_f = nullptr;
     int *null_i = nullptr;
     if (rocprim::select(nullptr, sel_bytes, rocprim::counting_iterator<int>(0), null_f, null_i, null_i, size_t(n), st) !=
         hipSuccess)
         return RPD_E_DEVICE;
-    const size_t off_keep = 256, off_cnt = (off_keep + size_t(n) + 255) & ~size_t(255), off_tmp = off_cnt + 256;
+    const size_t off_keep = (size_t(nreq) * 8 + 255) & ~size_t(255);
+    const size_t off_cnt = (off_keep + size_t(n) + 255) & ~size_t(255), off_tmp = off_cnt + 256;
     if (scratch.ensure(off_tmp + sel_bytes + 256) != RPD_OK) return RPD_E_NOMEM;
     char *base = static_cast<char *>(scratch.ptr);
     auto *mbits = reinterpret_cast<unsigned long long *>(base);
     auto *keep = reinterpret_cast<uint8_t *>(base + off_keep);
     int *cnt = reinterpret_cast<int *>(base + off_cnt);
-    if (hipMemsetAsync(mbits, 0, sizeof(unsigned long long), st) != hipSuccess) return RPD_E_DEVICE;
-    if (c.n_dvox > 0)
-        coverage_kernel<<<unsigned((c.n_dvox + 255) / 256), 256, 0, st>>>(c.stored64, c.counters, c.dim, c.dvox,
-                                                                           c.n_dvox, c.g, c.vs, c.wmin, mbits);
-    skip_kernel<<<unsigned((n + 255) / 256), 256, 0, st>>>(c.stored64, c.perm, n, c.dist, c.g, c.vs, c.wmin, mbits,
-                                                           keep, c.out, c.status, c.kused);
+    if (hipMemsetAsync(mbits, 0, size_t(nreq) * 8, st) != hipSuccess) return RPD_E_DEVICE;
+    for (int r = 0; r < nreq; ++r) {
+        const CoverArgs &c = reqs[r];
+        if (c.n <= 0) continue;
+        if (c.base < 0 || c.base + c.n > n) return RPD_E_ARG;
+        if (!c.apply) {  // every cell of this request is kept
+            if (hipMemsetAsync(keep + c.base, 1, size_t(c.n), st) != hipSuccess) return RPD_E_DEVICE;
+            continue;
+        }
+        if (c.n_dvox > 0)
+            coverage_kernel<<<unsigned((c.n_dvox + 255) / 256), 256, 0, st>>>(c.stored64, c.counters, c.dim, c.dvox,
+                                                                               c.n_dvox, c.g, c.vs, c.wmin, mbits + r);
+        skip_kernel<<<unsigned((c.n + 255) / 256), 256, 0, st>>>(c.stored64 + c.base, c.perm + c.base, c.n, c.dist,
+                                                                 c.g, c.vs, c.wmin, mbits + r, keep + c.base, c.out,
+                                                                 c.status, c.kused);
+    }
     size_t sb = sel_bytes;
     if (rocprim::select(base + off_tmp, sb, rocprim::counting_iterator<int>(0), keep, ids_out, cnt, size_t(n), st) !=
         hipSuccess)

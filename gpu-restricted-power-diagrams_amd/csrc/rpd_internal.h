@@ -77,10 +77,24 @@ int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int
 int rpd_bin_seeds_in(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, void *scratch,
                      size_t scratch_bytes, int *flag_host, int *perm, int *counters, void *stored, double4 *stored64);
 size_t rpd_bin_scratch_bytes(hipStream_t st, int64_t n, int dim);
+// Several diagrams in one launch (a packed batch): request r owns the
+// stored ids [base_r, base_r + n_r) of the shared arrays (seeds, sorted
+// copies, permutation with original indices base_r + i, outputs), and its
+// kNN grid is the counters (global stored ids) at its own offset; req_of[s]
+// is the request of stored id s.  Per request, the kNN reads:
+struct KnnReq {
+    const int *counters;  // dim^3 + 1 entries, global stored ids
+    int dim;
+};
 // K nearest of each query (stored ids ids[0..n_ids), nullptr: all), or, with
-// lower set, the K after the neighbour lower[q] (stored-id indexed).
+// lower set, the K after the neighbour lower[q] (stored-id indexed).  With
+// kreqs set, each query searches the grid of its request (req_of) instead of
+// (counters, dim).
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
-                   const int *ids, int64_t n_ids, int K, int *out, const int *lower = nullptr);
+                   const int *ids, int64_t n_ids, int K, int *out, const int *lower = nullptr,
+                   const KnnReq *kreqs = nullptr, const uint16_t *req_of = nullptr);
+// adds base to n ints (binning of one request of a packed batch)
+int rpd_add_base(hipStream_t st, int *v, int64_t n, int base);
 
 // ---- clip (rpd_clip.hip) --------------------------------------------------
 struct DomainDev {
@@ -99,6 +113,25 @@ struct DomainDev {
     const int *dvox = nullptr;       // the domain voxels (distance 0), ascending
     int n_dvox = 0;
     bool has_border_vertices = false;
+};
+
+// Per-request fields of a packed batch that the clip kernels read in place
+// of the launch's own (seed count, weight minimum, domain).
+struct ReqDesc {
+    int64_t n_seeds;
+    double wmin;
+    int g;
+    double vs;
+    const int8_t *flags;
+    const int8_t *centre;
+    const double4 *gpts;
+    const int *tris;
+    const int *tlist;
+    const int *toffs;
+    const int *tvoffs;
+    const uint32_t *tvox;
+    const uint2 *tinfo;
+    const uint8_t *dist;  // nullptr: no cull
 };
 
 struct ClipLaunch {
@@ -140,6 +173,9 @@ struct ClipLaunch {
     int *hids;
     int H, hc;
     int *dcnt;             // debug counters of the pass (kDcStride ints, nullable)
+    // packed batch (nullable): per request fields, request of each stored id
+    const ReqDesc *reqs;
+    const uint16_t *req_of;
 };
 
 constexpr int kFailCounters = 16;
@@ -158,12 +194,14 @@ constexpr int kDcUnionMax = RPD_DCOUNT_UNION_MAX;
 constexpr int kDcStride = RPD_DCOUNT_INTS;
 
 // Cells whose restriction is provably empty, before the kNN (rpd_cull.hip).
+// One per request: its cells are the stored ids [base, base + n).
 struct CoverArgs {
-    const double4 *stored64;  // sorted seeds, FP64
-    const int *perm;          // stored -> original
-    const int *counters;      // kNN grid: dim^3 + 1 exclusive prefix
+    const double4 *stored64;  // sorted seeds, FP64 (every request)
+    const int *perm;          // stored -> original (every request)
+    const int *counters;      // the request's kNN grid: dim^3 + 1 exclusive prefix, global stored ids
     int dim;
-    int64_t n;
+    int64_t base, n;
+    bool apply;               // false: every cell is kept
     const int *dvox;          // domain voxels
     int n_dvox;
     const uint8_t *dist;      // Chebyshev voxel distance to the domain voxels
@@ -175,8 +213,10 @@ struct CoverArgs {
     int *kused;
 };
 // Writes the outputs of the skipped cells and the stored ids of the others,
-// ascending, to ids_out (n entries); *n_out = their number.  Synchronizes st.
-int rpd_cover_skip(hipStream_t st, const CoverArgs &c, DevBuf &scratch, int *ids_out, int *n_out);
+// ascending, to ids_out (n_total entries); *n_out = their number.  The
+// requests' ranges lie in [0, n_total).  Synchronizes st.
+int rpd_cover_skip(hipStream_t st, const CoverArgs *reqs, int nreq, int64_t n_total, DevBuf &scratch, int *ids_out,
+                   int *n_out);
 
 // Bytes of one queued-cell record.
 size_t rpd_clip_record_bytes(int form, int P, int V);

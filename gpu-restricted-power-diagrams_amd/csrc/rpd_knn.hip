@@ -262,6 +262,10 @@ struct KnnArgs {
     // nullable: per stored id, a neighbour already known; only keys above
     // its key are listed (the K after it)
     const int *lower;
+    // packed batch (nullable): the grid of request req_of[q] replaces
+    // (counters, dim)
+    const KnnReq *reqs;
+    const uint16_t *req_of;
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -383,9 +387,15 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
             buf.s = reinterpret_cast<uint32_t *>(kb + 12 * KEEP);
         }
     }
-    const int q = a.ids ? a.ids[qi] : int(qi);
+    const int q = uni(a.ids ? a.ids[qi] : int(qi));
     const typename Vec4T<R>::T p = a.pts[q];
-    const int dim = a.dim;
+    int dim = a.dim;
+    const int *counters = a.counters;
+    if (a.reqs) {
+        const KnnReq &rq = a.reqs[uni(int(a.req_of[q]))];
+        dim = rq.dim;
+        counters = rq.counters;
+    }
     const int ci = cell_coord(R(p.x), dim), cj = cell_coord(R(p.y), dim), ck = cell_coord(R(p.z), dim);
     const int K = a.K;
     const double cs = 1000.0 / double(dim);
@@ -493,8 +503,8 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
                     }
                     if (xlo >= 0 && xhi < dim && xlo <= xhi) {
                         const int row = y * dim + z * dim * dim;
-                        sa = a.counters[row + xlo];
-                        sb = a.counters[row + xhi + 1];
+                        sa = counters[row + xlo];
+                        sb = counters[row + xhi + 1];
                     }
                 }
             }
@@ -663,15 +673,30 @@ int rpd_bin_seeds_in(hipStream_t st, const double4 *seeds, int64_t n, int prec, 
     return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }
 
+namespace rpd {
+__global__ void add_base_kernel(int *__restrict__ v, int64_t n, int base) {
+    const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (i < n) v[i] += base;
+}
+}  // namespace rpd
+
+int rpd_add_base(hipStream_t st, int *v, int64_t n, int base) {
+    if (n <= 0 || base == 0) return RPD_OK;
+    add_base_kernel<<<unsigned((n + 255) / 256), 256, 0, st>>>(v, n, base);
+    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+}
+
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
-                   const int *ids, int64_t n_ids, int K, int *out, const int *lower) {
+                   const int *ids, int64_t n_ids, int K, int *out, const int *lower, const KnnReq *kreqs,
+                   const uint16_t *req_of) {
     if (n_ids <= 0) return RPD_OK;
     if (K <= 0 || K > 256) return RPD_E_LIMIT;
     const unsigned blocks = unsigned((n_ids + kKnnWavesPerBlock - 1) / kKnnWavesPerBlock);
     const dim3 tb(64 * kKnnWavesPerBlock);
 #define RPD_KNN_LAUNCH(R, V4)                                                                       \
     do {                                                                                            \
-        KnnArgs<R> a{static_cast<const V4 *>(stored), perm, counters, ids, n_ids, dim, K, out, lower}; \
+        KnnArgs<R> a{static_cast<const V4 *>(stored), perm, counters, ids, n_ids, dim, K, out, lower, \
+                     kreqs, req_of};                                                                \
         if (K <= 64)                                                                                \
             knn_kernel<R, 1><<<blocks, tb, 0, st>>>(a);                                             \
         else if (K <= 128)                                                                          \

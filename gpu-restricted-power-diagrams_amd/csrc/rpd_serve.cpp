@@ -5,16 +5,20 @@
 // with the grid kept resident across launches (voronoi.h:126-131) and its
 // per-launch buffers reset after each one (voronoi.cpp:1401).
 //
-// Design: a FIFO of submitted requests and `max_inflight` device slots.  A
-// slot is a solver context of its own (its own HIP stream and resident
-// buffers, which grow to the largest request it has served and are reused).
-// A slot takes the request at the head of the queue as soon as the request's
-// device footprint fits in what the HBM budget has left; requests are
-// admitted strictly in arrival order, so a large request is never starved by
-// small ones behind it.  Admitted requests run concurrently on their slots'
-// streams, so the cell work of several requests shares the device.  Every
-// request is solved by the same kernels on its own buffers, with no state
-// shared between requests, so its result does not depend on what else runs.
+// Design: a FIFO of submitted requests and `max_inflight` batch slots.  A
+// slot is a solver context of its own (its own HIP streams and resident
+// buffers, which grow to the largest batch it has served and are reused).  A
+// free slot takes the longest run of requests at the head of the queue that
+// share the head's parameters and whose device footprints fit in what the
+// HBM budget has left (at most max_batch requests and kBatchMaxSeeds seeds),
+// and solves them as one packed batch (rpd_solve_batch): every kernel of a
+// pass is one launch over the cells of all of them.  Requests are admitted strictly in arrival order, so
+// a large request is never starved by small ones behind it, and a request
+// larger than the whole budget runs alone.  With two slots, the next batch
+// is admitted while the previous one finishes its re-queue tail, so the
+// device stays busy across batch boundaries.  Every cell of a batch is
+// computed as in a solve of its request alone, so a request's result does
+// not depend on what it was batched with.
 
 #include <algorithm>
 #include <chrono>
@@ -55,6 +59,12 @@ struct Request {
 // and cell records of pass 1 and, bounded by the same size, of the re-queue
 // passes that run beside it; plus the kept states of pass 1's cells whose
 // list ran out (room for 2/5 of the cells, plus 1024).
+// Seeds of one batch: requests are packed up to this total (a larger request
+// runs alone), so that the queue spreads over the slots and each batch fills
+// the device in its first pass while its re-queue tail overlaps the other
+// slots' work.
+constexpr int64_t kBatchMaxSeeds = int64_t(1) << 19;
+
 int64_t footprint_bytes(int64_t n, const rpd_params &p) {
     const int64_t K = std::max(1, p.K), P = std::max(1, p.P);
     const int64_t S = p.V <= 64 ? 1 : (p.V <= 128 ? 2 : 4);
@@ -79,10 +89,19 @@ struct rpd_server {
     std::vector<std::thread> workers;
     std::vector<rpd_ctx *> slots;
 
+    static bool same_params(const rpd_params &a, const rpd_params &b) {
+        return std::memcmp(&a, &b, sizeof(rpd_params)) == 0;
+    }
+
     void worker(int s) {
         rpd_ctx *ctx = slots[size_t(s)];
+        std::vector<Request *> b;
+        std::vector<rpd_domain *> doms;
+        std::vector<const double *> xs;
+        std::vector<int64_t> ns;
         for (;;) {
-            Request *r = nullptr;
+            int64_t bytes = 0;
+            b.clear();
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] {
@@ -93,27 +112,56 @@ struct rpd_server {
                     // budget runs alone
                     return inflight_bytes + h->footprint <= cfg.hbm_budget_bytes || running == 0;
                 });
-                if (stop && queue.empty()) return;
                 if (stop) return;
-                r = queue.front();
-                queue.pop_front();
-                inflight_bytes += r->footprint;
+                // the head, then the requests behind it with the same
+                // parameters while the footprints fit
+                int64_t n_total = 0;
+                while (!queue.empty() && int(b.size()) < cfg.max_batch) {
+                    Request *h = queue.front();
+                    if (!b.empty()) {
+                        if (!same_params(h->prm, b.front()->prm)) break;
+                        if (inflight_bytes + bytes + h->footprint > cfg.hbm_budget_bytes) break;
+                        if (n_total + h->n > kBatchMaxSeeds) break;
+                    }
+                    queue.pop_front();
+                    b.push_back(h);
+                    bytes += h->footprint;
+                    n_total += h->n;
+                }
+                inflight_bytes += bytes;
                 ++running;
             }
-            r->out.assign(size_t(r->n) * 4, 0.0);
-            int rc = rpd_set_seeds(ctx, r->seeds.data(), r->n);
-            // served requests return moments only: no Hessian
-            r->prm.H = 0;
-            if (rc == RPD_OK) rc = rpd_solve(ctx, r->dom, &r->prm, nullptr);
-            if (rc == RPD_OK) rc = rpd_read_moments(ctx, r->out.data());
-            r->rc = rc;
-            r->lat_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
-            std::vector<double>().swap(r->seeds);
+            doms.clear();
+            xs.clear();
+            ns.clear();
+            for (Request *r : b) {
+                doms.push_back(r->dom);
+                xs.push_back(r->seeds.data());
+                ns.push_back(r->n);
+            }
+            int rc = rpd_solve_batch(ctx, int32_t(b.size()), doms.data(), xs.data(), ns.data(), &b.front()->prm,
+                                     nullptr);
+            for (size_t i = 0; i < b.size(); ++i) {
+                Request *r = b[i];
+                int rr = rc;
+                int64_t first = 0, count = 0;
+                if (rr == RPD_OK) rr = rpd_batch_range(ctx, int32_t(i), &first, &count);
+                if (rr == RPD_OK) {
+                    r->out.assign(size_t(r->n) * 4, 0.0);
+                    rr = rpd_read_moments_range(ctx, first, count, r->out.data());
+                }
+                r->rc = rr;
+            }
+            const Clock::time_point t_done = Clock::now();
+            for (Request *r : b) {
+                r->lat_ms = std::chrono::duration<double, std::milli>(t_done - r->t_submit).count();
+                std::vector<double>().swap(r->seeds);
+            }
             {
                 std::lock_guard<std::mutex> lk(mu);
-                inflight_bytes -= r->footprint;
+                inflight_bytes -= bytes;
                 --running;
-                done.push_back(r);
+                for (Request *r : b) done.push_back(r);
             }
             cv_done.notify_all();
             cv_work.notify_all();
@@ -131,7 +179,8 @@ int rpd_server_create(rpd_ctx *ctx, const rpd_server_config *cfg, rpd_server **o
     if (!s) return RPD_E_NOMEM;
     s->device = dev;
     s->cfg.hbm_budget_bytes = cfg && cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (int64_t(64) << 30);
-    s->cfg.max_inflight = cfg && cfg->max_inflight > 0 ? std::min(cfg->max_inflight, 16) : 4;
+    s->cfg.max_inflight = cfg && cfg->max_inflight > 0 ? std::min(cfg->max_inflight, 16) : 2;
+    s->cfg.max_batch = cfg && cfg->max_batch > 0 ? std::min(cfg->max_batch, RPD_BATCH_MAX_REQUESTS) : 256;
     for (int i = 0; i < s->cfg.max_inflight; ++i) {
         rpd_ctx *c = nullptr;
         const int rc = rpd_create(dev, &c);
@@ -172,6 +221,9 @@ int rpd_server_submit(rpd_server *s, rpd_domain *dom, const double *xyzw, int64_
         r->prm = *prm;
     else
         rpd_default_params(&r->prm);
+    // served requests return moments only: no Hessian
+    r->prm.H = 0;
+    r->prm.HC = 1;
     r->seeds.assign(xyzw, xyzw + size_t(n) * 4);
     r->footprint = footprint_bytes(n, r->prm);
     r->t_submit = Clock::now();

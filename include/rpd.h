@@ -230,6 +230,28 @@ int rpd_set_seeds_device(rpd_ctx *ctx, const double *d_xyzw, int64_t n);
    integrate with the status re-queue.  dom may be NULL (domain [0,1000]^3). */
 int rpd_solve(rpd_ctx *ctx, rpd_domain *dom, const rpd_params *prm, rpd_stats *stats);
 
+/* Packed batch: nreq diagrams solved together, one launch of each kernel for
+   all of their cells.  No counterpart in the reference, which runs one
+   VoroAlgo::launch per diagram (voronoi.cpp:1244-1403) with the grid
+   resident across launches (voronoi.h:126-131) and batches at most 100000
+   cell ids of one diagram per kernel pair (voronoi.cpp:960-973).  Request r:
+   ns[r] seeds xyzw[r] (host, copied) against doms[r] (NULL: [0,1000]^3;
+   doms itself may be NULL).  Each request is binned and searched on its
+   own, and each of its cells gives what rpd_solve of those seeds alone
+   gives.  Afterwards the context holds the requests' cells back to back:
+   rpd_read_moments / rpd_read_status return request 0's cells (original
+   order), then request 1's, and so on (rpd_batch_range gives each range).
+   rpd_solve and rpd_knn refuse such a context until rpd_set_seeds.  No
+   Hessian (H = 0) and no bordering rings (rpd_compact_used with
+   keep_distance > 0).  At most RPD_BATCH_MAX_REQUESTS requests. */
+#define RPD_BATCH_MAX_REQUESTS 4096
+int rpd_solve_batch(rpd_ctx *ctx, int32_t nreq, rpd_domain *const *doms, const double *const *xyzw, const int64_t *ns,
+                    const rpd_params *prm, rpd_stats *stats);
+/* Range [first, first + count) of request r's cells after rpd_solve_batch. */
+int rpd_batch_range(const rpd_ctx *ctx, int32_t r, int64_t *first, int64_t *count);
+/* count cells' moments (4 doubles each) from cell `first` of the last solve. */
+int rpd_read_moments_range(rpd_ctx *ctx, int64_t first, int64_t count, double *out4);
+
 /* Debug counters of pass `pass` (0 the first, k the k-th re-queue pass; the
    two re-queue chains add into the same k) of the last solve run with
    debug_counters set (the reference's debugCounter0..4, ConvexCell.cl:15-39,
@@ -325,14 +347,18 @@ int rpd_compact_used_device(rpd_ctx *ctx, double keep_volume, int32_t keep_dista
 /* ---- serving --------------------------------------------------------------
    A request loop for streams of solves with different seed counts and
    weights against resident domains (the inner step of Lloyd relaxation or
-   semi-discrete transport).  Requests that arrive while others run are
-   admitted as slots and HBM budget free up; admitted requests share the
-   device (continuous batching of their cell work into one queue). */
+   semi-discrete transport).  The queued requests are packed into batches
+   (rpd_solve_batch: one launch of each kernel over the cells of every
+   request of the batch), in arrival order, as long as their parameters are
+   equal and their footprints fit the HBM budget.  Up to max_inflight
+   batches run at once, each on its own context and streams, so a batch is
+   admitted while the previous one finishes its tail.  A request's result is
+   the same alone and in any batch. */
 typedef struct rpd_server rpd_server;
 typedef struct {
-    int64_t hbm_budget_bytes;     /* admission limit on request footprints   */
-    int32_t max_inflight;         /* requests resident at once               */
-    int32_t reserved;
+    int64_t hbm_budget_bytes;     /* admission limit on the footprints in flight (default 64 GiB) */
+    int32_t max_inflight;         /* batches running at once (default 2, at most 16)              */
+    int32_t max_batch;            /* requests per batch (default 256, at most RPD_BATCH_MAX_REQUESTS) */
 } rpd_server_config;
 
 int rpd_server_create(rpd_ctx *ctx, const rpd_server_config *cfg, rpd_server **out);

@@ -40,13 +40,10 @@ def _fp32_eval(p1, p2, p3, e, fused):
     W = a[0] * m23 - a[1] * m13 + a[2] * m12
     ev = [f(v) for v in e]
     sv = ev[0] * X + ev[1] * Y + ev[2] * Z + ev[3] * W
-    ab = lambda u, v: abs(mul(u, v))  # noqa: E731
-    a12 = ab(p2[0], p3[1]) + ab(p2[1], p3[0])
-    a13 = ab(p2[0], p3[2]) + ab(p2[2], p3[0])
-    a14 = ab(p2[0], p3[3]) + ab(p2[3], p3[0])
-    a23 = ab(p2[1], p3[2]) + ab(p2[2], p3[1])
-    a24 = ab(p2[1], p3[3]) + ab(p2[3], p3[1])
-    a34 = ab(p2[2], p3[3]) + ab(p2[3], p3[2])
+    # the kernel's sums |u_i v_j| + |u_j v_i| as fma(|u_i|, |v_j|, |u_j| |v_i|)
+    ab = lambda i, j: f(np.float64(abs(f(p2[i]))) * np.float64(abs(f(p3[j])))  # noqa: E731
+                        + np.float64(abs(f(p2[j])) * abs(f(p3[i]))))
+    a12, a13, a14, a23, a24, a34 = ab(0, 1), ab(0, 2), ab(0, 3), ab(1, 2), ab(1, 3), ab(2, 3)
     ax, ay, az, aw = (abs(v) for v in a)
     MX = aw * a23 + ay * a34 + az * a24
     MY = ax * a34 + aw * a13 + az * a14
