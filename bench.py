@@ -390,7 +390,8 @@ def main():
     t_finish = float(np.mean([s["ms_finish_first"] for s in sm])) * 1e-3
     roofline = roofline_cells = None
     if flop is not None:
-        ka, kb = "clip_cells_kernel<float, double, true, 1>", "finish_cells_kernel<float, double, true, 1, false>"
+        ka = "clip_cells_kernel<float, double, true, 1, false>"
+        kb = "finish_cells_kernel<float, double, true, 1, false, false>"
         roofline_cells = valu_roofline(*flop["mixed"]["cells"], n, t_cells, ka + " (mixed form, pass 1)",
                                        measured_traffic(ka))
         roofline = valu_roofline(*flop["mixed"]["finish"], n, t_finish, kb + " (mixed form, pass 1)",

@@ -1016,9 +1016,12 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         ch.pr.n_ids = ch.n_res;
         ch.pz.n_ids = ch.n_scr;
         ch.pr.caps[0] = nc.K - ch.cap.K;  // the neighbours after the ones used
-        // recomputed cells need the grown K only when some of them ran out
-        // of neighbours without a kept state (the record buffer was full)
-        ch.pz.caps[0] = std::min(256, cnt[2 + RPD_SECURITY_RADIUS_NOT_REACHED] > n_sec ? nc.K : ch.cap.K);
+        // recomputed cells list the grown K: a cell sent back for exact
+        // predicates or an overflow starts from scratch, and with the old K
+        // as many of them would run out of neighbours again as in the
+        // previous pass (one more pass for them); the loop reads only the
+        // neighbours it needs
+        ch.pz.caps[0] = std::min(256, nc.K);
         ch.pr.caps[1] = ch.pz.caps[1] = nc.P;
         ch.pr.caps[2] = ch.pz.caps[2] = nc.V;
         ch.pr.caps[3] = ch.pz.caps[3] = nc.T;
