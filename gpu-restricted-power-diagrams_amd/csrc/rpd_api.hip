@@ -124,6 +124,8 @@ struct rpd_ctx {
     // states of resumable cells (two record buffers per chain, their
     // counters) and, per stored id, the record slot and the last neighbour
     DevBuf rs[4], rs_cnt, rslot, lastnb;
+    // record slots of pass 1's finish-kernel failures (laid out as their list)
+    DevBuf fslot;
     bool solved = false;
     hipEvent_t ev[4][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
     hipEvent_t dep[2] = {};    // per chain: orders work between its streams
@@ -716,6 +718,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     RPDCHK(c, c->rs_cnt.ensure(4 * sizeof(int)), "alloc resume counters");
     RPDCHK(c, c->rslot.ensure(size_t(n) * 4), "alloc resume slots");
     RPDCHK(c, c->lastnb.ensure(size_t(n) * 4), "alloc resume slots");
+    RPDCHK(c, c->fslot.ensure(size_t(n) * 8), "alloc failure slots");
     RPDCHK(c, c->cnt.ensure(64), "alloc cnt");
     c->hess_H = hess ? prm->H : 0;
     c->hess_HC = hess ? prm->HC : 1;
@@ -974,7 +977,9 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         Pass pr, pz;
         ClipLaunch Lr, Lz;
         Resume rz;
+        bool from_finish = false;  // the chain of pass 1's finish-kernel failures
     };
+    ClipLaunch L1{};  // pass 1's launch; its finish kernel's records serve chain B's first pass
     auto cnt_pin = [&](int list) { return c->pin + kPinCounters + kFailCounters * list; };
     // plans the chain's next pass from the counters and queues its kNN and
     // bisector kernels (phase 1), or ends the chain (phase 3)
@@ -1001,6 +1006,58 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         ch.nc = nc;
         ch.nxt = ch.outs[ch.k & 1];
         if (!clear_list(ch.nxt, s)) return RPD_E_DEVICE;
+        // Chain B's first pass: pass 1's finish kernel failed these cells in
+        // the restriction (a flagged test, or the triangle, vertex or plane
+        // cap met by the cone clips).  Their bisector loop is done,
+        // certified, and in pass 1's records, which are the FP64 form's
+        // records too, so the FP64 finish kernel runs over those records
+        // with the grown caps; a full FP64 recomputation would rebuild the
+        // same records first.  The FP32 form's records hold FP32 planes (16
+        // bytes, a different layout): its cells take the full pass.
+        if (ch.from_finish && ch.k == 0 && ch.n_res == 0 && ch.n_scr > 0 && form0 != RPD_PREC_FP32 &&
+            rpd_clip_record_bytes(form0, ch.cap.P, ch.cap.V) == rpd_clip_record_bytes(RPD_PREC_FP64, ch.cap.P, ch.cap.V)) {
+            ch.rz = Resume{};
+            ch.pr = Pass{};
+            ch.pz = Pass{};
+            ch.pz.slot = ch.sl;
+            ch.pz.index = 1;
+            ch.pz.form = RPD_PREC_FP64;
+            ch.pz.n_ids = ch.n_scr;
+            ch.pz.caps[0] = ch.cap.K;
+            ch.pz.caps[1] = ch.cap.P;
+            ch.pz.caps[2] = ch.cap.V;
+            ch.pz.caps[3] = nc.T;
+            ch.pz.caps[1] = nc.P;
+            ch.pz.caps[2] = nc.V;
+            ClipLaunch &L = ch.Lz;
+            L = L1;
+            L.rec_P = ch.cap.P;  // the records' layout: pass 1's caps
+            L.rec_V = ch.cap.V;
+            L.P = nc.P;
+            L.V = nc.V;
+            L.T = nc.T;
+            L.slots = c->fslot.as<int>() + n - n_exact;  // the failures' range of the list, in place
+            L.fl_slot = nullptr;
+            L.fl_cnt = c->fl[ch.nxt].cnt.as<int>();
+            L.fl_eo = c->fl[ch.nxt].eo.as<int>() + n;
+            L.fl_sec = c->fl[ch.nxt].sec.as<int>();
+            L.fl_cnt_host = nullptr;
+            L.rs_out = nullptr;
+            L.rs_in = nullptr;
+            L.ev_mid = nullptr;
+            L.dcnt = dcnt_of(1);
+            if (hipEventRecord(c->ev[ch.sl][0], s) != hipSuccess || hipEventRecord(c->ev[ch.sl][1], s) != hipSuccess ||
+                hipEventRecord(c->ev[ch.sl][2], s) != hipSuccess)
+                return RPD_E_DEVICE;
+            int r = rpd_launch_finish(s, RPD_PREC_FP64, L, int(ch.n_scr), 0);
+            if (r != RPD_OK) return r;
+            if (hipEventRecord(c->ev[ch.sl][3], s) != hipSuccess ||
+                hipMemcpyAsync(cnt_pin(ch.nxt), c->fl[ch.nxt].cnt.ptr, kFailCounters * sizeof(int),
+                               hipMemcpyDeviceToHost, s) != hipSuccess)
+                return RPD_E_DEVICE;
+            ch.phase = 2;
+            return RPD_OK;
+        }
         ch.rz = Resume{};
         ch.rz.out = ch.rsb[ch.k & 1];
         ch.rz.out_stride = int(rpd_clip_record_bytes(RPD_PREC_FP64, nc.P, nc.V));
@@ -1115,7 +1172,6 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         left1.P = P;
         RPDCHK(c, resume_buf(0, cap_rs1, left1.stride, c->streams[0]), "alloc resume records");
     }
-    ClipLaunch L1{};
     {
         hipStream_t s = c->streams[0];
         PassBufs &b = c->pb[0];
@@ -1177,6 +1233,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         L.fl_eo = c->fl[1].eo.as<int>() + n;
         L.fl_sec = c->fl[1].sec.as<int>();
         L.fl_cnt_host = nullptr;
+        L.fl_slot = c->fslot.as<int>() + n;  // the finish kernel's failures keep their record slots
     }
     const Caps caps1{K, P, V, T};
     Chain ca, cb;
@@ -1194,6 +1251,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     cb.cnt = cnt_pin(1);
     cb.cap = caps1;
     cb.phase = 4;
+    cb.from_finish = true;
     RPDCHK(c, chain_front(ca), "re-queue chain (bisector failures)");
     {
         hipStream_t s = c->streams[0];

@@ -117,6 +117,15 @@ struct ClipArgs {
     // n_seeds, wmin and the domain's
     const ReqDesc *reqs;
     const uint16_t *req_of;
+    // finish kernel: work item i is the record at slot slots[i] (nullable:
+    // slot i); fl_slot (nullable) receives the record slot of each failure it
+    // lists, at the failure's index in fl_eo
+    const int *slots;
+    int *fl_slot;
+    // layout of the records the finish kernel reads: planes and vertex
+    // slots they were written with (the finish kernel's own P and S may be
+    // larger)
+    int rec_P, rec_S;
 };
 
 // The cell of stored id sid reads its request's seed count, weight minimum
@@ -145,7 +154,8 @@ __device__ __forceinline__ void bind_request(ClipArgs &a, int sid) {
 // (vertex, plane, K, triangle) always; needs_exact_predicates,
 // inconsistent_boundary and find_another_beginning_vertex when the form was
 // not FP64 (in FP64 a recomputation would fail the same way).  One lane.
-__device__ __forceinline__ void note_failure(const ClipArgs &a, int sid, int st, bool resumable = false) {
+__device__ __forceinline__ void note_failure(const ClipArgs &a, int sid, int st, bool resumable = false,
+                                             int rslot = -1) {
     if (!a.fl_cnt || st == RPD_SUCCESS || st == RPD_EMPTY_CELL || st == RPD_HESSIAN_OVERFLOW) return;
     const bool exact = st == RPD_NEEDS_EXACT_PREDICATES || st == RPD_INCONSISTENT_BOUNDARY ||
                        st == RPD_FIND_ANOTHER_BEGINNING_VERTEX;
@@ -157,6 +167,7 @@ __device__ __forceinline__ void note_failure(const ClipArgs &a, int sid, int st,
     }
     const int k = atomicAdd(a.fl_cnt + (exact ? 0 : 1), 1);
     a.fl_eo[exact ? -1 - k : k] = sid;
+    if (a.fl_slot) a.fl_slot[exact ? -1 - k : k] = rslot;
 }
 
 __device__ __forceinline__ uint32_t pack3(uint32_t a, uint32_t b, uint32_t c) { return a | (b << 8) | (c << 16); }
@@ -935,8 +946,9 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     const int wpb = blockDim.x >> 6;
     const int cpb = wpb / W;  // cells per block; W > 1 implies cpb == 1
     const int ci = wv / W, sub = wv - ci * W;
-    const int slot = int(blockIdx.x) * cpb + ci;
-    if (slot >= nq) return;  // uniform over the waves of a cell
+    const int item = int(blockIdx.x) * cpb + ci;
+    if (item >= nq) return;  // uniform over the waves of a cell
+    const int slot = a.slots ? uni(a.slots[item]) : item;
 
     char *cell_l = lds_raw + size_t(ci) * a.cell_stride;
     int *fails = reinterpret_cast<int *>(cell_l);                  // [W][2]
@@ -960,7 +972,8 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     const int nvalid = uni(hd.nvalid);
     c.nb_plane = uni(hd.nb_plane);
     const Pl<RP> *rpl = reinterpret_cast<const Pl<RP> *>(rec + sizeof(RecHeader));
-    const uint32_t *rtr = reinterpret_cast<const uint32_t *>(rec + sizeof(RecHeader) + size_t(P) * sizeof(Pl<RP>));
+    const uint32_t *rtr =
+        reinterpret_cast<const uint32_t *>(rec + sizeof(RecHeader) + size_t(a.rec_P) * sizeof(Pl<RP>));
     for (int p = lane; p < c.nb_plane; p += 64) c.planes[p] = rpl[p];
     for (int i = lane; i < 256; i += 64) c.nxt[i] = 0xff;
     wsync();
@@ -984,7 +997,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
             a.out[orig] = make_double4(m0, m1, m2, m3);
             a.status[orig] = st;
             if (a.kused) a.kused[orig] = kused;
-            note_failure(a, sid, st);
+            note_failure(a, sid, st, false, slot);
         }
     };
     auto fail = [&](int st) { finish(st, 0.0, 0.0, 0.0, 0.0); };
@@ -1583,7 +1596,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     // H written
     int st_out = RPD_SUCCESS;
     if (HESS) {
-        const int *rnid = reinterpret_cast<const int *>(rtr + 64 * S);
+        const int *rnid = reinterpret_cast<const int *>(rtr + 64 * a.rec_S);
         const int H = a.H;
         int cnt = 0;
         for (int pb = 0; pb < nbk; pb += 64) {
@@ -1706,7 +1719,11 @@ static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     a.status = L.status;
     a.kused = L.kused;
     a.rec = static_cast<char *>(L.rec);
-    a.rec_stride = int(rpd_clip_record_bytes(form, L.P, L.V));
+    const int rec_P = L.rec_P > 0 ? L.rec_P : L.P, rec_V = L.rec_V > 0 ? L.rec_V : L.V;
+    if (rec_P > L.P || rec_V > L.V) return RPD_E_ARG;
+    a.rec_stride = int(rpd_clip_record_bytes(form, rec_P, rec_V));
+    a.rec_P = rec_P;
+    a.rec_S = rec_V <= 64 ? 1 : (rec_V <= 128 ? 2 : 4);
     a.qcount = L.qcount;
     a.fl_cnt = L.fl_cnt;
     a.fl_eo = L.fl_eo;
@@ -1728,6 +1745,8 @@ static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     a.dcnt = L.dcnt;
     a.reqs = L.reqs;
     a.req_of = L.req_of;
+    a.slots = L.slots;
+    a.fl_slot = L.fl_slot;
     return RPD_OK;
 }
 

@@ -176,6 +176,13 @@ struct ClipLaunch {
     // packed batch (nullable): per request fields, request of each stored id
     const ReqDesc *reqs;
     const uint16_t *req_of;
+    // finish kernel: records at these slots instead of 0..nq-1 (nullable),
+    // and where the record slot of each listed failure goes (nullable, laid
+    // out as fl_eo)
+    const int *slots;
+    int *fl_slot;
+    // finish kernel: the P and V the records were written with (0: P, V)
+    int rec_P, rec_V;
 };
 
 constexpr int kFailCounters = 16;

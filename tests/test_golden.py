@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import ROOT, moment_errors, oracle_grid
+from conftest import ROOT, cell_errors, moment_errors, oracle_grid
 
 GOLD = os.path.join(ROOT, "tests", "golden")
 
@@ -62,15 +62,23 @@ def test_oracle_reproduces_knn_golden(orc):
 
 
 @pytest.mark.gpu
-def test_gpu_c1_golden(rpd, ctx):
+def test_gpu_c1_golden(rpd, orc, ctx):
+    """The CPU configuration against its committed fixtures (of the mean cell
+    volume), and per cell against the FP64 oracle in each seed's frame."""
     x, grid, z = _c1(rpd)
     dom = ctx.domain(grid)
     ctx.set_seeds(x)
+    ref1, _, _, _ = orc.solve(x, oracle_grid(orc, grid), K=50, P=30, V=40, T=100, frame=1)
     for prec, ref, tol in ((rpd.PREC_FP64, "moments_fp64", 1e-12), (rpd.PREC_MIXED, "moments_fp64", 1e-6),
                            (rpd.PREC_FP32, "moments_fp32", 1e-4)):
         ctx.solve(dom, K=50, P=30, V=40, T=100, precision=prec)
-        ev, em = moment_errors(ctx.moments(), z[ref], x.shape[0])
+        got = ctx.moments()
+        ev, em = moment_errors(got, z[ref], x.shape[0])
         assert ev < tol and em < tol, (prec, ev, em)
+        if prec != rpd.PREC_FP32:
+            cv, cb, nc = cell_errors(got, ref1, x.shape[0])
+            print("C1", prec, "per cell", cv, cb, nc, flush=True)
+            assert cv <= tol and cb <= tol and nc > 1000, (prec, cv, cb, nc)
     dom.close()
 
 
