@@ -107,10 +107,10 @@ def cpu_model():
     return "unknown"
 
 
-SLOTS = 4  # batches in flight in the serving stream
+SLOTS = 2  # batches in flight in the serving stream (2 streams each: the 4 hardware queues)
 
 
-def serving(rpd, ctx, n_req, log):
+def serving(rpd, ctx, n_req, log, slots=SLOTS):
     """Serving stream: Poisson arrivals, seed counts log-uniform in [1e3, 1e6],
     uniform seeds with w ~ U[0, 0.25 h^2], one of 4 resident domains at 96^3,
     successive requests on one seed set with weights perturbed by
@@ -138,7 +138,7 @@ def serving(rpd, ctx, n_req, log):
     cells = sum(x.shape[0] for _, x in reqs)
 
     def run(rate):
-        srv = rpd.Server(ctx, max_inflight=SLOTS)
+        srv = rpd.Server(ctx, max_inflight=slots)
         lat = []
         done = threading.Event()
 
@@ -187,7 +187,7 @@ def serving(rpd, ctx, n_req, log):
     res = {"requests": len(reqs), "cells": int(cells), "arrival_rate_per_s": rate,
            "backtoback_cells_per_s": cells / t_b2b, "stream_cells_per_s": cells / t_str,
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-           "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)), "slots": SLOTS,
+           "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)), "slots": slots,
            "batching": "packed batches (rpd_solve_batch) of the queued requests, up to 2^19 seeds each",
            "workload": "Poisson arrivals at 70% of back-to-back throughput; n log-uniform [1e3,1e6]; 4 domains "
                        "@96^3; weights perturbed N(0,0.01h^2) per reuse; mixed K50 P32 V48"}
@@ -244,6 +244,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-only", action="store_true",
                     help="only run the serving stream (prints its JSON object)")
+    ap.add_argument("--serve-slots", type=int, default=SLOTS, help="batches in flight in the serving stream")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference kernel for HBM counter scaling)")
     ap.add_argument("--mesh-grid", action="store_true",
@@ -259,7 +260,7 @@ def main():
         import __graft_entry__ as ge
         rpd = ge.load_package()
         ctx = rpd.Context(0)
-        print(json.dumps(serving(rpd, ctx, args.serve_requests, log)), flush=True)
+        print(json.dumps(serving(rpd, ctx, args.serve_requests, log, args.serve_slots)), flush=True)
         ctx.close()
         return
 

@@ -76,7 +76,8 @@ constexpr int kWminBlocks = 256;
 struct rpd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;      // == streams[0]
-    hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};  // logical streams (two may share one)
+    int n_streams = 4;                                                 // streams created
     std::string err;
     int64_t n = 0;
     DevBuf seeds;  // original order, FP64 xyzw
@@ -371,10 +372,22 @@ float elapsed_ms(hipEvent_t a, hipEvent_t b) {
 // device of a context (used by the serving loop to open its slots)
 int rpd_ctx_device(const rpd_ctx *c) { return c ? c->device : 0; }
 
+// A context on n_streams HIP streams (4 or 2).  Its passes are laid out on 4
+// logical streams: 0 pass 1 and the finish failures' chain, 1 and 2 the
+// bisector failures' chain (resumed and recomputed cells), 3 the finish
+// failures' recomputed cells.  With 2 streams, 0 and 3 share one, 1 and 2
+// the other: pass 1's finish kernel still overlaps the bisector failures'
+// chain, and a serving slot holds 2 of the device's hardware queues.
+int rpd_create_streams(int device, int n_streams, rpd_ctx **out);
+
 extern "C" {
 
-int rpd_create(int device, rpd_ctx **out) {
-    if (!out) return RPD_E_ARG;
+int rpd_create(int device, rpd_ctx **out) { return rpd_create_streams(device, 4, out); }
+
+}  // extern "C"
+
+int rpd_create_streams(int device, int n_streams, rpd_ctx **out) {
+    if (!out || (n_streams != 2 && n_streams != 4)) return RPD_E_ARG;
     *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RPD_E_DEVICE;
@@ -383,11 +396,16 @@ int rpd_create(int device, rpd_ctx **out) {
     if (resolve_scratch(device) != RPD_OK) return RPD_E_DEVICE;
     auto *c = new rpd_ctx();
     c->device = device;
-    for (auto &s : c->streams)
-        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+    for (int i = 0; i < n_streams; ++i)
+        if (hipStreamCreateWithFlags(&c->streams[i], hipStreamNonBlocking) != hipSuccess) {
             rpd_destroy(c);
             return RPD_E_DEVICE;
         }
+    if (n_streams == 2) {
+        c->streams[2] = c->streams[1];
+        c->streams[3] = c->streams[0];
+    }
+    c->n_streams = n_streams;
     c->stream = c->streams[0];
     for (auto &es : c->ev)
         for (auto &e : es)
@@ -416,18 +434,20 @@ int rpd_create(int device, rpd_ctx **out) {
     return RPD_OK;
 }
 
+extern "C" {
+
 void rpd_destroy(rpd_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
-    for (auto &s : c->streams)
-        if (s) hipStreamSynchronize(s);
+    for (int i = 0; i < c->n_streams; ++i)
+        if (c->streams[i]) hipStreamSynchronize(c->streams[i]);
     for (auto &es : c->ev)
         for (auto &e : es)
             if (e) hipEventDestroy(e);
     for (auto &e : c->dep)
         if (e) hipEventDestroy(e);
-    for (auto &s : c->streams)
-        if (s) hipStreamDestroy(s);
+    for (int i = 0; i < c->n_streams; ++i)
+        if (c->streams[i]) hipStreamDestroy(c->streams[i]);
     if (c->pin) hipHostFree(c->pin);
     if (c->pin_red) hipHostFree(c->pin_red);
     delete c;

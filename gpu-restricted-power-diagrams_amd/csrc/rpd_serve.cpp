@@ -6,8 +6,11 @@
 // per-launch buffers reset after each one (voronoi.cpp:1401).
 //
 // Design: a FIFO of submitted requests and `max_inflight` batch slots.  A
-// slot is a solver context of its own (its own HIP streams and resident
-// buffers, which grow to the largest batch it has served and are reused).  A
+// slot is a solver context of its own (two HIP streams and resident buffers,
+// which grow to the largest batch it has served and are reused).  HIP maps
+// streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default): with the
+// default 2 slots every stream has a queue of its own, so no slot's kernels
+// wait in order behind another slot's.  A
 // free slot takes the longest run of requests at the head of the queue that
 // share the head's parameters and whose device footprints fit in what the
 // HBM budget has left (at most max_batch requests and kBatchMaxSeeds seeds),
@@ -32,7 +35,8 @@
 
 #include "rpd.h"
 
-int rpd_ctx_device(const rpd_ctx *ctx);  // rpd_api.hip
+int rpd_ctx_device(const rpd_ctx *ctx);                            // rpd_api.hip
+int rpd_create_streams(int device, int n_streams, rpd_ctx **out);  // rpd_api.hip
 
 namespace {
 
@@ -183,7 +187,9 @@ int rpd_server_create(rpd_ctx *ctx, const rpd_server_config *cfg, rpd_server **o
     s->cfg.max_batch = cfg && cfg->max_batch > 0 ? std::min(cfg->max_batch, RPD_BATCH_MAX_REQUESTS) : 256;
     for (int i = 0; i < s->cfg.max_inflight; ++i) {
         rpd_ctx *c = nullptr;
-        const int rc = rpd_create(dev, &c);
+        // two streams per slot: the default two slots hold the device's
+        // four hardware queues, one stream each
+        const int rc = rpd_create_streams(dev, 2, &c);
         if (rc != RPD_OK) {
             for (rpd_ctx *q : s->slots) rpd_destroy(q);
             delete s;
