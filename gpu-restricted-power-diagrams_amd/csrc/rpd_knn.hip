@@ -8,12 +8,13 @@
 //   * query: one wavefront per query seed over the sorted array.  Candidate
 //     cells are visited shell by shell (Chebyshev rings with per-axis bounds);
 //     each row of a shell is one contiguous range of the sorted array, so the
-//     64 lanes load consecutive seeds.  The running top-K is held across the
-//     lanes (lane i holds rank i, two registers per lane when K > 64) and is
-//     updated by a bitonic sort/merge or, for sparse batches, by ordered
-//     insertion.  The search stops once the K-th distance is below the lower
-//     bound of the next shell, so the result is the exact K nearest under the
-//     key (d^2, original index), d^2 = ((dx*dx + dy*dy) + dz*dz) unfused.
+//     64 lanes load consecutive seeds.  Candidates below the running K-th key
+//     are kept in an LDS buffer; when it fills, and at the end of each shell,
+//     a wave radix select finds the exact K-th key and the buffer keeps the K
+//     keys up to it.  The search stops once the K-th distance is below the
+//     lower bound of the next shell, and one bitonic sort orders the K kept
+//     keys, so the result is the exact K nearest under the key (d^2, original
+//     index), d^2 = ((dx*dx + dy*dy) + dz*dz) unfused.
 
 #include <hip/hip_runtime.h>
 #include <cstring>
