@@ -316,45 +316,51 @@ __device__ int select_rank(const typename KeyT::Buf &buf, int M, int rank, uint3
         }
     }
     uint4 *h4 = reinterpret_cast<uint4 *>(hist);
-#pragma unroll 1
-    for (int pass = 0; pass < 4 * KeyT::kWords; ++pass) {
-        const int q = pass >> 2, sh = 24 - 8 * (pass & 3);
-        h4[lane] = make_uint4(0u, 0u, 0u, 0u);
-        wave_lds_sync();
+    // words in an unrolled loop, digits of a word in a rolled one: the word
+    // index stays a constant, so w[j][q] is a register, not a select chain
+    bool one = false;
 #pragma unroll
-        for (int j = 0; j < SL; ++j)
-            if ((alive >> j) & 1) atomicAdd(&hist[(w[j][q] >> sh) & 255u], 1u);
-        wave_lds_sync();
-        const uint4 h = h4[lane];
-        const int sum = int(h.x + h.y + h.z + h.w);
-        const int incl = wave_incl_scan_op(sum, 0, [](int a, int b) { return a + b; });
-        const int excl = incl - sum;
-        const int L = lowest_bit(ballot(excl <= rank && rank < incl));
-        // the bin of lane L's four that holds the rank
-        int before = rlane(excl, L);
-        const int c0 = int(rlane(h.x, L)), c1 = int(rlane(h.y, L)), c2 = int(rlane(h.z, L)), c3 = int(rlane(h.w, L));
-        int bin = 4 * L, cnt = c0;
-        if (rank >= before + c0) {
-            before += c0;
-            bin += 1;
-            cnt = c1;
-            if (rank >= before + c1) {
-                before += c1;
+    for (int q = 0; q < KeyT::kWords; ++q) {
+#pragma unroll 1
+        for (int sh = 24; sh >= 0 && !one; sh -= 8) {
+            h4[lane] = make_uint4(0u, 0u, 0u, 0u);
+            wave_lds_sync();
+#pragma unroll
+            for (int j = 0; j < SL; ++j)
+                if ((alive >> j) & 1) atomicAdd(&hist[(w[j][q] >> sh) & 255u], 1u);
+            wave_lds_sync();
+            const uint4 h = h4[lane];
+            const int sum = int(h.x + h.y + h.z + h.w);
+            const int incl = wave_incl_scan_op(sum, 0, [](int a, int b) { return a + b; });
+            const int excl = incl - sum;
+            const int L = lowest_bit(ballot(excl <= rank && rank < incl));
+            // the bin of lane L's four that holds the rank
+            int before = rlane(excl, L);
+            const int c0 = int(rlane(h.x, L)), c1 = int(rlane(h.y, L)), c2 = int(rlane(h.z, L)),
+                      c3 = int(rlane(h.w, L));
+            int bin = 4 * L, cnt = c0;
+            if (rank >= before + c0) {
+                before += c0;
                 bin += 1;
-                cnt = c2;
-                if (rank >= before + c2) {
-                    before += c2;
+                cnt = c1;
+                if (rank >= before + c1) {
+                    before += c1;
                     bin += 1;
-                    cnt = c3;
+                    cnt = c2;
+                    if (rank >= before + c2) {
+                        before += c2;
+                        bin += 1;
+                        cnt = c3;
+                    }
                 }
             }
-        }
-        rank -= before;
+            rank -= before;
 #pragma unroll
-        for (int j = 0; j < SL; ++j)
-            if (((w[j][q] >> sh) & 255u) != uint32_t(bin)) alive &= ~(1u << j);
-        wave_lds_sync();  // the histogram is cleared again by the next pass
-        if (cnt == 1) break;
+            for (int j = 0; j < SL; ++j)
+                if (((w[j][q] >> sh) & 255u) != uint32_t(bin)) alive &= ~(1u << j);
+            wave_lds_sync();  // the histogram is cleared again by the next pass
+            one = cnt == 1;
+        }
     }
     // the one key left (keys are distinct, so one is left after every digit)
     const int L = lowest_bit(ballot(alive != 0));
