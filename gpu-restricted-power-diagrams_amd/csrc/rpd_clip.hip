@@ -269,16 +269,22 @@ struct WaveCell {
     uint64_t valid[S];
     R maxr2;
 
+    // R2: also the vertex's squared distance from the seed (the bisector
+    // loop's stop test reads it through maxr2; the finish kernel does not)
+    template <bool R2 = true>
     __device__ void set_vertex(int s, uint32_t tr, const Pl<R> &p1, const Pl<R> &p2, const Pl<R> &p3) {
         tri[s] = tr;
         vertex_from_planes<R, FILTER>(p1, p2, p3, X[s], Y[s], Z[s], W[s], MXYZ[s], MW[s]);
-        const R iw = R(1) / W[s];
-        const R ix = X[s] * iw, iy = Y[s] * iw, iz = Z[s] * iw;
-        r2[s] = ix * ix + iy * iy + iz * iz;
+        if (R2) {
+            const R iw = R(1) / W[s];
+            const R ix = X[s] * iw, iy = Y[s] * iw, iz = Z[s] * iw;
+            r2[s] = ix * ix + iy * iy + iz * iz;
+        }
     }
 
+    template <bool R2 = true>
     __device__ void set_vertex_lds(int s, uint32_t tr) {
-        set_vertex(s, tr, cvt<R>(planes[t0(tr)]), cvt<R>(planes[t1(tr)]), cvt<R>(planes[t2(tr)]));
+        set_vertex<R2>(s, tr, cvt<R>(planes[t0(tr)]), cvt<R>(planes[t1(tr)]), cvt<R>(planes[t2(tr)]));
     }
 
     __device__ void refresh_max_r2() {
@@ -312,7 +318,9 @@ struct WaveCell {
 
     // One clip by plane e (wave-uniform, test precision).  Lane wl holds the
     // same plane in the storage precision in eP (wl < 0: every lane does).
-    // Returns kNoClip, kClipped or a failure status.
+    // Returns kNoClip, kClipped or a failure status.  R2: keep r2 and maxr2
+    // up to date.
+    template <bool R2 = true>
     __device__ int clip(const Pl<R> &e, int wl, const Pl<PS> &eP) {
         uint64_t rem[S];
         int nr = 0, nvalid = 0;
@@ -457,7 +465,7 @@ struct WaveCell {
             const uint64_t newm = ballot(take);
             if (take) {
                 const uint32_t tr = nvs[q];
-                set_vertex(s, tr, e, cvt<R>(planes[t1(tr)]), cvt<R>(planes[t2(tr)]));
+                set_vertex<R2>(s, tr, e, cvt<R>(planes[t1(tr)]), cvt<R>(planes[t2(tr)]));
             }
             if (FILTER) ill |= ballot(take && fabsf(float(W[s])) < kFilterMinW * MW[s]);
             carry += __popcll(fr);
@@ -465,7 +473,7 @@ struct WaveCell {
         }
         nb_plane = pid + 1;
         if (FILTER && ill) return RPD_NEEDS_EXACT_PREDICATES;
-        refresh_max_r2();
+        if (R2) refresh_max_r2();
         return kClipped;
     }
 };
@@ -985,10 +993,11 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         c.W[s] = R(1);
         c.r2[s] = R(0);
         c.MXYZ[s] = c.MW[s] = 0.f;
-        if (j < nvalid) c.set_vertex_lds(s, rtr[j]);
+        if (j < nvalid) c.template set_vertex_lds<false>(s, rtr[j]);
         c.valid[s] = ballot(j < nvalid);
     }
-    c.refresh_max_r2();
+    // (the vertices' distances from the seed are not kept here: the
+    // restriction takes the cell's radius in FP64 below)
 
     const double4 S0 = a.seeds[sid];
     const int orig = a.perm[sid];
@@ -1536,7 +1545,8 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                 const Pl<RP> *qs[4] = {&q0, &q1, &q2, &q3};
 #pragma unroll
                 for (int f2 = 0; f2 < 4; ++f2)
-                    if (res < 0) res = c.nb_plane >= P ? int(RPD_PLANE_OVERFLOW) : c.clip(cvt<R>(*qs[f2]), -1, *qs[f2]);
+                    if (res < 0)
+                        res = c.nb_plane >= P ? int(RPD_PLANE_OVERFLOW) : c.template clip<false>(cvt<R>(*qs[f2]), -1, *qs[f2]);
                 if (res >= 0 && res != RPD_EMPTY_CELL) {
                     fail_t = tid;
                     fail_st = res;
