@@ -806,18 +806,21 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     }
     bool any_cover = false;
     for (const auto &ca : cov) any_cover = any_cover || ca.apply;
+    // The kept cells' count comes back through pinned memory without a wait:
+    // pass 1's kNN is queued over n ids and reads the count on the device,
+    // and the host waits for the count (event ev[1][1]) only after that
+    // launch, so the wait overlaps the kNN.
+    const int *cover_count = nullptr;
+    int *cover_pin = c->pin + kPinCounters - 3;
     if (any_cover) {
         RPDCHK(c, c->ids1.ensure(size_t(n) * 4), "alloc ids");
-        int kept = 0;
         HIPCHK(c, hipEventRecord(c->ev[1][0], c->stream));
-        RPDCHK(c, rpd_cover_skip(c->stream, cov.data(), int(cov.size()), n, c->cull_scratch, c->ids1.as<int>(), &kept),
+        RPDCHK(c,
+               rpd_cover_skip(c->stream, cov.data(), int(cov.size()), n, c->cull_scratch, c->ids1.as<int>(), cover_pin,
+                              &cover_count),
                "domain cover");
         HIPCHK(c, hipEventRecord(c->ev[1][1], c->stream));
-        HIPCHK(c, hipEventSynchronize(c->ev[1][1]));
-        st.ms_cover = elapsed_ms(c->ev[1][0], c->ev[1][1]);
         ids1 = c->ids1.as<int>();
-        n1 = kept;
-        st.skipped_cells = n - kept;
     }
     const ReqDesc *reqs = batch ? c->req_desc.as<ReqDesc>() : nullptr;
     const KnnReq *kreqs = batch ? c->knn_req.as<KnnReq>() : nullptr;
@@ -1174,36 +1177,44 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     //   * chain B, the finish kernel's failures, once it is done (streams 0,
     //     3), with the caps of pass 1 grown only for them.
     if (!clear_list(0, c->stream) || !clear_list(1, c->stream)) return set_err(c, RPD_E_DEVICE, "clear lists");
-    p1.slot = 0;
-    p1.form = form0;
-    p1.n_ids = n1;
-    p1.caps[0] = K;
-    p1.caps[1] = P;
-    p1.caps[2] = V;
-    p1.caps[3] = T;
-    // states of pass 1's cells whose K neighbours do not reach their
-    // security radius are kept (FP64 planes: FP64 and mixed forms), for up to
-    // 40% of the cells; the others are recomputed from scratch
     Left left1;
-    const int64_t cap_rs1 = std::min<int64_t>(n1, n1 * 2 / 5 + 1024);
-    if (form0 != RPD_PREC_FP32 && prm->max_extra_steps > 0 && n1 > 0) {
-        left1.which = 0;
-        left1.stride = int(rpd_clip_record_bytes(form0, P, V));
-        left1.P = P;
-        RPDCHK(c, resume_buf(0, cap_rs1, left1.stride, c->streams[0]), "alloc resume records");
-    }
     {
         hipStream_t s = c->streams[0];
         PassBufs &b = c->pb[0];
+        // pass 1's kNN: over the kept cells, their count read on the device
+        // while the host does not know it yet (n rows at most)
         RPDCHK(c, b.knn.ensure(size_t(std::max<int64_t>(n1, 1)) * K * 4), "alloc knn");
-        RPDCHK(c, b.rec.ensure(size_t(std::max<int64_t>(n1, 1)) * rpd_clip_record_bytes(form0, P, V)), "alloc records");
-        RPDCHK(c, b.qcount.ensure(sizeof(int)), "alloc qcount");
         HIPCHK(c, hipEventRecord(c->ev[0][0], s));
         RPDCHK(c,
                rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids1, n1, K,
-                              b.knn.as<int>(), nullptr, kreqs, req_of),
+                              b.knn.as<int>(), nullptr, kreqs, req_of, cover_count),
                "pass 1 knn");
         HIPCHK(c, hipEventRecord(c->ev[0][1], s));
+        if (cover_count) {
+            HIPCHK(c, hipEventSynchronize(c->ev[1][1]));
+            st.ms_cover = elapsed_ms(c->ev[1][0], c->ev[1][1]);
+            n1 = *cover_pin;
+            st.skipped_cells = n - n1;
+        }
+        p1.slot = 0;
+        p1.form = form0;
+        p1.n_ids = n1;
+        p1.caps[0] = K;
+        p1.caps[1] = P;
+        p1.caps[2] = V;
+        p1.caps[3] = T;
+        // states of pass 1's cells whose K neighbours do not reach their
+        // security radius are kept (FP64 planes: FP64 and mixed forms), for
+        // up to 40% of the cells; the others are recomputed from scratch
+        const int64_t cap_rs1 = std::min<int64_t>(n1, n1 * 2 / 5 + 1024);
+        if (form0 != RPD_PREC_FP32 && prm->max_extra_steps > 0 && n1 > 0) {
+            left1.which = 0;
+            left1.stride = int(rpd_clip_record_bytes(form0, P, V));
+            left1.P = P;
+            RPDCHK(c, resume_buf(0, cap_rs1, left1.stride, c->streams[0]), "alloc resume records");
+        }
+        RPDCHK(c, b.rec.ensure(size_t(std::max<int64_t>(n1, 1)) * rpd_clip_record_bytes(form0, P, V)), "alloc records");
+        RPDCHK(c, b.qcount.ensure(sizeof(int)), "alloc qcount");
         ClipLaunch &L = L1;
         L.seeds = c->stored64.as<double4>();
         L.perm = c->perm.as<int>();

@@ -96,9 +96,10 @@ __global__ __launch_bounds__(256) void skip_kernel(const double4 *__restrict__ s
 using namespace rpd;
 
 int rpd_cover_skip(hipStream_t st, const CoverArgs *reqs, int nreq, int64_t n_total, DevBuf &scratch, int *ids_out,
-                   int *n_out) {
-    *n_out = 0;
-    if (n_total <= 0 || nreq <= 0) return RPD_OK;
+                   int *n_out, const int **count_dev) {
+    if (count_dev) *count_dev = nullptr;
+    if (!count_dev) *n_out = 0;
+    if (n_total <= 0 || nreq <= 0) return RPD_E_ARG;
     const int64_t n = n_total;
     // scratch: M bits per request | keep flags | selected count | rocprim temp
     size_t sel_bytes = 0;
@@ -135,6 +136,10 @@ int rpd_cover_skip(hipStream_t st, const CoverArgs *reqs, int nreq, int64_t n_to
         hipSuccess)
         return RPD_E_DEVICE;
     if (hipMemcpyAsync(n_out, cnt, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return RPD_E_DEVICE;
-    if (hipStreamSynchronize(st) != hipSuccess) return RPD_E_DEVICE;
+    if (count_dev) {
+        *count_dev = cnt;
+    } else if (hipStreamSynchronize(st) != hipSuccess) {
+        return RPD_E_DEVICE;
+    }
     return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }

@@ -90,9 +90,11 @@ struct KnnReq {
 // lower set, the K after the neighbour lower[q] (stored-id indexed).  With
 // kreqs set, each query searches the grid of its request (req_of) instead of
 // (counters, dim).
+// n_dev (nullable): the number of queries in device memory, at most n_ids
+// (the launch covers n_ids; waves past the count exit).
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
                    const int *ids, int64_t n_ids, int K, int *out, const int *lower = nullptr,
-                   const KnnReq *kreqs = nullptr, const uint16_t *req_of = nullptr);
+                   const KnnReq *kreqs = nullptr, const uint16_t *req_of = nullptr, const int *n_dev = nullptr);
 // adds base to n ints (binning of one request of a packed batch)
 int rpd_add_base(hipStream_t st, int *v, int64_t n, int base);
 
@@ -221,9 +223,11 @@ struct CoverArgs {
 };
 // Writes the outputs of the skipped cells and the stored ids of the others,
 // ascending, to ids_out (n_total entries); *n_out = their number.  The
-// requests' ranges lie in [0, n_total).  Synchronizes st.
+// requests' ranges lie in [0, n_total).  count_dev null: synchronizes st.
+// Else nothing waits: *count_dev receives the device copy of the number and
+// *n_out (pinned host memory) is written by a copy queued on st.
 int rpd_cover_skip(hipStream_t st, const CoverArgs *reqs, int nreq, int64_t n_total, DevBuf &scratch, int *ids_out,
-                   int *n_out);
+                   int *n_out, const int **count_dev = nullptr);
 
 // Bytes of one queued-cell record.
 size_t rpd_clip_record_bytes(int form, int P, int V);

@@ -267,6 +267,8 @@ struct KnnArgs {
     // (counters, dim)
     const KnnReq *reqs;
     const uint16_t *req_of;
+    // nullable: the number of queries, in device memory (<= n_ids)
+    const int *n_dev;
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -368,32 +370,14 @@ __device__ int select_rank(const typename KeyT::Buf &buf, int M, int rank, uint3
     return L + 64 * jj;
 }
 
+// The K nearest of query qi (its stored id ids[qi]): the wave's LDS pieces
+// are cand (kCandBuf ints), hist (256 words) and the key buffer buf.
 template <typename R, int S>
-__global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> a) {
+__device__ __forceinline__ void knn_query(const KnnArgs<R> &a, int64_t qi, int lane, int *cand, uint32_t *hist,
+                                          const typename Key<R>::Buf &buf) {
     using KeyT = Key<R>;
     constexpr int KEEP = kKeep<S>;
     constexpr int SL = KEEP / 64;
-    __shared__ int cand_all[kKnnWavesPerBlock][kCandBuf];
-    __shared__ __attribute__((aligned(16))) uint32_t hist_all[kKnnWavesPerBlock][256];
-    __shared__ __attribute__((aligned(16))) char keep_all[kKnnWavesPerBlock][KEEP * KeyT::kBufBytes];
-    const int lane = lane_id();
-    const int wv = uni(int(threadIdx.x >> 6));  // wave-uniform: lets ids, seeds and pointers live in SGPRs
-    const int64_t qi = int64_t(blockIdx.x) * kKnnWavesPerBlock + wv;
-    if (qi >= a.n_ids) return;  // wave-uniform
-    int *cand = cand_all[wv];
-    uint32_t *hist = hist_all[wv];
-    typename KeyT::Buf buf;
-    {
-        char *kb = keep_all[wv];
-        if constexpr (sizeof(R) == 4) {
-            buf.k = reinterpret_cast<uint64_t *>(kb);
-            buf.s = reinterpret_cast<uint32_t *>(kb + 8 * KEEP);
-        } else {
-            buf.d = reinterpret_cast<uint64_t *>(kb);
-            buf.o = reinterpret_cast<uint32_t *>(kb + 8 * KEEP);
-            buf.s = reinterpret_cast<uint32_t *>(kb + 12 * KEEP);
-        }
-    }
     const int q = uni(a.ids ? a.ids[qi] : int(qi));
     const typename Vec4T<R>::T p = a.pts[q];
     int dim = a.dim;
@@ -569,6 +553,36 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
     }
 }
 
+// One wave per query.  With n_dev the launch covers an upper bound of the
+// queries (n_ids) and the waves past the count exit at once.  (A grid-stride
+// loop over fewer workgroups was measured slower: it holds more registers,
+// 67 against 56 VGPRs for FP32 keys.)
+template <typename R, int S>
+__global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> a) {
+    using KeyT = Key<R>;
+    constexpr int KEEP = kKeep<S>;
+    __shared__ int cand_all[kKnnWavesPerBlock][kCandBuf];
+    __shared__ __attribute__((aligned(16))) uint32_t hist_all[kKnnWavesPerBlock][256];
+    __shared__ __attribute__((aligned(16))) char keep_all[kKnnWavesPerBlock][KEEP * KeyT::kBufBytes];
+    const int lane = lane_id();
+    const int wv = uni(int(threadIdx.x >> 6));  // wave-uniform: lets ids, seeds and pointers live in SGPRs
+    const int64_t nq = a.n_dev ? min(a.n_ids, int64_t(uni(*a.n_dev))) : a.n_ids;
+    typename KeyT::Buf buf;
+    {
+        char *kb = keep_all[wv];
+        if constexpr (sizeof(R) == 4) {
+            buf.k = reinterpret_cast<uint64_t *>(kb);
+            buf.s = reinterpret_cast<uint32_t *>(kb + 8 * KEEP);
+        } else {
+            buf.d = reinterpret_cast<uint64_t *>(kb);
+            buf.o = reinterpret_cast<uint32_t *>(kb + 8 * KEEP);
+            buf.s = reinterpret_cast<uint32_t *>(kb + 12 * KEEP);
+        }
+    }
+    const int64_t qi = int64_t(blockIdx.x) * kKnnWavesPerBlock + wv;
+    if (qi < nq) knn_query<R, S>(a, qi, lane, cand_all[wv], hist_all[wv], buf);
+}
+
 }  // namespace rpd
 
 // ---------------------------------------------------------------------------
@@ -695,7 +709,7 @@ int rpd_add_base(hipStream_t st, int *v, int64_t n, int base) {
 
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
                    const int *ids, int64_t n_ids, int K, int *out, const int *lower, const KnnReq *kreqs,
-                   const uint16_t *req_of) {
+                   const uint16_t *req_of, const int *n_dev) {
     if (n_ids <= 0) return RPD_OK;
     if (K <= 0 || K > 256) return RPD_E_LIMIT;
     const unsigned blocks = unsigned((n_ids + kKnnWavesPerBlock - 1) / kKnnWavesPerBlock);
@@ -703,7 +717,7 @@ int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm
 #define RPD_KNN_LAUNCH(R, V4)                                                                       \
     do {                                                                                            \
         KnnArgs<R> a{static_cast<const V4 *>(stored), perm, counters, ids, n_ids, dim, K, out, lower, \
-                     kreqs, req_of};                                                                \
+                     kreqs, req_of, n_dev};                                                         \
         if (K <= 64)                                                                                \
             knn_kernel<R, 1><<<blocks, tb, 0, st>>>(a);                                             \
         else if (K <= 128)                                                                          \
