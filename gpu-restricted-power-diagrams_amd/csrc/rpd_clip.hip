@@ -259,9 +259,12 @@ constexpr uint64_t kCube2 = 0x0101030003020000ull;  // 0 0 2 3 0 3 1 1
 template <typename R, typename RP, bool FILTER, int S, typename PS = RP>
 struct WaveCell {
     Pl<PS> *planes;
-    uint32_t *rt;
     uint32_t *nvs;
     uint8_t *nxt;
+    // bit matrix of directed edges, P rows of ceil(P / 32) words: bit v of
+    // row u set while a removed vertex's triangle holds the edge u -> v
+    // (all zero between clips)
+    uint32_t *em;
     int P, V, nb_plane, lane;
     uint32_t tri[S];
     R X[S], Y[S], Z[S], W[S], r2[S];
@@ -343,33 +346,46 @@ struct WaveCell {
         if (nr == nvalid) return RPD_EMPTY_CELL;
         const int pid = nb_plane;
         if (lane == (wl < 0 ? 0 : wl)) planes[pid] = eP;
-        // removed triangles, in slot-major order
-        int carry = 0;
+        // boundary edges: an edge u->v of a removed triangle whose reverse
+        // v->u belongs to no removed triangle.  Every removed triangle marks
+        // its three directed edges in the bit matrix em (row u, bit v), reads
+        // the three reverse edges back, then clears the words it set.
+        uint32_t bm[S];
+        const uint32_t rw = uint32_t(P + 31) >> 5;  // words per row
+        // edge u -> v: word u * rw + v / 32, bit v % 32
+        auto edge_word = [&](uint32_t u, uint32_t v) { return em + (u * rw + (v >> 5)); };
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if ((rem[s] >> lane) & 1) rt[carry + mbcnt(rem[s])] = tri[s];
-            carry += __popcll(rem[s]);
+            if ((rem[s] >> lane) & 1) {
+                const uint32_t tr = tri[s];
+                const uint32_t a0 = t0(tr), a1 = t1(tr), a2 = t2(tr);
+                atomicOr(edge_word(a0, a1), 1u << (a1 & 31));
+                atomicOr(edge_word(a1, a2), 1u << (a2 & 31));
+                atomicOr(edge_word(a2, a0), 1u << (a0 & 31));
+            }
         }
         wsync();
-        // boundary edges: an edge u->v of a removed triangle whose reverse
-        // v->u belongs to no removed triangle
-        uint32_t bm[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             bm[s] = 0;
             if ((rem[s] >> lane) & 1) {
                 const uint32_t tr = tri[s];
                 const uint32_t a0 = t0(tr), a1 = t1(tr), a2 = t2(tr);
-                const uint32_t r01 = (a1 << 8) | a0, r12 = (a2 << 8) | a1, r20 = (a0 << 8) | a2;
-                bool h01 = false, h12 = false, h20 = false;
-                for (int j = 0; j < nr; ++j) {
-                    const uint32_t u = rt[j];
-                    const uint32_t e0 = (t0(u) << 8) | t1(u), e1 = (t1(u) << 8) | t2(u), e2 = (t2(u) << 8) | t0(u);
-                    h01 |= (r01 == e0) | (r01 == e1) | (r01 == e2);
-                    h12 |= (r12 == e0) | (r12 == e1) | (r12 == e2);
-                    h20 |= (r20 == e0) | (r20 == e1) | (r20 == e2);
-                }
+                const bool h01 = (*edge_word(a1, a0) >> (a0 & 31)) & 1u;  // a1 -> a0 removed too
+                const bool h12 = (*edge_word(a2, a1) >> (a1 & 31)) & 1u;
+                const bool h20 = (*edge_word(a0, a2) >> (a2 & 31)) & 1u;
                 bm[s] = (h01 ? 0u : 1u) | (h12 ? 0u : 2u) | (h20 ? 0u : 4u);
+            }
+        }
+        wsync();
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if ((rem[s] >> lane) & 1) {
+                const uint32_t tr = tri[s];
+                const uint32_t a0 = t0(tr), a1 = t1(tr), a2 = t2(tr);
+                *edge_word(a0, a1) = 0;
+                *edge_word(a1, a2) = 0;
+                *edge_word(a2, a0) = 0;
             }
         }
         // slot-major exclusive offsets of the new vertices
@@ -454,7 +470,7 @@ struct WaveCell {
         }
         if (inconsistent) return RPD_INCONSISTENT_BOUNDARY;
         // new vertices fill the lowest free slots
-        carry = 0;
+        int carry = 0;
         uint64_t ill = 0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
@@ -478,8 +494,8 @@ struct WaveCell {
     }
 };
 
-// LDS of one wave: planes[P] | plane sources src[P] (int32) | rt[64S] |
-// nvs[64S] | nxt[256] | tu[T] (finish kernel) | chunk / cone planes cpl[64] |
+// LDS of one wave: planes[P] | plane sources src[P] (int32) | nvs[64S] |
+// nxt[256] | em[P][ceil(P / 32)] | tu[T] (finish kernel) | chunk / cone planes cpl[64] |
 // go_out flags[16] | triangle corners tcor[16][9] (finish kernel)
 template <typename R, typename RP, bool FILTER, int S, typename PS>
 __device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, FILTER, S, PS> &c,
@@ -489,12 +505,12 @@ __device__ __forceinline__ void bind_lds(char *L, int P, int T, WaveCell<R, RP, 
     size_t off = (size_t(P) * sizeof(Pl<PS>) + 15) & ~size_t(15);
     if (src) *src = reinterpret_cast<int *>(L + off);
     off += (4 * size_t(P) + 15) & ~size_t(15);
-    c.rt = reinterpret_cast<uint32_t *>(L + off);
-    off += 4 * 64 * S;
     c.nvs = reinterpret_cast<uint32_t *>(L + off);
     off += 4 * 64 * S;
     c.nxt = reinterpret_cast<uint8_t *>(L + off);
     off += 256;
+    c.em = reinterpret_cast<uint32_t *>(L + off);
+    off += 4 * size_t(P) * size_t((P + 31) >> 5);
     if (tu) *tu = reinterpret_cast<int *>(L + off);
     off += 4 * size_t(T);
     off = (off + 15) & ~size_t(15);
@@ -518,8 +534,8 @@ constexpr size_t kVtxRecBytes = 40;  // sizeof(VtxRec<double>), the larger one
 // LDS bytes of one wave (pl: bytes of one stored plane; corners: the finish
 // kernel's triangle corners and vertex copy)
 __host__ __device__ constexpr size_t finish_wave_lds(int P, int S, int T, size_t pl, bool corners = false) {
-    return ((((((P * pl + 15) & ~size_t(15)) + ((4 * size_t(P) + 15) & ~size_t(15)) + 2 * 4 * 64 * size_t(S) +
-               256 + 4 * size_t(T)) +
+    return ((((((P * pl + 15) & ~size_t(15)) + ((4 * size_t(P) + 15) & ~size_t(15)) + 4 * 64 * size_t(S) +
+               256 + 4 * size_t(P) * size_t((P + 31) >> 5) + 4 * size_t(T)) +
               15) & ~size_t(15)) + 64 * pl + 64 + (corners ? 16 * 9 * 8 + 64 * size_t(S) * kVtxRecBytes : 0) +
             15) & ~size_t(15);
 }
@@ -619,6 +635,7 @@ void clip_cells_kernel(ClipArgs a) {
         return;
     }
     for (int i = lane; i < 256; i += 64) c.nxt[i] = 0xff;
+    for (int i = lane; i < P * ((P + 31) >> 5); i += 64) c.em[i] = 0;
 
     // Domain cull: a seed whose voxel is d >= 2 voxels (Chebyshev) from every
     // voxel that holds triangles or lies inside has no point of the domain
@@ -984,6 +1001,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         reinterpret_cast<const uint32_t *>(rec + sizeof(RecHeader) + size_t(a.rec_P) * sizeof(Pl<RP>));
     for (int p = lane; p < c.nb_plane; p += 64) c.planes[p] = rpl[p];
     for (int i = lane; i < 256; i += 64) c.nxt[i] = 0xff;
+    for (int i = lane; i < P * ((P + 31) >> 5); i += 64) c.em[i] = 0;
     wsync();
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -1539,14 +1557,17 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                 const int kk = jb + W * t;
                 const int tid = uni(tu[kk]);
                 if (tid > fail_t) continue;
-                const Pl<RP> q0 = cpl[4 * t], q1 = cpl[4 * t + 1], q2 = cpl[4 * t + 2], q3 = cpl[4 * t + 3];
                 double4 ck = make_double4(0.0, 0.0, 0.0, 0.0);
                 int res = kNoClip;
-                const Pl<RP> *qs[4] = {&q0, &q1, &q2, &q3};
+                // each cone plane read from LDS just before its clip: the four
+                // held at once would take 32 registers through the clips
 #pragma unroll
-                for (int f2 = 0; f2 < 4; ++f2)
-                    if (res < 0)
-                        res = c.nb_plane >= P ? int(RPD_PLANE_OVERFLOW) : c.template clip<false>(cvt<R>(*qs[f2]), -1, *qs[f2]);
+                for (int f2 = 0; f2 < 4; ++f2) {
+                    if (res < 0) {
+                        const Pl<RP> q = cpl[4 * t + f2];
+                        res = c.nb_plane >= P ? int(RPD_PLANE_OVERFLOW) : c.template clip<false>(cvt<R>(q), -1, q);
+                    }
+                }
                 if (res >= 0 && res != RPD_EMPTY_CELL) {
                     fail_t = tid;
                     fail_st = res;
