@@ -1494,16 +1494,26 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                     const float exyz = fabsf(float(e.x)) + fabsf(float(e.y)) + fabsf(float(e.z)),
                                 ew = fabsf(float(e.w));
                     gone = true;
-                    for (int j = 0; j < nvc && gone; ++j) {
-                        const VtxRec<R> v = vtx[j];
-                        const R sv = e.x * v.X + e.y * v.Y + e.z * v.Z + e.w * v.W;
-                        R m;
-                        if (FILTER)
-                            m = R(kFilterKappa * (exyz * v.MXYZ + ew * v.MW));
-                        else
-                            m = R(sizeof(R) == 8 ? 0x1.0p-33 : 0x1.0p-16) *
-                                (fabs(e.x * v.X) + fabs(e.y * v.Y) + fabs(e.z * v.Z) + fabs(e.w * v.W));
-                        gone = sv > m;
+                    // U vertices a step, their loads issued together (the
+                    // last vertex stands in past the end: testing it twice
+                    // changes nothing)
+                    constexpr int U = sizeof(R) == 4 ? 4 : 2;
+                    for (int j0 = 0; j0 < nvc && gone; j0 += U) {
+                        VtxRec<R> v[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) v[u] = vtx[min(j0 + u, nvc - 1)];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const R sv = e.x * v[u].X + e.y * v[u].Y + e.z * v[u].Z + e.w * v[u].W;
+                            R m;
+                            if (FILTER)
+                                m = R(kFilterKappa * (exyz * v[u].MXYZ + ew * v[u].MW));
+                            else
+                                m = R(sizeof(R) == 8 ? 0x1.0p-33 : 0x1.0p-16) *
+                                    (fabs(e.x * v[u].X) + fabs(e.y * v[u].Y) + fabs(e.z * v[u].Z) +
+                                     fabs(e.w * v[u].W));
+                            gone = gone && sv > m;
+                        }
                     }
                 }
             }
@@ -1527,17 +1537,26 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                         py[v] = tcor[9 * i + 3 * v + 1];
                         pz[v] = tcor[9 * i + 3 * v + 2];
                     }
-                    for (int j = f; j < c.nb_plane && !sep; j += 4) {
-                        const Pl<RP> q = c.planes[j];
-                        const double nx = double(q.x), ny = double(q.y), nz = double(q.z), nw = double(q.w);
-                        const double m = face_margin * (1.0 + fabs(nw));
-                        bool out = true;
+                    // two planes a step, loaded together; past the end the
+                    // last plane stands in (any plane of the cell may
+                    // separate, and the triangle's other lanes test it too)
+                    for (int j = f; j < c.nb_plane && !sep; j += 8) {
+                        Pl<RP> q2[2];
 #pragma unroll
-                        for (int v = 0; v < 3; ++v) {
-                            out = out && nx * px[v] + ny * py[v] + nz * pz[v] + nw < -m;
-                            out = out && nx * (px[v] - lx0) + ny * (py[v] - ly0) + nz * (pz[v] - lz0) <= 0.0;
+                        for (int u = 0; u < 2; ++u) q2[u] = c.planes[min(j + 4 * u, c.nb_plane - 1)];
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            const double nx = double(q2[u].x), ny = double(q2[u].y), nz = double(q2[u].z),
+                                         nw = double(q2[u].w);
+                            const double m = face_margin * (1.0 + fabs(nw));
+                            bool out = true;
+#pragma unroll
+                            for (int v = 0; v < 3; ++v) {
+                                out = out && nx * px[v] + ny * py[v] + nz * pz[v] + nw < -m;
+                                out = out && nx * (px[v] - lx0) + ny * (py[v] - ly0) + nz * (pz[v] - lz0) <= 0.0;
+                            }
+                            sep = sep || out;
                         }
-                        sep = out;
                     }
                 }
                 skipm |= ballot(sep);
