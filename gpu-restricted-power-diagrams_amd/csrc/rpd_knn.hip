@@ -537,19 +537,33 @@ __device__ __forceinline__ void knn_query(const KnnArgs<R> &a, int64_t qi, int l
         const int rn = r + 1;
         if (ci - rn < 0 && ci + rn >= dim && cj - rn < 0 && cj + rn >= dim && ck - rn < 0 && ck + rn >= dim) break;
     }
-    // the (at most K) kept keys in ascending order: one sort
-    KeyT top[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int e = s * 64 + lane;
-        top[s] = e < M ? KeyT::get(buf, e) : KeyT::inf();
-    }
-    bitonic_sort<S>(top, false);
     int *out = a.out + qi * int64_t(K);
+    if constexpr (S == 1 && sizeof(R) == 4) {
+        // K <= 64, one-word keys: each kept key's place is the number of
+        // kept keys below it (keys are distinct); places past M are padding.
+        // (With FP64 keys, two words, the sort below is faster.)
+        const KeyT mine = lane < M ? KeyT::get(buf, lane) : KeyT::inf();
+        int rank = 0;
+#pragma unroll 4
+        for (int j = 0; j < M; ++j) rank += KeyT::get(buf, j) < mine ? 1 : 0;
+        if (lane < M)
+            out[rank] = int(mine.s);
+        else if (lane < K)
+            out[lane] = -1;
+    } else {
+        // the (at most K) kept keys in ascending order: one sort
+        KeyT top[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const int e = s * 64 + lane;
-        if (e < K) out[e] = top[s].is_inf() ? -1 : int(top[s].s);
+        for (int s = 0; s < S; ++s) {
+            const int e = s * 64 + lane;
+            top[s] = e < M ? KeyT::get(buf, e) : KeyT::inf();
+        }
+        bitonic_sort<S>(top, false);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int e = s * 64 + lane;
+            if (e < K) out[e] = top[s].is_inf() ? -1 : int(top[s].s);
+        }
     }
 }
 
