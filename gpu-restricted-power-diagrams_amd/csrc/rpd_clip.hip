@@ -279,7 +279,13 @@ struct WaveCell {
         tri[s] = tr;
         vertex_from_planes<R, FILTER>(p1, p2, p3, X[s], Y[s], Z[s], W[s], MXYZ[s], MW[s]);
         if (R2) {
-            const R iw = R(1) / W[s];
+            // FP32: the hardware reciprocal (1 ulp); r2 is read only by the
+            // stop and skip tests, whose radius margin (2e-4) is far larger
+            R iw;
+            if constexpr (sizeof(R) == 4)
+                iw = __builtin_amdgcn_rcpf(W[s]);
+            else
+                iw = R(1) / W[s];
             const R ix = X[s] * iw, iy = Y[s] * iw, iz = Z[s] * iw;
             r2[s] = ix * ix + iy * iy + iz * iz;
         }
@@ -719,7 +725,15 @@ void clip_cells_kernel(ClipArgs a) {
     int cnb = -1;
     int loaded = -1;
     const double rmarg = sizeof(R) == 8 ? 1.0 + 1e-9 : 1.0 + 2e-4;  // FP32 R may be low by ~1e-4
-    double Rcur = sqrt(double(c.maxr2));
+    // the cell's radius (FP32 forms: the hardware square root, 1 ulp, far
+    // inside the radius margin)
+    auto radius = [&]() -> double {
+        if constexpr (sizeof(R) == 4)
+            return double(__builtin_amdgcn_sqrtf(c.maxr2));
+        else
+            return sqrt(double(c.maxr2));
+    };
+    double Rcur = radius();
     auto stop_at = [&](double dl, double Rm) -> bool {
         // CCIsSecurityRadiusReached (reference stop, weights ignored) or the
         // exact power stop d (d - 2R) >= w_s - w_min
@@ -818,7 +832,7 @@ void clip_cells_kernel(ClipArgs a) {
             return;
         }
         if (res == kClipped) {
-            Rcur = sqrt(double(c.maxr2));
+            Rcur = radius();
             if (double(c.maxr2) * 1.001 < cull_r2) {
                 finish(RPD_SUCCESS, 0.0, 0.0, 0.0, 0.0, kused);
                 return;
