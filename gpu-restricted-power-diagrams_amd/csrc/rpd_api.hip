@@ -129,7 +129,7 @@ struct rpd_ctx {
     DevBuf fslot;
     bool solved = false;
     hipEvent_t ev[4][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
-    hipEvent_t dep[2] = {};    // per chain: orders work between its streams
+    hipEvent_t dep[3] = {};    // per chain: orders work between its streams; [2]: pass 1's queue length copied
     int *pin = nullptr;        // kPinWords pinned host words
     double *pin_red = nullptr;  // 2 x kWminBlocks pinned doubles: per-block weight minima
     // smallest weight of the non-border seeds and of all seeds, for the
@@ -857,8 +857,9 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         if (r == RPD_OK) r = b.qcount.ensure(sizeof(int));
         if (r != RPD_OK) return r;
         if (hipEventRecord(c->ev[sl][0], s) != hipSuccess) return RPD_E_DEVICE;
+        // (re-queue passes run beside a finish kernel: one-wave workgroups)
         r = rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids, ps.n_ids,
-                           K_, b.knn.as<int>(), rz.in ? c->lastnb.as<int>() : nullptr, kreqs, req_of);
+                           K_, b.knn.as<int>(), rz.in ? c->lastnb.as<int>() : nullptr, kreqs, req_of, nullptr, true);
         if (r != RPD_OK) return r;
         if (hipEventRecord(c->ev[sl][1], s) != hipSuccess) return RPD_E_DEVICE;
         L = ClipLaunch{};
@@ -1001,6 +1002,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         ClipLaunch Lr, Lz;
         Resume rz;
         bool from_finish = false;  // the chain of pass 1's finish-kernel failures
+        bool back_r = false, back_z = false;  // phase 1: the finish kernel of pr / pz queued
     };
     ClipLaunch L1{};  // pass 1's launch; its finish kernel's records serve chain B's first pass
     auto cnt_pin = [&](int list) { return c->pin + kPinCounters + kFailCounters * list; };
@@ -1122,6 +1124,8 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
             r = pass_front(ch.pz, c->fl[ch.cur].eo.as<int>() + n - n_exact, ch.nxt, nullptr, ch.rz, ch.Lz);
             if (r != RPD_OK) return r;
         }
+        ch.back_r = ch.n_res == 0;
+        ch.back_z = ch.n_scr == 0;
         ch.phase = 1;
         return RPD_OK;
     };
@@ -1131,13 +1135,32 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     auto chain_poll = [&](Chain &ch, bool &progress) -> int {
         hipStream_t s = c->streams[ch.sl], s2 = c->streams[ch.sl2];
         if (ch.phase == 1) {
-            const int d1 = done(s), d2 = ch.n_scr > 0 ? done(s2) : 1;
-            if (d1 < 0 || d2 < 0) return RPD_E_DEVICE;
-            if (!d1 || !d2) return RPD_OK;
-            // pass 1's finish kernel may still run beside these ones
+            // each finish kernel as soon as its own bisector kernel is done;
+            // pass 1's finish kernel may still run beside them
             const int beside = p1_running ? p1.nq : 0;
-            if (ch.n_res > 0 && pass_back(ch.pr, ch.Lr, beside) != RPD_OK) return RPD_E_DEVICE;
-            if (ch.n_scr > 0 && pass_back(ch.pz, ch.Lz, beside) != RPD_OK) return RPD_E_DEVICE;
+            bool s_done = false;  // s found idle by this poll, before any launch on it
+            if (!ch.back_r) {
+                const int d = done(s);
+                if (d < 0) return RPD_E_DEVICE;
+                if (d) {
+                    s_done = true;
+                    if (pass_back(ch.pr, ch.Lr, beside) != RPD_OK) return RPD_E_DEVICE;
+                    ch.back_r = true;
+                    progress = true;
+                }
+            }
+            if (!ch.back_z) {
+                // one stream for both (two-stream contexts): pz's bisector
+                // kernel was queued after pr's, so s idle covers it
+                const int d = s2 == s && s_done ? 1 : done(s2);
+                if (d < 0) return RPD_E_DEVICE;
+                if (d) {
+                    if (pass_back(ch.pz, ch.Lz, beside) != RPD_OK) return RPD_E_DEVICE;
+                    ch.back_z = true;
+                    progress = true;
+                }
+            }
+            if (!ch.back_r || !ch.back_z) return RPD_OK;
             // the counters of the pass, once both finish kernels are done
             if (ch.n_scr > 0 && (hipEventRecord(c->dep[ch.dep], s2) != hipSuccess ||
                                  hipStreamWaitEvent(s, c->dep[ch.dep], 0) != hipSuccess))
@@ -1172,8 +1195,8 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     // finish kernel's to list 1.  Two chains of re-queue passes follow, at
     // once:
     //   * chain A, the bisector failures, known as soon as that kernel is
-    //     done: its first pass is queued (streams 1, 2) before pass 1's
-    //     finish kernel, so that the two share the device from the start;
+    //     done: its first pass is queued (streams 1, 2) while pass 1's finish
+    //     kernel, queued behind the bisector kernel, runs;
     //   * chain B, the finish kernel's failures, once it is done (streams 0,
     //     3), with the caps of pass 1 grown only for them.
     if (!clear_list(0, c->stream) || !clear_list(1, c->stream)) return set_err(c, RPD_E_DEVICE, "clear lists");
@@ -1258,13 +1281,28 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         L.dcnt = dcnt_of(0);
         L.reqs = reqs;
         L.req_of = req_of;
-        RPDCHK(c, rpd_launch_cells(s, form0, L, c->pin + 0, true), "pass 1 cells");
-        p1.nq = c->pin[0];
+        RPDCHK(c, rpd_launch_cells(s, form0, L, c->pin + 0, false), "pass 1 cells");
+        HIPCHK(c, hipEventRecord(c->dep[2], s));  // queue length and failure counters copied
         L.fl_cnt = c->fl[1].cnt.as<int>();
         L.fl_eo = c->fl[1].eo.as<int>() + n;
         L.fl_sec = c->fl[1].sec.as<int>();
         L.fl_cnt_host = nullptr;
         L.fl_slot = c->fslot.as<int>() + n;  // the finish kernel's failures keep their record slots
+        // The finish kernel is queued at once, over the n1 cells the queue
+        // can hold at most, reading its length on the device: no host round
+        // trip between the two kernels.  Chain A's first pass follows once
+        // the host has the bisector kernel's failure counters; its kNN uses
+        // one-wave workgroups, which fit beside the finish kernel's.
+        L.nq_dev = L.qcount;
+        RPDCHK(c, rpd_launch_finish(s, form0, L, int(n1), 0), "pass 1 finish");
+        HIPCHK(c, hipEventRecord(c->ev[0][3], s));
+        HIPCHK(c, hipMemcpyAsync(cnt_pin(1), c->fl[1].cnt.ptr, kFailCounters * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipEventSynchronize(c->dep[2]));
+        p1.nq = c->pin[0];
+        // no cell to clip (every one skipped): the bisector kernel did not
+        // run and copied no failure counters
+        if (n1 <= 0) std::fill(cnt_pin(0), cnt_pin(0) + kFailCounters, 0);
+        L.nq_dev = nullptr;
     }
     const Caps caps1{K, P, V, T};
     Chain ca, cb;
@@ -1284,12 +1322,6 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     cb.phase = 4;
     cb.from_finish = true;
     RPDCHK(c, chain_front(ca), "re-queue chain (bisector failures)");
-    {
-        hipStream_t s = c->streams[0];
-        RPDCHK(c, rpd_launch_finish(s, form0, L1, p1.nq), "pass 1 finish");
-        HIPCHK(c, hipEventRecord(c->ev[0][3], s));
-        HIPCHK(c, hipMemcpyAsync(cnt_pin(1), c->fl[1].cnt.ptr, kFailCounters * sizeof(int), hipMemcpyDeviceToHost, s));
-    }
     while (ca.phase != 3 || cb.phase != 3) {
         bool progress = false;
         if (cb.phase == 4) {

@@ -126,6 +126,9 @@ struct ClipArgs {
     // slots they were written with (the finish kernel's own P and S may be
     // larger)
     int rec_P, rec_S;
+    // finish kernel (nullable): the queue length in device memory, read when
+    // the launch covers an upper bound of it
+    const int *nq_dev;
 };
 
 // The cell of stored id sid reads its request's seed count, weight minimum
@@ -986,7 +989,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
     const int cpb = wpb / W;  // cells per block; W > 1 implies cpb == 1
     const int ci = wv / W, sub = wv - ci * W;
     const int item = int(blockIdx.x) * cpb + ci;
-    if (item >= nq) return;  // uniform over the waves of a cell
+    if (item >= nq || (a.nq_dev && item >= uni(*a.nq_dev))) return;  // uniform over the waves of a cell
     const int slot = a.slots ? uni(a.slots[item]) : item;
 
     char *cell_l = lds_raw + size_t(ci) * a.cell_stride;
@@ -1810,6 +1813,7 @@ static int make_args(int form, const ClipLaunch &L, ClipArgs &a) {
     a.reqs = L.reqs;
     a.req_of = L.req_of;
     a.slots = L.slots;
+    a.nq_dev = L.nq_dev;
     a.fl_slot = L.fl_slot;
     return RPD_OK;
 }

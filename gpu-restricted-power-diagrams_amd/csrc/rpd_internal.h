@@ -91,10 +91,13 @@ struct KnnReq {
 // kreqs set, each query searches the grid of its request (req_of) instead of
 // (counters, dim).
 // n_dev (nullable): the number of queries in device memory, at most n_ids
-// (the launch covers n_ids; waves past the count exit).
+// (the launch covers n_ids; waves past the count exit).  narrow: one-wave
+// workgroups, for launches beside a finish kernel (whose workgroups leave too
+// little free LDS on a CU for wider ones).
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
                    const int *ids, int64_t n_ids, int K, int *out, const int *lower = nullptr,
-                   const KnnReq *kreqs = nullptr, const uint16_t *req_of = nullptr, const int *n_dev = nullptr);
+                   const KnnReq *kreqs = nullptr, const uint16_t *req_of = nullptr, const int *n_dev = nullptr,
+                   bool narrow = false);
 // adds base to n ints (binning of one request of a packed batch)
 int rpd_add_base(hipStream_t st, int *v, int64_t n, int base);
 
@@ -185,6 +188,9 @@ struct ClipLaunch {
     int *fl_slot;
     // finish kernel: the P and V the records were written with (0: P, V)
     int rec_P, rec_V;
+    // finish kernel (nullable): the queue length in device memory, at most
+    // the nq it is launched with (items past it exit)
+    const int *nq_dev;
 };
 
 constexpr int kFailCounters = 16;

@@ -567,17 +567,20 @@ __device__ __forceinline__ void knn_query(const KnnArgs<R> &a, int64_t qi, int l
     }
 }
 
-// One wave per query.  With n_dev the launch covers an upper bound of the
-// queries (n_ids) and the waves past the count exit at once.  (A grid-stride
-// loop over fewer workgroups was measured slower: it holds more registers,
-// 67 against 56 VGPRs for FP32 keys.)
-template <typename R, int S>
-__global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> a) {
+// One wave per query, WPB waves per workgroup.  With n_dev the launch covers
+// an upper bound of the queries (n_ids) and the waves past the count exit at
+// once.  (A grid-stride loop over fewer workgroups was measured slower: it
+// holds more registers, 67 against 56 VGPRs for FP32 keys.)  Launches that
+// run beside a finish kernel use one-wave workgroups: a CU that the finish
+// kernel's workgroups fill keeps less free LDS than a four-wave workgroup
+// needs, so such workgroups waited until the finish kernel drained.
+template <typename R, int S, int WPB>
+__global__ __launch_bounds__(64 * WPB) void knn_kernel(KnnArgs<R> a) {
     using KeyT = Key<R>;
     constexpr int KEEP = kKeep<S>;
-    __shared__ int cand_all[kKnnWavesPerBlock][kCandBuf];
-    __shared__ __attribute__((aligned(16))) uint32_t hist_all[kKnnWavesPerBlock][256];
-    __shared__ __attribute__((aligned(16))) char keep_all[kKnnWavesPerBlock][KEEP * KeyT::kBufBytes];
+    __shared__ int cand_all[WPB][kCandBuf];
+    __shared__ __attribute__((aligned(16))) uint32_t hist_all[WPB][256];
+    __shared__ __attribute__((aligned(16))) char keep_all[WPB][KEEP * KeyT::kBufBytes];
     const int lane = lane_id();
     const int wv = uni(int(threadIdx.x >> 6));  // wave-uniform: lets ids, seeds and pointers live in SGPRs
     const int64_t nq = a.n_dev ? min(a.n_ids, int64_t(uni(*a.n_dev))) : a.n_ids;
@@ -593,7 +596,7 @@ __global__ __launch_bounds__(64 * kKnnWavesPerBlock) void knn_kernel(KnnArgs<R> 
             buf.s = reinterpret_cast<uint32_t *>(kb + 12 * KEEP);
         }
     }
-    const int64_t qi = int64_t(blockIdx.x) * kKnnWavesPerBlock + wv;
+    const int64_t qi = int64_t(blockIdx.x) * WPB + wv;
     if (qi < nq) knn_query<R, S>(a, qi, lane, cand_all[wv], hist_all[wv], buf);
 }
 
@@ -723,26 +726,35 @@ int rpd_add_base(hipStream_t st, int *v, int64_t n, int base) {
 
 int rpd_launch_knn(hipStream_t st, int prec, const void *stored, const int *perm, const int *counters, int dim,
                    const int *ids, int64_t n_ids, int K, int *out, const int *lower, const KnnReq *kreqs,
-                   const uint16_t *req_of, const int *n_dev) {
+                   const uint16_t *req_of, const int *n_dev, bool narrow) {
     if (n_ids <= 0) return RPD_OK;
     if (K <= 0 || K > 256) return RPD_E_LIMIT;
-    const unsigned blocks = unsigned((n_ids + kKnnWavesPerBlock - 1) / kKnnWavesPerBlock);
-    const dim3 tb(64 * kKnnWavesPerBlock);
+    const int wpb = narrow ? 1 : kKnnWavesPerBlock;
+    const unsigned blocks = unsigned((n_ids + wpb - 1) / wpb);
+    const dim3 tb(64 * wpb);
+#define RPD_KNN_LAUNCH_W(R, S)                                 \
+    do {                                                       \
+        if (narrow)                                            \
+            knn_kernel<R, S, 1><<<blocks, tb, 0, st>>>(a);     \
+        else                                                   \
+            knn_kernel<R, S, kKnnWavesPerBlock><<<blocks, tb, 0, st>>>(a); \
+    } while (0)
 #define RPD_KNN_LAUNCH(R, V4)                                                                       \
     do {                                                                                            \
         KnnArgs<R> a{static_cast<const V4 *>(stored), perm, counters, ids, n_ids, dim, K, out, lower, \
                      kreqs, req_of, n_dev};                                                         \
         if (K <= 64)                                                                                \
-            knn_kernel<R, 1><<<blocks, tb, 0, st>>>(a);                                             \
+            RPD_KNN_LAUNCH_W(R, 1);                                                                 \
         else if (K <= 128)                                                                          \
-            knn_kernel<R, 2><<<blocks, tb, 0, st>>>(a);                                             \
+            RPD_KNN_LAUNCH_W(R, 2);                                                                 \
         else                                                                                        \
-            knn_kernel<R, 4><<<blocks, tb, 0, st>>>(a);                                             \
+            RPD_KNN_LAUNCH_W(R, 4);                                                                 \
     } while (0)
     if (prec == 0)
         RPD_KNN_LAUNCH(float, float4);
     else
         RPD_KNN_LAUNCH(double, double4);
 #undef RPD_KNN_LAUNCH
+#undef RPD_KNN_LAUNCH_W
     return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }
