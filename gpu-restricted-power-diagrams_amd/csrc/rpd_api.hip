@@ -522,10 +522,11 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
                 tvox[size_t(cur[size_t(g->tri_list[q])]++)] =
                     (uint32_t(v / (gg * gg)) << 20) | (uint32_t((v / gg) % gg) << 10) | uint32_t(v % gg);
     }
-    // per list entry of voxel v: v, and which neighbours of v that precede
-    // it in scan order also list the triangle (bit (dz+1)*9 + (dy+1)*3 +
-    // (dx+1)); bit 27 when a preceding voxel listing it is farther away
-    std::vector<uint32_t> tinfo(2 * size_t(std::max(g->num_tri_list, 0)));
+    // per list entry of voxel v: v, and which voxels within 2 of v per axis
+    // that precede it in scan order also list the triangle (64-bit mask, bit
+    // (dz+2)*25 + (dy+2)*5 + (dx+2), below 62 for a preceding voxel); bit 63
+    // when a preceding voxel listing it is farther away
+    std::vector<uint32_t> tinfo(4 * size_t(std::max(g->num_tri_list, 0)));
     if (r == RPD_OK) {
         const size_t gg = size_t(g->size);
         for (size_t v = 0; v < nv; ++v) {
@@ -533,18 +534,20 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
             const uint32_t pv = (uint32_t(vz) << 20) | (uint32_t(vy) << 10) | uint32_t(vx);
             for (int q = g->offsets[v]; q < g->offsets[v + 1]; ++q) {
                 const int t = g->tri_list[q];
-                uint32_t m = 0;
+                uint64_t m = 0;
                 for (int u = tvoffs[size_t(t)]; u < tvoffs[size_t(t) + 1]; ++u) {
                     const uint32_t w = tvox[size_t(u)];
                     if (w >= pv) break;
                     const int dx = int(w & 1023u) - vx, dy = int((w >> 10) & 1023u) - vy, dz = int(w >> 20) - vz;
-                    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1)
-                        m |= 1u << 27;
+                    if (dx < -2 || dx > 2 || dy < -2 || dy > 2 || dz < -2 || dz > 2)
+                        m |= 1ull << 63;
                     else
-                        m |= 1u << ((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+                        m |= 1ull << ((dz + 2) * 25 + (dy + 2) * 5 + (dx + 2));
                 }
-                tinfo[2 * size_t(q)] = pv;
-                tinfo[2 * size_t(q) + 1] = m;
+                tinfo[4 * size_t(q)] = pv;
+                tinfo[4 * size_t(q) + 1] = uint32_t(m);
+                tinfo[4 * size_t(q) + 2] = uint32_t(m >> 32);
+                tinfo[4 * size_t(q) + 3] = 0;
             }
         }
     }
@@ -573,7 +576,7 @@ int rpd_domain_create(rpd_ctx *c, const rpd_grid_desc *g, rpd_domain **out) {
     d->dev.toffs = d->toffs.as<int>();
     d->dev.tvoffs = d->tvoffs.as<int>();
     d->dev.tvox = d->tvox.as<uint32_t>();
-    d->dev.tinfo = d->tinfo.as<uint2>();
+    d->dev.tinfo = d->tinfo.as<uint4>();
     d->dev.dist = d->dist.as<uint8_t>();
     d->dev.dvox = d->dvox.as<int>();
     d->dev.n_dvox = int(dvox.size());

@@ -79,7 +79,7 @@ struct ClipArgs {
     int cell_stride;  // LDS bytes shared by the waves of one cell (finish kernel)
     const int *tvoffs;    // per triangle: offsets into tvox
     const uint32_t *tvox; // per triangle: sorted voxels whose list holds it, z<<20 | y<<10 | x
-    const uint2 *tinfo;   // per triangle-list entry: its voxel, earlier neighbours holding it
+    const uint4 *tinfo;   // per triangle-list entry: its voxel, earlier near voxels holding it
     int security_mode;    // 0 exact power stop, 1 reference stop
     double wmin;          // minimum weight over non-border seeds
     const uint8_t *dist;  // Chebyshev voxel distance to the nearest domain voxel (nullable)
@@ -938,19 +938,32 @@ void clip_cells_kernel(ClipArgs a) {
     }
 }
 
-// tinfo[q].y of a triangle-list entry (voxel v): bit (dz+1)*9 + (dy+1)*3 +
-// (dx+1) set when the voxel v + (dx, dy, dz) precedes v in scan order and
-// its list holds the same triangle; kTinfoFar when such a voxel is not a
-// neighbour of v.  tinfo[q].x is v packed as z<<20 | y<<10 | x.
-constexpr uint32_t kTinfoFar = 1u << 27;
+// tinfo[q] of a triangle-list entry (voxel v): x is v packed as
+// z<<20 | y<<10 | x; (z << 32 | y) a 64-bit mask with bit
+// (dz+2)*25 + (dy+2)*5 + (dx+2) set when the voxel v + (dx, dy, dz), within 2
+// voxels of v per axis, precedes v in scan order and its list holds the same
+// triangle (such offsets are the ones below the centre, bit 62, of the
+// 5x5x5 block, so the bits are 0..61); kTinfoFar when a preceding voxel
+// whose list holds it lies farther away.
+constexpr uint64_t kTinfoFar = 1ull << 63;
 
-// Mask of the offsets (dx, dy, dz) in {-1,0,1}^3 that keep a voxel inside the
-// box, given whether it lies above the box minimum / below its maximum per axis.
-__device__ __forceinline__ uint32_t box_neighbours(bool xl, bool xh, bool yl, bool yh, bool zl, bool zh) {
-    const uint32_t ax = (xl ? 1u : 0u) | 2u | (xh ? 4u : 0u);
-    const uint32_t y9 = (yl ? 0x7u : 0u) | 0x38u | (yh ? 0x1C0u : 0u);
-    const uint32_t z27 = (zl ? 0x1FFu : 0u) | 0x3FE00u | (zh ? 0x7FC0000u : 0u);
-    return (ax * 0x1249249u) & (y9 * 0x40201u) & z27;
+// Mask of the preceding offsets of the 5x5x5 block (bits as in tinfo) that
+// keep voxel (x, y, z) inside the box [mn, mx].
+__device__ __forceinline__ uint64_t box_near(int x, int y, int z, const int *mn, const int *mx) {
+    // per axis: the offsets d in [-2, 2] with mn <= v + d <= mx, bit d + 2
+    auto axis = [](int v, int lo, int hi) -> uint32_t {
+        const int a = max(lo - v, -2), b = min(hi - v, 2);
+        return a > b ? 0u : ((1u << (b - a + 1)) - 1u) << (a + 2);
+    };
+    const uint32_t ax = axis(x, mn[0], mx[0]), ay = axis(y, mn[1], mx[1]), az = axis(z, mn[2], mx[2]);
+    // the (dy, dx) plane: bit 5 (dy + 2) + dx + 2, i.e. ax repeated at 5 bit k
+    // of ay (no carries: the copies are 5 bits apart)
+    const uint32_t ry = (ay & 1u) | ((ay & 2u) << 4) | ((ay & 4u) << 8) | ((ay & 8u) << 12) | ((ay & 16u) << 16);
+    const uint64_t pyx = uint64_t(ax) * uint64_t(ry);
+    // planes dz = -2, -1, 0 at bits 0, 25, 50 (dz > 0 never precedes); bits
+    // past 63 of the dz = 0 plane are offsets after the centre, never set
+    const uint64_t rz = ((az & 1u) ? 1ull : 0ull) | ((az & 2u) ? 1ull << 25 : 0ull) | ((az & 4u) ? 1ull << 50 : 0ull);
+    return pyx * rz;
 }
 
 // ---------------------------------------------------------------------------
@@ -1250,10 +1263,10 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         // list entries of one row of the box are one range of tlist; the
         // rows' ranges are spread over the lanes.  An entry is kept in the
         // first voxel of the box, in scan order, whose list holds its
-        // triangle: tinfo holds the entry's voxel and a mask of the
-        // neighbours of that voxel that precede it and hold the triangle
-        // too (or kTinfoFar when one of those is not a neighbour: then the
-        // triangle's own voxel list is read).  Entries are kept in scan order.
+        // triangle: tinfo holds the entry's voxel and a mask of the voxels
+        // within 2 of it per axis that precede it and hold the triangle too
+        // (or kTinfoFar when one farther away does: then the triangle's own
+        // voxel list is read).  Entries are kept in scan order.
         int nu = 0;
         const uint32_t pmn = (uint32_t(vmn[2]) << 20) | (uint32_t(vmn[1]) << 10) | uint32_t(vmn[0]);
         const uint32_t pmx = (uint32_t(vmx[2]) << 20) | (uint32_t(vmx[1]) << 10) | uint32_t(vmx[0]);
@@ -1289,9 +1302,10 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                     }
                     const int q = seg_a[lo] + (e - seg_off[lo]);
                     t = a.tlist[q];
-                    const uint2 inf = a.tinfo[q];
+                    const uint4 inf = a.tinfo[q];
                     const uint32_t pv = inf.x;
-                    if (inf.y & kTinfoFar) {
+                    const uint64_t near = (uint64_t(inf.z) << 32) | inf.y;
+                    if (near & kTinfoFar) {
                         keep = true;
                         for (int u = a.tvoffs[t], ue = a.tvoffs[t + 1]; u < ue; ++u) {
                             const uint32_t w = a.tvox[u];
@@ -1305,8 +1319,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
                         }
                     } else {
                         const int x = int(pv & 1023u), y = int((pv >> 10) & 1023u), z = int(pv >> 20);
-                        keep = (inf.y & box_neighbours(x > vmn[0], x < vmx[0], y > vmn[1], y < vmx[1], z > vmn[2],
-                                                       z < vmx[2])) == 0;
+                        keep = (near & box_near(x, y, z, vmn, vmx)) == 0;
                     }
                 }
                 const uint64_t m = ballot(keep);

@@ -113,7 +113,7 @@ struct DomainDev {
     const int *toffs = nullptr;
     const int *tvoffs = nullptr;     // per triangle: offsets into tvox
     const uint32_t *tvox = nullptr;  // per triangle: sorted voxels (z<<20 | y<<10 | x) whose list holds it
-    const uint2 *tinfo = nullptr;    // per triangle-list entry: its voxel, earlier neighbours holding it
+    const uint4 *tinfo = nullptr;    // per triangle-list entry: its voxel, earlier near voxels holding it
     const uint8_t *dist = nullptr;   // Chebyshev voxel distance to domain voxels
     const int *dvox = nullptr;       // the domain voxels (distance 0), ascending
     int n_dvox = 0;
@@ -135,7 +135,7 @@ struct ReqDesc {
     const int *toffs;
     const int *tvoffs;
     const uint32_t *tvox;
-    const uint2 *tinfo;
+    const uint4 *tinfo;
     const uint8_t *dist;  // nullptr: no cull
 };
 
