@@ -405,7 +405,7 @@ def main():
     roofline_knn = {"bound": "hbm", "achieved": knn_bytes / t_knn / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": knn_bytes / t_knn / 1e9 / PEAK_HBM_GBS, "queries": n_query,
                     "bytes": "(16 + 4K) per query + 16 per seed", "launch_ms": t_knn * 1e3,
-                    "kernel": "knn_kernel<float, 1>", "traffic": measured_traffic("knn_kernel<float, 1>")}
+                    "kernel": "knn_kernel<float, 1, 4>", "traffic": measured_traffic("knn_kernel<float, 1, 4>")}
 
     serve = None
     if not args.no_serving and args.serve_requests > 0:

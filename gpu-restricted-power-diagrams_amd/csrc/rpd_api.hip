@@ -65,11 +65,13 @@ struct FailBufs {
 };
 
 // pinned host words of a context: queue lengths (one per stream), then the
-// counters of each failure list
+// counters of each failure list, then the final status histogram
 constexpr int kPinCounters = 16;
 // smallest triangle cap a solve uses
 constexpr int kTriangleFloor = 256;
-constexpr int kPinWords = kPinCounters + 6 * kFailCounters;
+constexpr int kPinHist = kPinCounters + 6 * kFailCounters;
+constexpr int kPinWords = kPinHist + 16;
+static_assert(RPD_STATUS_NUM <= 16, "status histogram room");
 // blocks of the weight-minimum kernel
 constexpr int kWminBlocks = 256;
 
@@ -743,6 +745,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     RPDCHK(c, c->lastnb.ensure(size_t(n) * 4), "alloc resume slots");
     RPDCHK(c, c->fslot.ensure(size_t(n) * 8), "alloc failure slots");
     RPDCHK(c, c->cnt.ensure(64), "alloc cnt");
+    HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));  // the final status histogram's counts
     c->hess_H = hess ? prm->H : 0;
     c->hess_HC = hess ? prm->HC : 1;
     if (hess) {
@@ -1350,11 +1353,13 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     V = std::max(cap_a.V, cap_b.V);
     T = std::max(cap_a.T, cap_b.T);
     // final status histogram
-    HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));
     status_hist_kernel<<<unsigned(std::min<int64_t>(1024, (n + 255) / 256)), 256, 0, c->stream>>>(
         c->status.as<int>(), n, c->cnt.as<int>());
-    HIPCHK(c, hipMemcpyAsync(st.status_hist, c->cnt.ptr, sizeof(st.status_hist), hipMemcpyDeviceToHost, c->stream));
+    // through pinned memory: a copy into pageable memory is staged and
+    // starts late
+    HIPCHK(c, hipMemcpyAsync(c->pin + kPinHist, c->cnt.ptr, sizeof(st.status_hist), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(st.status_hist, c->pin + kPinHist, sizeof(st.status_hist));
     st.ms_knn = ms_knn + ms_fb_knn;
     st.ms_clip = st.ms_clip_first;
     st.ms_fallback = ms_fb;
