@@ -155,6 +155,18 @@ static void *g_scratch_ptr[kMaxDevices] = {};
 
 namespace {
 
+// Up to 12 device int arrays cleared by one single-workgroup launch: the
+// counters a solve starts from, without one fill launch each.
+struct ZeroSet {
+    int *p[12];
+    int n[12];
+    int k;
+};
+__global__ void zero_words_kernel(ZeroSet z) {
+    for (int j = 0; j < z.k; ++j)
+        for (int i = int(threadIdx.x); i < z.n[j]; i += int(blockDim.x)) z.p[j][i] = 0;
+}
+
 // Resolves the current device's copy of `scratch` once (device current).
 int resolve_scratch(int device) {
     if (device < 0 || device >= kMaxDevices) return RPD_E_DEVICE;
@@ -745,7 +757,31 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     RPDCHK(c, c->lastnb.ensure(size_t(n) * 4), "alloc resume slots");
     RPDCHK(c, c->fslot.ensure(size_t(n) * 8), "alloc failure slots");
     RPDCHK(c, c->cnt.ensure(64), "alloc cnt");
-    HIPCHK(c, hipMemsetAsync(c->cnt.ptr, 0, 64, c->stream));  // the final status histogram's counts
+    for (auto &b : c->pb) RPDCHK(c, b.qcount.ensure(sizeof(int)), "alloc qcount");
+    // The counters the solve starts from, cleared by one launch: the final
+    // status histogram's, every failure list's, the resume buffers' and the
+    // queue lengths.  A list, resume buffer or queue counter stays "fresh"
+    // (known zero) until a pass writes it; clearing a fresh one is skipped.
+    bool fresh_fl[6], fresh_rs[4], fresh_q[4];
+    {
+        ZeroSet z{};
+        z.p[z.k] = c->cnt.as<int>();
+        z.n[z.k++] = 16;
+        for (int i = 0; i < 6; ++i) {
+            z.p[z.k] = c->fl[i].cnt.as<int>();
+            z.n[z.k++] = kFailCounters;
+            fresh_fl[i] = true;
+        }
+        z.p[z.k] = c->rs_cnt.as<int>();
+        z.n[z.k++] = 4;
+        for (int i = 0; i < 4; ++i) {
+            z.p[z.k] = c->pb[i].qcount.as<int>();
+            z.n[z.k++] = 1;
+            fresh_rs[i] = fresh_q[i] = true;
+        }
+        zero_words_kernel<<<1, 64, 0, c->stream>>>(z);
+        HIPCHK(c, hipGetLastError());
+    }
     c->hess_H = hess ? prm->H : 0;
     c->hess_HC = hess ? prm->HC : 1;
     if (hess) {
@@ -912,6 +948,8 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         L.dcnt = dcnt_of(ps.index);
         L.reqs = reqs;
         L.req_of = req_of;
+        L.qcount_zeroed = fresh_q[sl];
+        fresh_q[sl] = false;
         return rpd_launch_cells(s, ps.form, L, c->pin + sl, false);
     };
     // the finish kernel of a pass whose bisector kernel (and queue length) is done
@@ -967,13 +1005,22 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         nc.T = std::min(nc.T, rpd_clip_max_T(RPD_PREC_FP64, nc.P, nc.V, hess));
         return nc;
     };
+    // (the list is written by the pass it is cleared for)
     auto clear_list = [&](int fl, hipStream_t s) {
+        if (fresh_fl[fl]) {
+            fresh_fl[fl] = false;
+            return true;
+        }
         return hipMemsetAsync(c->fl[fl].cnt.ptr, 0, kFailCounters * sizeof(int), s) == hipSuccess;
     };
     // Record buffer `which` for cap records of stride bytes, counter cleared.
     auto resume_buf = [&](int which, int64_t cap, int stride, hipStream_t s) -> int {
         const int r = c->rs[which].ensure(size_t(std::max<int64_t>(cap, 1)) * size_t(stride));
         if (r != RPD_OK) return r;
+        if (fresh_rs[which]) {
+            fresh_rs[which] = false;
+            return RPD_OK;
+        }
         return hipMemsetAsync(c->rs_cnt.as<int>() + which, 0, sizeof(int), s) == hipSuccess ? RPD_OK
                                                                                               : RPD_E_DEVICE;
     };
@@ -1287,7 +1334,10 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         L.dcnt = dcnt_of(0);
         L.reqs = reqs;
         L.req_of = req_of;
+        L.qcount_zeroed = fresh_q[0];
+        fresh_q[0] = false;
         RPDCHK(c, rpd_launch_cells(s, form0, L, c->pin + 0, false), "pass 1 cells");
+        L.qcount_zeroed = false;
         HIPCHK(c, hipEventRecord(c->dep[2], s));  // queue length and failure counters copied
         L.fl_cnt = c->fl[1].cnt.as<int>();
         L.fl_eo = c->fl[1].eo.as<int>() + n;

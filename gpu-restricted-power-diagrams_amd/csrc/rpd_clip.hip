@@ -1838,7 +1838,7 @@ int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out,
     const int r = make_args(form, L, a);
     if (r != RPD_OK) return r;
     const int S = L.V <= 64 ? 1 : (L.V <= 128 ? 2 : 4);
-    if (hipMemsetAsync(L.qcount, 0, sizeof(int), st) != hipSuccess) return RPD_E_DEVICE;
+    if (!L.qcount_zeroed && hipMemsetAsync(L.qcount, 0, sizeof(int), st) != hipSuccess) return RPD_E_DEVICE;
 
     // kernel A: the bisector loop of every cell
     {

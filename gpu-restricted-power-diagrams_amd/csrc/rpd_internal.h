@@ -157,6 +157,7 @@ struct ClipLaunch {
     int *kused;            // original order (nullable)
     void *rec;             // n_ids * rpd_clip_record_bytes(form, P, V) bytes
     int *qcount;           // one device int: queued cells
+    bool qcount_zeroed;    // the caller has cleared qcount (rpd_launch_cells does not)
     hipEvent_t ev_mid;     // recorded after the bisector kernel (nullable)
     // failure list the kernels append to (nullable): kFailCounters counters
     // ([0] exact entries, [1] overflow entries, [2 + s] cells per status s,
