@@ -105,6 +105,10 @@ struct rpd_ctx {
     bool dcnt_ok = false;
     int64_t ccount = 0;  // cells in cout / cperm after the last compaction
     int last_K = 0, last_P = 0, last_V = 0, last_sm = 0, last_steps = 0;
+    // cells the cover test kept in the last solve over kept_hint_n seeds
+    // (-1: none): the size of the next pass-1 kNN launch, which is queued
+    // before the count is known
+    int64_t kept_hint = -1, kept_hint_n = -1;
     // packed batch set by rpd_solve_batch (empty: the seeds are one
     // diagram): per request its domain, stored-id range, weight minima, kNN
     // grid size and counters offset; on the device the per-request fields
@@ -1258,12 +1262,19 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         hipStream_t s = c->streams[0];
         PassBufs &b = c->pb[0];
         // pass 1's kNN: over the kept cells, their count read on the device
-        // while the host does not know it yet (n rows at most)
+        // while the host does not know it yet.  The launch covers the last
+        // solve's count over as many seeds, with a margin (n rows at most, or
+        // all n without such a solve); rows past the count exit at once, and
+        // rows past the launch, if the count turns out larger, get a second
+        // launch once the host has it.
         RPDCHK(c, b.knn.ensure(size_t(std::max<int64_t>(n1, 1)) * K * 4), "alloc knn");
+        int64_t rows = n1;
+        if (cover_count && c->kept_hint >= 0 && c->kept_hint_n == n)
+            rows = std::min<int64_t>(n1, c->kept_hint + c->kept_hint / 16 + 4096);
         HIPCHK(c, hipEventRecord(c->ev[0][0], s));
         RPDCHK(c,
-               rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids1, n1, K,
-                              b.knn.as<int>(), nullptr, kreqs, req_of, cover_count),
+               rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim, ids1, rows,
+                              K, b.knn.as<int>(), nullptr, kreqs, req_of, cover_count),
                "pass 1 knn");
         HIPCHK(c, hipEventRecord(c->ev[0][1], s));
         if (cover_count) {
@@ -1271,6 +1282,16 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
             st.ms_cover = elapsed_ms(c->ev[1][0], c->ev[1][1]);
             n1 = *cover_pin;
             st.skipped_cells = n - n1;
+            c->kept_hint = n1;
+            c->kept_hint_n = n;
+            if (n1 > rows) {
+                RPDCHK(c,
+                       rpd_launch_knn(s, kprec, c->stored.ptr, c->perm.as<int>(), c->counters.as<int>(), c->dim,
+                                      ids1 + rows, n1 - rows, K, b.knn.as<int>() + size_t(rows) * size_t(K), nullptr,
+                                      kreqs, req_of),
+                       "pass 1 knn (rest)");
+                HIPCHK(c, hipEventRecord(c->ev[0][1], s));
+            }
         }
         p1.slot = 0;
         p1.form = form0;

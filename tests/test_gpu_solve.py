@@ -134,6 +134,41 @@ def test_domain_cover_skip(rpd, ctx, prec):
     assert (np.abs(full[(got == 0).all(1), 3]) <= 1e-12 * 1e9 / n).all()
 
 
+def test_knn_launch_sized_from_last_solve(rpd, ctx):
+    """Pass 1's kNN is launched over the last solve's kept count (same seed
+    count) plus a margin, before the new count is known: a larger count gets
+    a second launch for the rest, a smaller one leaves empty waves.  Both
+    orders give the bits of a fresh context, which launches over every
+    seed."""
+    n = 120000
+    h = (1.0 / n) ** (1 / 3)
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=17, wmax=0.25 * h * h * 1e6)
+    small = ctx.domain(rpd.Grid.shape(rpd.SHAPE_SPHERE, 48, (0.5, 0.5, 0.5, 0.12)))
+    large = ctx.domain(rpd.Grid.shape(rpd.SHAPE_TORUS, 48, (0.5, 0.5, 0.5, 0.3, 0.12)))
+    caps = dict(K=50, P=32, V=48, T=100, precision=rpd.PREC_MIXED)
+    fresh = {}
+    for name, d in (("small", small), ("large", large)):
+        c2 = rpd.Context(0)
+        d2 = c2.domain(d.grid)
+        c2.set_seeds(x)
+        st = c2.solve(d2, **caps)
+        fresh[name] = (c2.moments(), st["skipped_cells"])
+        d2.close()
+        c2.close()
+    kept = {k: n - v[1] for k, v in fresh.items()}
+    # the large domain keeps more than the small one's count plus the margin
+    assert kept["large"] > kept["small"] + kept["small"] // 16 + 4096
+    ctx.set_seeds(x)
+    for order in (("small", "large"), ("large", "small")):
+        for name in order:
+            d = small if name == "small" else large
+            st = ctx.solve(d, **caps)
+            print("kNN launch size", order, name, n - st["skipped_cells"], flush=True)
+            assert np.array_equal(ctx.moments(), fresh[name][0]), (order, name)
+    small.close()
+    large.close()
+
+
 @pytest.mark.parametrize("caps", [dict(K=64, P=34, V=60, T=100), dict(K=12, P=16, V=40, T=8)])
 def test_caps_do_not_change_results(rpd, ctx, caps):
     """FP64 form: a solve whose small caps send cells through re-queue passes
