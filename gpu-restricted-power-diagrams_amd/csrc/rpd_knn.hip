@@ -318,13 +318,36 @@ __device__ int select_rank(const typename KeyT::Buf &buf, int M, int rank, uint3
         }
     }
     uint4 *h4 = reinterpret_cast<uint4 *>(hist);
+    // The first word's digits start at the highest bit in which two keys in
+    // play differ (the keys' distances span a few octaves, so the top bits
+    // are mostly shared): windows of 8 bits from there down, the last one
+    // at bit 0 (windows may overlap; bits every key shares make one bin).
+    int sh0 = -1;  // -1: every key in play has the same first word
+    {
+        uint32_t mine = 0;
+        bool has = false;
+#pragma unroll
+        for (int j = 0; j < SL; ++j)
+            if (!has && ((alive >> j) & 1)) {
+                mine = w[j][0];
+                has = true;
+            }
+        const uint32_t ref = rlane(mine, lowest_bit(ballot(has)));
+        uint32_t diff = 0;
+#pragma unroll
+        for (int j = 0; j < SL; ++j)
+            if ((alive >> j) & 1) diff |= w[j][0] ^ ref;
+        diff = uint32_t(rlane(wave_incl_scan_op(int(diff), 0, [](int a, int b) { return a | b; }), 63));
+        if (diff) sh0 = max(31 - __clz(int(diff)) - 7, 0);
+    }
     // words in an unrolled loop, digits of a word in a rolled one: the word
     // index stays a constant, so w[j][q] is a register, not a select chain
     bool one = false;
 #pragma unroll
     for (int q = 0; q < KeyT::kWords; ++q) {
+        int sh = q == 0 ? sh0 : 24;
 #pragma unroll 1
-        for (int sh = 24; sh >= 0 && !one; sh -= 8) {
+        for (; sh >= 0 && !one; sh = sh > 0 && sh < 8 ? 0 : sh - 8) {
             h4[lane] = make_uint4(0u, 0u, 0u, 0u);
             wave_lds_sync();
 #pragma unroll
