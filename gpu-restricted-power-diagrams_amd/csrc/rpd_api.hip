@@ -67,6 +67,8 @@ struct FailBufs {
 // pinned host words of a context: queue lengths (one per stream), then the
 // counters of each failure list, then the final status histogram
 constexpr int kPinCounters = 16;
+// the pinned word that receives a queued binning's crowded-cell flag
+constexpr int kPinBinFlag = kPinCounters - 4;
 // smallest triangle cap a solve uses
 constexpr int kTriangleFloor = 256;
 constexpr int kPinHist = kPinCounters + 6 * kFailCounters;
@@ -360,7 +362,11 @@ __global__ __launch_bounds__(256) void mark_neighbours_kernel(const char *__rest
     }
 }
 
-int ensure_binning(rpd_ctx *c, int prec) {
+// Bins the seeds in key precision prec unless they are.  With pending set
+// the binning is only queued: *pending tells whether it was, and the caller,
+// once it has synchronised the stream, finishes it with finish_binning.
+int ensure_binning(rpd_ctx *c, int prec, bool *pending = nullptr) {
+    if (pending) *pending = false;
     if (c->bin_prec == prec) return RPD_OK;
     const int64_t n = c->n;
     const int dim = rpd_knn_dim(n);
@@ -370,12 +376,31 @@ int ensure_binning(rpd_ctx *c, int prec) {
     RPDCHK(c, c->counters.ensure(size_t(nc + 1) * 4), "alloc counters");
     RPDCHK(c, c->stored.ensure(size_t(n) * (prec == 0 ? 16 : 32)), "alloc stored");
     RPDCHK(c, c->stored64.ensure(size_t(n) * 32), "alloc stored64");
-    RPDCHK(c,
-           rpd_bin_seeds(c->stream, c->seeds.as<double4>(), n, prec, dim, c->bin_scratch, c->perm.as<int>(),
-                         c->counters.as<int>(), c->stored.ptr, c->stored64.as<double4>()),
-           "binning");
+    if (pending) {
+        RPDCHK(c,
+               rpd_bin_seeds_async(c->stream, c->seeds.as<double4>(), n, prec, dim, c->bin_scratch,
+                                   c->pin + kPinBinFlag, c->perm.as<int>(), c->counters.as<int>(), c->stored.ptr,
+                                   c->stored64.as<double4>()),
+               "binning");
+        *pending = true;
+    } else {
+        RPDCHK(c,
+               rpd_bin_seeds(c->stream, c->seeds.as<double4>(), n, prec, dim, c->bin_scratch, c->perm.as<int>(),
+                             c->counters.as<int>(), c->stored.ptr, c->stored64.as<double4>()),
+               "binning");
+    }
     c->bin_prec = prec;
     c->dim = dim;
+    return RPD_OK;
+}
+// After the stream is synchronised: the crowded-cell path of a queued binning
+// when its flag is set
+int finish_binning(rpd_ctx *c) {
+    if (!c->pin[kPinBinFlag]) return RPD_OK;
+    RPDCHK(c,
+           rpd_bin_seeds_redo(c->stream, c->seeds.as<double4>(), c->n, c->bin_prec, c->dim, c->bin_scratch,
+                              c->perm.as<int>(), c->stored.ptr, c->stored64.as<double4>()),
+           "binning (crowded cells)");
     return RPD_OK;
 }
 
@@ -733,11 +758,12 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     }
     if (batch) {
         if (c->bin_prec != kprec) return set_err(c, RPD_E_STATE, "rpd_solve_batch: binned in another precision");
-    } else {
-        RPDCHK(c, ensure_binning(c, kprec), "binning");
     }
+    bool bin_pending = false;
+    if (!batch) RPDCHK(c, ensure_binning(c, kprec, &bin_pending), "binning");
     HIPCHK(c, hipEventRecord(c->ev[0][1], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bin_pending) RPDCHK(c, finish_binning(c), "binning");
     st.ms_bin = elapsed_ms(c->ev[0][0], c->ev[0][1]);
     if (new_wmin) {
         c->wmin = c->wall = INFINITY;

@@ -77,6 +77,13 @@ int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int
 int rpd_bin_seeds_in(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, void *scratch,
                      size_t scratch_bytes, int *flag_host, int *perm, int *counters, void *stored, double4 *stored64);
 size_t rpd_bin_scratch_bytes(hipStream_t st, int64_t n, int dim);
+// The binning queued without a wait: the crowded-cell flag goes to
+// flag_pinned (pinned host memory).  Once the stream has been synchronised, a
+// set flag calls for rpd_bin_seeds_redo (the crowded-cell path; it waits).
+int rpd_bin_seeds_async(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch,
+                        int *flag_pinned, int *perm, int *counters, void *stored, double4 *stored64);
+int rpd_bin_seeds_redo(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch,
+                       int *perm, void *stored, double4 *stored64);
 // Several diagrams in one launch (a packed batch): request r owns the
 // stored ids [base_r, base_r + n_r) of the shared arrays (seeds, sorted
 // copies, permutation with original indices base_r + i, outputs), and its

@@ -646,20 +646,108 @@ int rpd_knn_dim(int64_t n) {
 // Bytes of binning scratch for n seeds on a dim^3 grid (0 on a HIP error).
 // Layout: cell_in, cell_out, rank / orig (n ints each), counts (dim^3 + 1),
 // the crowded-cell flag, rocPRIM temporary storage.
-size_t rpd_bin_scratch_bytes(hipStream_t st, int64_t n, int dim) {
+
+// Layout of the binning's scratch: cell ids (in, out), ranks, per-cell
+// counts, the crowded-cell flag, the library sort's / scan's temporary.
+struct BinLayout {
+    size_t off_cell_out, off_rank, off_counts, off_flag, off_tmp, sort_bytes, scan_bytes, need;
+    unsigned end_bit;
+};
+static int bin_layout(hipStream_t st, int64_t n, int dim, BinLayout &b) {
     const int64_t nc = int64_t(dim) * dim * dim;
-    size_t sort_bytes = 0, scan_bytes = 0;
+    const size_t bytes_cell = size_t(n) * 4;
     int *null_i = nullptr;
-    unsigned end_bit = 1;
-    while ((int64_t(1) << end_bit) < nc) ++end_bit;
-    if (rocprim::radix_sort_pairs(nullptr, sort_bytes, null_i, null_i, null_i, null_i, n, 0u, end_bit, st) !=
+    b.end_bit = 1;
+    while ((int64_t(1) << b.end_bit) < nc) ++b.end_bit;
+    b.sort_bytes = b.scan_bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, b.sort_bytes, null_i, null_i, null_i, null_i, n, 0u, b.end_bit, st) !=
         hipSuccess)
-        return 0;
-    if (rocprim::exclusive_scan(nullptr, scan_bytes, null_i, null_i, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
+        return RPD_E_DEVICE;
+    if (rocprim::exclusive_scan(nullptr, b.scan_bytes, null_i, null_i, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
         hipSuccess)
-        return 0;
-    const size_t off_flag = (3 * size_t(n) * 4 + size_t(nc + 1) * 4 + 255) & ~size_t(255);
-    return off_flag + 256 + std::max(sort_bytes, scan_bytes) + 256;
+        return RPD_E_DEVICE;
+    b.off_cell_out = bytes_cell;
+    b.off_rank = 2 * bytes_cell;
+    b.off_counts = 3 * bytes_cell;
+    b.off_flag = (b.off_counts + size_t(nc + 1) * 4 + 255) & ~size_t(255);
+    b.off_tmp = b.off_flag + 256;
+    b.need = b.off_tmp + std::max(b.sort_bytes, b.scan_bytes) + 256;
+    return RPD_OK;
+}
+
+size_t rpd_bin_scratch_bytes(hipStream_t st, int64_t n, int dim) {
+    BinLayout b;
+    return bin_layout(st, n, dim, b) == RPD_OK ? b.need : 0;
+}
+
+template <typename R>
+static void gather_seeds(hipStream_t st, const double4 *seeds, const int *perm, int64_t n, void *stored,
+                         double4 *stored64) {
+    const int tb = 256;
+    const unsigned nb = unsigned((n + tb - 1) / tb);
+    gather_seeds_kernel<R><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<typename Vec4T<R>::T *>(stored), stored64);
+}
+
+// Queues the binning; its crowded-cell flag is copied to flag_host.
+static int bin_queue(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, void *scratch_ptr,
+                     size_t scratch_bytes, int *flag_host, int *perm, int *counters, void *stored,
+                     double4 *stored64) {
+    BinLayout b;
+    if (bin_layout(st, n, dim, b) != RPD_OK) return RPD_E_DEVICE;
+    if (b.need > scratch_bytes) return RPD_E_LIMIT;
+    const int64_t nc = int64_t(dim) * dim * dim;
+    char *base = static_cast<char *>(scratch_ptr);
+    int *cell_in = reinterpret_cast<int *>(base);
+    int *rank = reinterpret_cast<int *>(base + b.off_rank);
+    int *counts = reinterpret_cast<int *>(base + b.off_counts);
+    int *big = reinterpret_cast<int *>(base + b.off_flag);
+    void *tmp = base + b.off_tmp;
+    // the counts and the flag, which follows them, cleared at once
+    if (hipMemsetAsync(counts, 0, b.off_flag + sizeof(int) - b.off_counts, st) != hipSuccess) return RPD_E_DEVICE;
+    const int tb = 256;
+    const unsigned nb = unsigned((n + tb - 1) / tb);
+    if (prec == 0)
+        bin_cells_kernel<float><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, rank);
+    else
+        bin_cells_kernel<double><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, rank);
+    size_t scb = b.scan_bytes;
+    if (rocprim::exclusive_scan(tmp, scb, counts, counters, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
+        hipSuccess)
+        return RPD_E_DEVICE;
+    scatter_cells_kernel<<<nb, tb, 0, st>>>(cell_in, rank, n, counters, perm);
+    sort_cells_kernel<<<unsigned((nc + tb - 1) / tb), tb, 0, st>>>(counters, nc, perm, big);
+    // gathered at once; a crowded cell (rare) sorts and gathers again
+    if (prec == 0)
+        gather_seeds<float>(st, seeds, perm, n, stored, stored64);
+    else
+        gather_seeds<double>(st, seeds, perm, n, stored, stored64);
+    if (hipMemcpyAsync(flag_host, big, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return RPD_E_DEVICE;
+    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+}
+
+// The crowded-cell path: the permutation of a radix sort of (cell, index),
+// then the gather again; waits for the stream.
+static int bin_redo(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, void *scratch_ptr,
+                    int *perm, void *stored, double4 *stored64) {
+    BinLayout b;
+    if (bin_layout(st, n, dim, b) != RPD_OK) return RPD_E_DEVICE;
+    char *base = static_cast<char *>(scratch_ptr);
+    int *cell_in = reinterpret_cast<int *>(base);
+    int *cell_out = reinterpret_cast<int *>(base + b.off_cell_out);
+    int *rank = reinterpret_cast<int *>(base + b.off_rank);
+    void *tmp = base + b.off_tmp;
+    const int tb = 256;
+    const unsigned nb = unsigned((n + tb - 1) / tb);
+    iota_kernel<<<nb, tb, 0, st>>>(rank, n);
+    size_t sb = b.sort_bytes;
+    if (rocprim::radix_sort_pairs(tmp, sb, cell_in, cell_out, rank, perm, n, 0u, b.end_bit, st) != hipSuccess)
+        return RPD_E_DEVICE;
+    if (prec == 0)
+        gather_seeds<float>(st, seeds, perm, n, stored, stored64);
+    else
+        gather_seeds<double>(st, seeds, perm, n, stored, stored64);
+    if (hipStreamSynchronize(st) != hipSuccess) return RPD_E_DEVICE;
+    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }
 
 int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch, int *perm,
@@ -670,68 +758,25 @@ int rpd_bin_seeds(hipStream_t st, const double4 *seeds, int64_t n, int prec, int
     int flag_host = 0;
     return rpd_bin_seeds_in(st, seeds, n, prec, dim, scratch.ptr, need, &flag_host, perm, counters, stored, stored64);
 }
-
 int rpd_bin_seeds_in(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, void *scratch_ptr,
                      size_t scratch_bytes, int *flag_host, int *perm, int *counters, void *stored, double4 *stored64) {
-    const int64_t nc = int64_t(dim) * dim * dim;
-    const size_t bytes_cell = size_t(n) * 4;
-    size_t sort_bytes = 0, scan_bytes = 0;
-    int *null_i = nullptr;
-    unsigned end_bit = 1;
-    while ((int64_t(1) << end_bit) < nc) ++end_bit;
-    if (rocprim::radix_sort_pairs(nullptr, sort_bytes, null_i, null_i, null_i, null_i, n, 0u, end_bit, st) !=
-        hipSuccess)
-        return RPD_E_DEVICE;
-    if (rocprim::exclusive_scan(nullptr, scan_bytes, null_i, null_i, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
-        hipSuccess)
-        return RPD_E_DEVICE;
-    const size_t off_cell_out = bytes_cell, off_rank = 2 * bytes_cell, off_counts = 3 * bytes_cell;
-    const size_t off_flag = (off_counts + size_t(nc + 1) * 4 + 255) & ~size_t(255);
-    const size_t off_tmp = off_flag + 256;
-    const size_t need = off_tmp + std::max(sort_bytes, scan_bytes) + 256;
-    if (need > scratch_bytes) return RPD_E_LIMIT;
-    char *base = static_cast<char *>(scratch_ptr);
-    int *cell_in = reinterpret_cast<int *>(base);
-    int *cell_out = reinterpret_cast<int *>(base + off_cell_out);
-    int *rank = reinterpret_cast<int *>(base + off_rank);
-    int *counts = reinterpret_cast<int *>(base + off_counts);
-    int *big = reinterpret_cast<int *>(base + off_flag);
-    void *tmp = base + off_tmp;
-    if (hipMemsetAsync(counts, 0, size_t(nc + 1) * 4, st) != hipSuccess ||
-        hipMemsetAsync(big, 0, sizeof(int), st) != hipSuccess)
-        return RPD_E_DEVICE;
-    const int tb = 256;
-    const unsigned nb = unsigned((n + tb - 1) / tb);
-    if (prec == 0)
-        bin_cells_kernel<float><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, rank);
-    else
-        bin_cells_kernel<double><<<nb, tb, 0, st>>>(seeds, n, dim, cell_in, counts, rank);
-    size_t scb = scan_bytes;
-    if (rocprim::exclusive_scan(tmp, scb, counts, counters, 0, size_t(nc + 1), rocprim::plus<int>(), st) !=
-        hipSuccess)
-        return RPD_E_DEVICE;
-    scatter_cells_kernel<<<nb, tb, 0, st>>>(cell_in, rank, n, counters, perm);
-    sort_cells_kernel<<<unsigned((nc + tb - 1) / tb), tb, 0, st>>>(counters, nc, perm, big);
-    auto gather = [&]() {
-        if (prec == 0)
-            gather_seeds_kernel<float><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<float4 *>(stored), stored64);
-        else
-            gather_seeds_kernel<double><<<nb, tb, 0, st>>>(seeds, perm, n, static_cast<double4 *>(stored), stored64);
-    };
-    // gathered at once; a crowded cell (rare) sorts and gathers again
-    gather();
-    if (hipMemcpyAsync(flag_host, big, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return RPD_E_DEVICE;
-    if (*flag_host) {
-        iota_kernel<<<nb, tb, 0, st>>>(rank, n);
-        size_t sb = sort_bytes;
-        if (rocprim::radix_sort_pairs(tmp, sb, cell_in, cell_out, rank, perm, n, 0u, end_bit, st) != hipSuccess)
-            return RPD_E_DEVICE;
-        gather();
-        if (hipStreamSynchronize(st) != hipSuccess) return RPD_E_DEVICE;
-    }
-    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+    const int r = bin_queue(st, seeds, n, prec, dim, scratch_ptr, scratch_bytes, flag_host, perm, counters, stored,
+                            stored64);
+    if (r != RPD_OK) return r;
+    if (hipStreamSynchronize(st) != hipSuccess) return RPD_E_DEVICE;
+    if (*flag_host) return bin_redo(st, seeds, n, prec, dim, scratch_ptr, perm, stored, stored64);
+    return RPD_OK;
+}
+int rpd_bin_seeds_async(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch,
+                        int *flag_pinned, int *perm, int *counters, void *stored, double4 *stored64) {
+    const size_t need = rpd_bin_scratch_bytes(st, n, dim);
+    if (need == 0) return RPD_E_DEVICE;
+    if (scratch.ensure(need) != RPD_OK) return RPD_E_NOMEM;
+    return bin_queue(st, seeds, n, prec, dim, scratch.ptr, need, flag_pinned, perm, counters, stored, stored64);
+}
+int rpd_bin_seeds_redo(hipStream_t st, const double4 *seeds, int64_t n, int prec, int dim, DevBuf &scratch,
+                       int *perm, void *stored, double4 *stored64) {
+    return bin_redo(st, seeds, n, prec, dim, scratch.ptr, perm, stored, stored64);
 }
 
 namespace rpd {

@@ -43,3 +43,30 @@ def test_knn_fewer_than_k(rpd, orc, ctx):
     _, ref, _ = orc.knn(x, 64, precision=1, method=1)
     assert np.array_equal(got, ref)
     assert (got[:, 39:] == -1).all() and (got[:, :39] >= 0).all()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_solve_binning_crowded_cells(rpd, orc, ctx, prec):
+    """A solve queues the binning of new seeds without waiting for it; grid
+    cells of more than 32 seeds take the sort path once the solve has
+    synchronised its stream.  The permutation is then the oracle's, and the
+    solve's cells and statuses are those of a context that binned the same
+    seeds through the waiting path (ctx.knn) before solving."""
+    n = 6000
+    x = rpd.generate_seeds(rpd.SEEDS_UNIFORM, n, seed=7)
+    k = 2 * n // 3
+    x[:k, :3] = 500.0 + (x[:k, :3] - 500.0) * 0.02
+    caps = dict(K=40, P=40, V=64, T=100, precision=prec)
+    key_prec = 1 if prec == rpd.PREC_FP64 else 0
+    ctx.set_seeds(x)
+    ctx.solve(None, **caps)
+    ref_perm, _, _ = orc.knn(x, 40, precision=key_prec, method=0)
+    print("crowded solve", prec, flush=True)
+    assert np.array_equal(ctx.perm(), ref_perm)
+    c2 = rpd.Context(0)
+    c2.set_seeds(x)
+    c2.knn(40, key_prec)
+    c2.solve(None, **caps)
+    assert np.array_equal(ctx.moments(), c2.moments())
+    assert np.array_equal(ctx.status(), c2.status())
+    c2.close()
