@@ -139,6 +139,7 @@ struct rpd_ctx {
     hipEvent_t ev[4][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
     hipEvent_t dep[3] = {};    // per chain: orders work between its streams; [2]: pass 1's queue length copied
     int *pin = nullptr;        // kPinWords pinned host words
+    int *pin_dev = nullptr;    // their device address (nullptr: copies go through the copy engine)
     double *pin_red = nullptr;  // 2 x kWminBlocks pinned doubles: per-block weight minima
     // smallest weight of the non-border seeds and of all seeds, for the
     // current seeds (valid once wmin_ok)
@@ -466,6 +467,10 @@ int rpd_create_streams(int device, int n_streams, rpd_ctx **out) {
         c->pin = nullptr;
         rpd_destroy(c);
         return RPD_E_DEVICE;
+    }
+    if (hipHostGetDevicePointer(reinterpret_cast<void **>(&c->pin_dev), c->pin, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        c->pin_dev = nullptr;
     }
     if (hipHostMalloc(reinterpret_cast<void **>(&c->pin_red), 2 * kWminBlocks * sizeof(double),
                       hipHostMallocDefault) != hipSuccess) {
@@ -917,6 +922,8 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         int out = -1;              // rs buffer written, -1: none
         int out_stride = 0, out_cap = 0;
     };
+    // device address of a pinned word (nullptr: no mapping)
+    auto pin_d = [&](const int *host) -> int * { return c->pin_dev ? c->pin_dev + (host - c->pin) : nullptr; };
     // kNN + bisector kernel, the queue length copied to the host (not waited for)
     auto pass_front = [&](Pass &ps, const int *ids, int fl_out, int *fl_cnt_host, const Resume &rz,
                           ClipLaunch &L) -> int {
@@ -980,6 +987,8 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         L.req_of = req_of;
         L.qcount_zeroed = fresh_q[sl];
         fresh_q[sl] = false;
+        L.nq_dev_out = pin_d(c->pin + sl);
+        L.fl_cnt_dev_out = fl_cnt_host ? pin_d(fl_cnt_host) : nullptr;
         return rpd_launch_cells(s, ps.form, L, c->pin + sl, false);
     };
     // the finish kernel of a pass whose bisector kernel (and queue length) is done
@@ -1041,7 +1050,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
             fresh_fl[fl] = false;
             return true;
         }
-        return hipMemsetAsync(c->fl[fl].cnt.ptr, 0, kFailCounters * sizeof(int), s) == hipSuccess;
+        return rpd_zero_ints(s, c->fl[fl].cnt.as<int>(), kFailCounters) == RPD_OK;
     };
     // Record buffer `which` for cap records of stride bytes, counter cleared.
     auto resume_buf = [&](int which, int64_t cap, int stride, hipStream_t s) -> int {
@@ -1051,8 +1060,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
             fresh_rs[which] = false;
             return RPD_OK;
         }
-        return hipMemsetAsync(c->rs_cnt.as<int>() + which, 0, sizeof(int), s) == hipSuccess ? RPD_OK
-                                                                                              : RPD_E_DEVICE;
+        return rpd_zero_ints(s, c->rs_cnt.as<int>() + which, 1);
     };
     // Where the previous pass of a chain left the states of its resumable
     // cells (rs buffer, stride, plane room), or which = -1.
@@ -1089,6 +1097,13 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     };
     ClipLaunch L1{};  // pass 1's launch; its finish kernel's records serve chain B's first pass
     auto cnt_pin = [&](int list) { return c->pin + kPinCounters + kFailCounters * list; };
+    // a failure list's counters to their pinned words, on stream s
+    auto copy_counts = [&](int list, hipStream_t s) -> bool {
+        if (c->pin_dev)
+            return rpd_pin_copy(s, c->fl[list].cnt.as<int>(), pin_d(cnt_pin(list)), kFailCounters) == RPD_OK;
+        return hipMemcpyAsync(cnt_pin(list), c->fl[list].cnt.ptr, kFailCounters * sizeof(int), hipMemcpyDeviceToHost,
+                              s) == hipSuccess;
+    };
     // plans the chain's next pass from the counters and queues its kNN and
     // bisector kernels (phase 1), or ends the chain (phase 3)
     auto chain_front = [&](Chain &ch) -> int {
@@ -1159,10 +1174,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
                 return RPD_E_DEVICE;
             int r = rpd_launch_finish(s, RPD_PREC_FP64, L, int(ch.n_scr), 0);
             if (r != RPD_OK) return r;
-            if (hipEventRecord(c->ev[ch.sl][3], s) != hipSuccess ||
-                hipMemcpyAsync(cnt_pin(ch.nxt), c->fl[ch.nxt].cnt.ptr, kFailCounters * sizeof(int),
-                               hipMemcpyDeviceToHost, s) != hipSuccess)
-                return RPD_E_DEVICE;
+            if (hipEventRecord(c->ev[ch.sl][3], s) != hipSuccess || !copy_counts(ch.nxt, s)) return RPD_E_DEVICE;
             ch.phase = 2;
             return RPD_OK;
         }
@@ -1248,9 +1260,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
             if (ch.n_scr > 0 && (hipEventRecord(c->dep[ch.dep], s2) != hipSuccess ||
                                  hipStreamWaitEvent(s, c->dep[ch.dep], 0) != hipSuccess))
                 return RPD_E_DEVICE;
-            if (hipMemcpyAsync(cnt_pin(ch.nxt), c->fl[ch.nxt].cnt.ptr, kFailCounters * sizeof(int),
-                               hipMemcpyDeviceToHost, s) != hipSuccess)
-                return RPD_E_DEVICE;
+            if (!copy_counts(ch.nxt, s)) return RPD_E_DEVICE;
             ch.phase = 2;
             progress = true;
         }
@@ -1383,7 +1393,10 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         L.req_of = req_of;
         L.qcount_zeroed = fresh_q[0];
         fresh_q[0] = false;
+        L.nq_dev_out = pin_d(c->pin + 0);
+        L.fl_cnt_dev_out = pin_d(cnt_pin(0));
         RPDCHK(c, rpd_launch_cells(s, form0, L, c->pin + 0, false), "pass 1 cells");
+        L.nq_dev_out = L.fl_cnt_dev_out = nullptr;
         L.qcount_zeroed = false;
         HIPCHK(c, hipEventRecord(c->dep[2], s));  // queue length and failure counters copied
         L.fl_cnt = c->fl[1].cnt.as<int>();
@@ -1399,7 +1412,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         L.nq_dev = L.qcount;
         RPDCHK(c, rpd_launch_finish(s, form0, L, int(n1), 0), "pass 1 finish");
         HIPCHK(c, hipEventRecord(c->ev[0][3], s));
-        HIPCHK(c, hipMemcpyAsync(cnt_pin(1), c->fl[1].cnt.ptr, kFailCounters * sizeof(int), hipMemcpyDeviceToHost, s));
+        if (!copy_counts(1, s)) return set_err(c, RPD_E_DEVICE, "pass 1 counters");
         HIPCHK(c, hipEventSynchronize(c->dep[2]));
         p1.nq = c->pin[0];
         // no cell to clip (every one skipped): the bisector kernel did not

@@ -1838,7 +1838,7 @@ int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out,
     const int r = make_args(form, L, a);
     if (r != RPD_OK) return r;
     const int S = L.V <= 64 ? 1 : (L.V <= 128 ? 2 : 4);
-    if (!L.qcount_zeroed && hipMemsetAsync(L.qcount, 0, sizeof(int), st) != hipSuccess) return RPD_E_DEVICE;
+    if (!L.qcount_zeroed && rpd_zero_ints(st, L.qcount, 1) != RPD_OK) return RPD_E_DEVICE;
 
     // kernel A: the bisector loop of every cell
     {
@@ -1875,12 +1875,46 @@ int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq_out,
         if (hipGetLastError() != hipSuccess) return RPD_E_DEVICE;
     }
     if (L.ev_mid && hipEventRecord(L.ev_mid, st) != hipSuccess) return RPD_E_DEVICE;
-    if (hipMemcpyAsync(nq_out, L.qcount, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return RPD_E_DEVICE;
-    if (L.fl_cnt && L.fl_cnt_host &&
-        hipMemcpyAsync(L.fl_cnt_host, L.fl_cnt, kFailCounters * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
-        return RPD_E_DEVICE;
+    if (L.nq_dev_out) {
+        const bool fl = L.fl_cnt && L.fl_cnt_host && L.fl_cnt_dev_out;
+        if (rpd_pin_copy(st, L.qcount, L.nq_dev_out, 1, fl ? L.fl_cnt : nullptr, fl ? L.fl_cnt_dev_out : nullptr,
+                         fl ? kFailCounters : 0) != RPD_OK)
+            return RPD_E_DEVICE;
+    } else {
+        if (hipMemcpyAsync(nq_out, L.qcount, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+            return RPD_E_DEVICE;
+        if (L.fl_cnt && L.fl_cnt_host &&
+            hipMemcpyAsync(L.fl_cnt_host, L.fl_cnt, kFailCounters * sizeof(int), hipMemcpyDeviceToHost, st) !=
+                hipSuccess)
+            return RPD_E_DEVICE;
+    }
     if (!wait) return RPD_OK;
     return hipStreamSynchronize(st) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+}
+
+namespace rpd {
+__global__ __launch_bounds__(64) void pin_copy_kernel(const int *src0, int *dst0, int n0, const int *src1,
+                                                      int *dst1, int n1) {
+    for (int i = int(threadIdx.x); i < n0; i += 64) dst0[i] = src0[i];
+    for (int i = int(threadIdx.x); i < n1; i += 64) dst1[i] = src1[i];
+    __threadfence_system();  // the host reads them once the stream is done
+}
+}  // namespace rpd
+
+namespace rpd {
+__global__ __launch_bounds__(64) void zero_ints_kernel(int *p, int n) {
+    for (int i = int(threadIdx.x); i < n; i += 64) p[i] = 0;
+}
+}  // namespace rpd
+
+int rpd_zero_ints(hipStream_t st, int *p, int n) {
+    zero_ints_kernel<<<1, 64, 0, st>>>(p, n);
+    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
+}
+
+int rpd_pin_copy(hipStream_t st, const int *src0, int *dst0, int n0, const int *src1, int *dst1, int n1) {
+    pin_copy_kernel<<<1, 64, 0, st>>>(src0, dst0, n0, src1, dst1, n1);
+    return hipGetLastError() == hipSuccess ? RPD_OK : RPD_E_DEVICE;
 }
 
 int rpd_launch_finish(hipStream_t st, int form, const ClipLaunch &L, int nq, int beside) {

@@ -173,6 +173,10 @@ struct ClipLaunch {
     // resumable cells at fl_sec[k]
     int *fl_cnt, *fl_eo, *fl_sec;
     int *fl_cnt_host;      // host copy of the counters, filled by rpd_launch_cells (nullable)
+    // nullable: device addresses of nq_out / fl_cnt_host (mapped pinned
+    // memory), written by a one-wave copy kernel instead of copy engine
+    // blits, which wait for room beside a running finish kernel
+    int *nq_dev_out, *fl_cnt_dev_out;
     // resumable cells (security radius not reached), see ClipArgs in rpd_clip.hip
     char *rs_out;          // records written by this pass (nullable)
     int rs_out_stride, rs_cap;
@@ -250,6 +254,15 @@ size_t rpd_clip_record_bytes(int form, int P, int V);
 int rpd_clip_max_T(int form, int P, int V, bool hess = false);
 // form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32.  The bisector-loop
 // kernel of L.n_ids cells, then copies to the host the number of cells it
+// Copies n0 ints src0 -> dst0 and n1 src1 -> dst1 with one one-wave kernel
+// (dst: device addresses of mapped pinned host memory; n1 may be 0).  Small
+// copies of counters to the host this way need one free wave slot, where a
+// copy engine blit waits for room beside a running finish kernel.
+int rpd_pin_copy(hipStream_t st, const int *src0, int *dst0, int n0, const int *src1 = nullptr, int *dst1 = nullptr,
+                 int n1 = 0);
+// Clears n device ints with one one-wave kernel (a fill blit, like a copy,
+// waits for room beside a running finish kernel).
+int rpd_zero_ints(hipStream_t st, int *p, int n);
 // queued (and the failure counters when L.fl_cnt_host is set); waits for
 // them unless wait is false (the caller synchronizes st before reading).
 int rpd_launch_cells(hipStream_t st, int form, const ClipLaunch &L, int *nq, bool wait = true);
