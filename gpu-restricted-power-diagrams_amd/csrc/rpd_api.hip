@@ -914,6 +914,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         int64_t n_ids = 0;
         int caps[4] = {0, 0, 0, 0};
         int nq = 0;
+        bool finish_only = false;  // a finish kernel over earlier records (events 1 and 3 only)
     };
     // resumable cells: records read (in, FP64 passes) and written (out)
     struct Resume {
@@ -1002,7 +1003,8 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     double ms_knn = 0, ms_fb_knn = 0, ms_fb = 0;
     auto account = [&](const Pass &ps, int index) {
         const int sl = ps.slot;
-        const double mk = elapsed_ms(c->ev[sl][0], c->ev[sl][1]), mc = elapsed_ms(c->ev[sl][1], c->ev[sl][3]);
+        const double mk = ps.finish_only ? 0.0 : elapsed_ms(c->ev[sl][0], c->ev[sl][1]),
+                     mc = elapsed_ms(c->ev[sl][1], c->ev[sl][3]);
         const int pi = std::min(index, RPD_MAX_PASSES - 1);
         st.pass_cells[pi] += int32_t(ps.n_ids);
         for (int k = 0; k < 4; ++k) st.pass_caps[pi][k] = ps.caps[k];
@@ -1169,9 +1171,8 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
             L.rs_in = nullptr;
             L.ev_mid = nullptr;
             L.dcnt = dcnt_of(1);
-            if (hipEventRecord(c->ev[ch.sl][0], s) != hipSuccess || hipEventRecord(c->ev[ch.sl][1], s) != hipSuccess ||
-                hipEventRecord(c->ev[ch.sl][2], s) != hipSuccess)
-                return RPD_E_DEVICE;
+            ch.pz.finish_only = true;
+            if (hipEventRecord(c->ev[ch.sl][1], s) != hipSuccess) return RPD_E_DEVICE;
             int r = rpd_launch_finish(s, RPD_PREC_FP64, L, int(ch.n_scr), 0);
             if (r != RPD_OK) return r;
             if (hipEventRecord(c->ev[ch.sl][3], s) != hipSuccess || !copy_counts(ch.nxt, s)) return RPD_E_DEVICE;
@@ -1467,7 +1468,11 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         c->status.as<int>(), n, c->cnt.as<int>());
     // through pinned memory: a copy into pageable memory is staged and
     // starts late
-    HIPCHK(c, hipMemcpyAsync(c->pin + kPinHist, c->cnt.ptr, sizeof(st.status_hist), hipMemcpyDeviceToHost, c->stream));
+    if (c->pin_dev)
+        RPDCHK(c, rpd_pin_copy(c->stream, c->cnt.as<int>(), c->pin_dev + kPinHist, RPD_STATUS_NUM), "status histogram");
+    else
+        HIPCHK(c, hipMemcpyAsync(c->pin + kPinHist, c->cnt.ptr, sizeof(st.status_hist), hipMemcpyDeviceToHost,
+                                 c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::memcpy(st.status_hist, c->pin + kPinHist, sizeof(st.status_hist));
     st.ms_knn = ms_knn + ms_fb_knn;

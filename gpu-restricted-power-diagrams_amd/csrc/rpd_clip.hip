@@ -42,6 +42,7 @@
 // form: everything in FP32, no filter.  Moments are accumulated in FP64.
 
 #include <hip/hip_runtime.h>
+#include <atomic>
 
 #include "rpd_internal.h"
 #include "rpd_wave.h"
@@ -1722,11 +1723,15 @@ using namespace rpd;
 static size_t plane_bytes(int form) { return form == RPD_PREC_FP32 ? 16 : 32; }
 
 // LDS bytes a workgroup may use on the current device.
+// LDS a workgroup may use on the current device (asked once per device)
 static size_t lds_limit() {
+    static std::atomic<int> cached[64];
     int dev = 0, v = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    if (dev >= 0 && dev < 64 && (v = cached[dev].load(std::memory_order_relaxed)) > 0) return size_t(v);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
         return 64 * 1024;
+    if (dev >= 0 && dev < 64) cached[dev].store(v, std::memory_order_relaxed);
     return size_t(v);
 }
 
