@@ -1438,6 +1438,33 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     cb.cap = caps1;
     cb.phase = 4;
     cb.from_finish = true;
+    // The final status histogram: counts into cnt (cleared at the solve's
+    // start), then to its pinned words, on stream s.  The counts of a
+    // histogram queued before are cleared first; that one has finished (the
+    // host has seen its stream idle since: its chain went on).
+    const unsigned hist_blocks = unsigned(std::min<int64_t>(1024, (n + 255) / 256));
+    bool hist_queued = false;
+    auto queue_hist = [&](hipStream_t s) -> int {
+        if (hist_queued && rpd_zero_ints(s, c->cnt.as<int>(), RPD_STATUS_NUM) != RPD_OK) return RPD_E_DEVICE;
+        status_hist_kernel<<<hist_blocks, 256, 0, s>>>(c->status.as<int>(), n, c->cnt.as<int>());
+        if (hipGetLastError() != hipSuccess) return RPD_E_DEVICE;
+        hist_queued = true;
+        // through pinned memory: a copy into pageable memory is staged and
+        // starts late
+        if (c->pin_dev) return rpd_pin_copy(s, c->cnt.as<int>(), c->pin_dev + kPinHist, RPD_STATUS_NUM);
+        return hipMemcpyAsync(c->pin + kPinHist, c->cnt.ptr, sizeof(st.status_hist), hipMemcpyDeviceToHost, s) ==
+                       hipSuccess
+                   ? RPD_OK
+                   : RPD_E_DEVICE;
+    };
+    // Once one chain is done, the other's queued pass may be the solve's
+    // last: the histogram is queued behind that pass's counters on its
+    // stream, and is ready when the host learns that the pass failed no
+    // cell.  It holds if the chain then ends (its pass count grew by one
+    // only).  A chain that is done stays done, so every such histogram is
+    // queued for the same chain, on its stream.
+    const Chain *hist_for = nullptr;
+    int hist_k = -1;
     RPDCHK(c, chain_front(ca), "re-queue chain (bisector failures)");
     while (ca.phase != 3 || cb.phase != 3) {
         bool progress = false;
@@ -1454,6 +1481,12 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         }
         RPDCHK(c, chain_poll(ca, progress), "re-queue chain (bisector failures)");
         RPDCHK(c, chain_poll(cb, progress), "re-queue chain (finish failures)");
+        const Chain *last = ca.phase == 3 && cb.phase == 2 ? &cb : (cb.phase == 3 && ca.phase == 2 ? &ca : nullptr);
+        if (last && !(hist_for == last && hist_k == last->k)) {
+            RPDCHK(c, queue_hist(c->streams[last->sl]), "status histogram");
+            hist_for = last;
+            hist_k = last->k;
+        }
         if (!progress) std::this_thread::yield();
     }
     const int steps_a = ca.k, steps_b = cb.k;
@@ -1463,17 +1496,13 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     P = std::max(cap_a.P, cap_b.P);
     V = std::max(cap_a.V, cap_b.V);
     T = std::max(cap_a.T, cap_b.T);
-    // final status histogram
-    status_hist_kernel<<<unsigned(std::min<int64_t>(1024, (n + 255) / 256)), 256, 0, c->stream>>>(
-        c->status.as<int>(), n, c->cnt.as<int>());
-    // through pinned memory: a copy into pageable memory is staged and
-    // starts late
-    if (c->pin_dev)
-        RPDCHK(c, rpd_pin_copy(c->stream, c->cnt.as<int>(), c->pin_dev + kPinHist, RPD_STATUS_NUM), "status histogram");
+    // final status histogram, unless the one queued early covers every pass
+    hipStream_t hist_s = c->stream;
+    if (hist_for && hist_for->k == hist_k + 1)
+        hist_s = c->streams[hist_for->sl];
     else
-        HIPCHK(c, hipMemcpyAsync(c->pin + kPinHist, c->cnt.ptr, sizeof(st.status_hist), hipMemcpyDeviceToHost,
-                                 c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+        RPDCHK(c, queue_hist(c->stream), "status histogram");
+    HIPCHK(c, hipStreamSynchronize(hist_s));
     std::memcpy(st.status_hist, c->pin + kPinHist, sizeof(st.status_hist));
     st.ms_knn = ms_knn + ms_fb_knn;
     st.ms_clip = st.ms_clip_first;
