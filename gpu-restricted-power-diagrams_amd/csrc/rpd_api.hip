@@ -43,6 +43,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "rpd_internal.h"
+#include "rpd_wave.h"
 
 // A domain keeps no pointer to the context that made it: it may outlive that
 // context and is shared by the contexts of one device.
@@ -270,14 +271,27 @@ void domain_distance(const rpd_grid_desc *g, std::vector<uint8_t> &dist) {
     }
 }
 
+// The lanes of a wave that hold one status add their count once (a ballot
+// per distinct status in the wave: nearly every cell ends with the same one),
+// then each workgroup adds its counts to out.  (One LDS atomic per cell
+// serialised on the one bin nearly every cell hits.)
 __global__ __launch_bounds__(256) void status_hist_kernel(const int *__restrict__ status, int64_t n,
                                                           int *__restrict__ out) {
     __shared__ int hist[RPD_STATUS_NUM];
     if (threadIdx.x < RPD_STATUS_NUM) hist[threadIdx.x] = 0;
     __syncthreads();
+    const int lane = rpd::lane_id();
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
         const int st = status[i];
-        if (st >= 0 && st < RPD_STATUS_NUM) atomicAdd(&hist[st], 1);
+        const bool ok = st >= 0 && st < RPD_STATUS_NUM;
+        uint64_t todo = rpd::ballot(ok);
+        while (todo) {
+            const int l = rpd::lowest_bit(todo);
+            const int v = __builtin_amdgcn_readlane(st, l);
+            const uint64_t m = rpd::ballot(ok && st == v);
+            if (lane == l) atomicAdd(&hist[v], __popcll(m));
+            todo &= ~m;
+        }
     }
     __syncthreads();
     if (threadIdx.x < RPD_STATUS_NUM && hist[threadIdx.x]) atomicAdd(&out[threadIdx.x], hist[threadIdx.x]);

@@ -26,6 +26,15 @@ namespace rpd {
 // margin over the rounding of M and of the seed test (units of the [0,1000] frame)
 constexpr double kCoverRel = 1e-9, kCoverAbs = 1e-6;
 
+// One thread per domain voxel.  The distance to a seed is rounded up: the
+// hardware FP32 square root (1 ulp) of d^2 rounded to FP32 is within 2^-22 of
+// the exact one, relative, so scaling it by 1 + 2^-18 bounds it from above
+// with room to spare (a d^2 that FP32 flushes to 0 is below 1e-37, far under
+// kCoverAbs).  Any upper bound of M is a valid one; this one is looser by
+// about 1e-5, relative, and takes a fraction of the VALU of the correctly
+// rounded FP64 square root.  (Four lanes per voxel, each over some of the 9
+// rows, was slower: the kernel is VALU bound and the rows do not split
+// evenly.)
 __global__ __launch_bounds__(256) void coverage_kernel(const double4 *__restrict__ stored64,
                                                        const int *__restrict__ counters, int dim,
                                                        const int *__restrict__ dvox, int n_dvox, int g, double vs,
@@ -38,6 +47,8 @@ __global__ __launch_bounds__(256) void coverage_kernel(const double4 *__restrict
         const double rv = 0.8660254037844387 * vs * (1.0 + 1e-12);  // sqrt(3)/2, rounded up
         auto cc = [&](double c) { return max(0, min(dim - 1, int(c * double(dim) / 1000.0))); };
         const int ci = cc(cx), cj = cc(cy), ck = cc(cz);
+        const int x0 = max(0, ci - 1), x1 = min(dim - 1, ci + 1);
+        constexpr double kUp = 1.0 + 0x1p-18;
         for (int dz = -1; dz <= 1; ++dz) {
             const int z = ck + dz;
             if (z < 0 || z >= dim) continue;
@@ -45,11 +56,11 @@ __global__ __launch_bounds__(256) void coverage_kernel(const double4 *__restrict
                 const int y = cj + dy;
                 if (y < 0 || y >= dim) continue;
                 const int row = (y + z * dim) * dim;
-                const int x0 = max(0, ci - 1), x1 = min(dim - 1, ci + 1);
                 for (int j = counters[row + x0], je = counters[row + x1 + 1]; j < je; ++j) {
                     const double4 t = stored64[j];
-                    const double dx = t.x - cx, dy2 = t.y - cy, dz2 = t.z - cz;
-                    const double dd = sqrt(dx * dx + dy2 * dy2 + dz2 * dz2) + rv;
+                    const double ex = t.x - cx, ey = t.y - cy, ez = t.z - cz;
+                    const double d2 = ex * ex + ey * ey + ez * ez;
+                    const double dd = double(__builtin_amdgcn_sqrtf(float(d2))) * kUp + rv;
                     best = fmin(best, dd * dd + (t.w - wmin));
                 }
             }
