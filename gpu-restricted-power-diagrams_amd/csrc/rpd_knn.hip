@@ -139,6 +139,9 @@ struct Key<float> {
     // the ordering key as 32-bit words, most significant first (radix select)
     static constexpr int kWords = 2;
     __device__ uint32_t word(int w) const { return w == 0 ? uint32_t(k >> 32) : uint32_t(k); }
+    // the largest key whose first word is w0 (no real key equals it: its
+    // other bits are all ones)
+    __device__ static Key top_of(uint32_t w0) { return Key{(uint64_t(w0) << 32) | 0xffffffffu, 0xffffffffu}; }
     // candidate buffer in LDS: ordering keys and stored indices
     struct Buf {
         uint64_t *k;
@@ -172,6 +175,9 @@ struct Key<double> {
     __device__ Key lane(int l) const { return Key{rlane(d, l), rlane(o, l), rlane(s, l)}; }
     static constexpr int kWords = 3;
     __device__ uint32_t word(int w) const { return w == 0 ? uint32_t(d >> 32) : (w == 1 ? uint32_t(d) : o); }
+    __device__ static Key top_of(uint32_t w0) {
+        return Key{(uint64_t(w0) << 32) | 0xffffffffu, 0xffffffffu, 0xffffffffu};
+    }
     struct Buf {
         uint64_t *d;
         uint32_t *o, *s;
@@ -393,6 +399,70 @@ __device__ int select_rank(const typename KeyT::Buf &buf, int M, int rank, uint3
     return L + 64 * jj;
 }
 
+// An upper bound of the key of rank `rank` among the M keys of the buffer,
+// from the first digit pass of select_rank alone: the largest key of the bin
+// that holds the rank (the keys' first words from the highest bit in which two
+// differ, 8 bits of them).  The bin's keys and those below it number at least
+// rank + 1, so the key of that rank is at most the bound, and no key equals
+// it.  false: every key in play has the same first word (no bound here).
+template <typename KeyT, int SL>
+__device__ bool bound_rank(const typename KeyT::Buf &buf, int M, int rank, uint32_t *hist, KeyT &ub) {
+    const int lane = lane_id();
+    uint32_t w[SL];
+    uint32_t alive = 0;
+#pragma unroll
+    for (int j = 0; j < SL; ++j) {
+        const int e = lane + 64 * j;
+        w[j] = e < M ? KeyT::get(buf, e).word(0) : 0u;
+        if (e < M) alive |= 1u << j;
+    }
+    uint32_t mine = 0;
+    bool has = false;
+#pragma unroll
+    for (int j = 0; j < SL; ++j)
+        if (!has && ((alive >> j) & 1)) {
+            mine = w[j];
+            has = true;
+        }
+    const uint32_t ref = rlane(mine, lowest_bit(ballot(has)));
+    uint32_t diff = 0;
+#pragma unroll
+    for (int j = 0; j < SL; ++j)
+        if ((alive >> j) & 1) diff |= w[j] ^ ref;
+    diff = uint32_t(rlane(wave_incl_scan_op(int(diff), 0, [](int a, int b) { return a | b; }), 63));
+    if (diff == 0) return false;
+    const int sh = max(31 - __clz(int(diff)) - 7, 0);
+    uint4 *h4 = reinterpret_cast<uint4 *>(hist);
+    h4[lane] = make_uint4(0u, 0u, 0u, 0u);
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < SL; ++j)
+        if ((alive >> j) & 1) atomicAdd(&hist[(w[j] >> sh) & 255u], 1u);
+    wave_lds_sync();
+    const uint4 h = h4[lane];
+    const int sum = int(h.x + h.y + h.z + h.w);
+    const int incl = wave_incl_scan_op(sum, 0, [](int a, int b) { return a + b; });
+    const int excl = incl - sum;
+    const int L = lowest_bit(ballot(excl <= rank && rank < incl));
+    int before = rlane(excl, L);
+    const int c0 = int(rlane(h.x, L)), c1 = int(rlane(h.y, L)), c2 = int(rlane(h.z, L));
+    int bin = 4 * L;
+    if (rank >= before + c0) {
+        before += c0;
+        bin += 1;
+        if (rank >= before + c1) {
+            before += c1;
+            bin += 1;
+            if (rank >= before + c2) bin += 1;
+        }
+    }
+    wave_lds_sync();  // the histogram is cleared again by its next user
+    // bits above the window are every key's; the bin's bits, then all ones
+    const uint32_t low = sh + 8 >= 32 ? 0xffffffffu : (1u << (sh + 8)) - 1u;
+    ub = KeyT::top_of((ref & ~low) | (uint32_t(bin) << sh) | ((1u << sh) - 1u));
+    return true;
+}
+
 // The K nearest of query qi (its stored id ids[qi]): the wave's LDS pieces
 // are cand (kCandBuf ints), hist (256 words) and the key buffer buf.
 template <typename R, int S>
@@ -425,10 +495,8 @@ __device__ __forceinline__ void knn_query(const KnnArgs<R> &a, int64_t qi, int l
         const int L = a.lower[q];
         klow = KeyT::make(dist2_unfused<R>(p, a.pts[L]), uint32_t(a.perm[L]), uint32_t(L));
     }
-    // exact K-th key of the buffer, and the buffer cut to the K keys <= it
-    auto shrink = [&]() {
-        const int sel = select_rank<KeyT, SL>(buf, M, K - 1, hist);
-        kth = KeyT::get(buf, sel);
+    // the buffer cut to its keys <= kth
+    auto cut = [&]() {
         KeyT v[SL];
         bool kp[SL];
 #pragma unroll
@@ -446,8 +514,13 @@ __device__ __forceinline__ void knn_query(const KnnArgs<R> &a, int64_t qi, int l
             if (kp[j]) v[j].put(buf, base + mbcnt(m));
             base += __popcll(m);
         }
-        M = base;  // == K
+        M = base;
         wave_lds_sync();
+    };
+    // exact K-th key of the buffer, and the buffer cut to the K keys <= it
+    auto shrink = [&]() {
+        kth = KeyT::get(buf, select_rank<KeyT, SL>(buf, M, K - 1, hist));
+        cut();
     };
     auto add_batch = [&](const KeyT &c) {
         const bool keep = !c.is_inf() && c < kth && (!has_low || klow < c);
@@ -550,8 +623,21 @@ __device__ __forceinline__ void knn_query(const KnnArgs<R> &a, int64_t qi, int l
                 wave_lds_sync();  // the next piece overwrites cand
             }
         }
-        // the exact K-th key of the candidates so far
-        if (M > K || (M == K && kth.is_inf())) shrink();
+        // The K-th key of the candidates so far: after the first shell, only
+        // a bound of it (one digit pass instead of a full select), which
+        // serves the next shell's cull and the stop test; the buffer keeps
+        // the keys up to the bound.  After later shells, the exact key.
+        if (r == 1 && M > K) {
+            KeyT ub;
+            if (bound_rank<KeyT, SL>(buf, M, K - 1, hist, ub)) {
+                kth = ub;
+                cut();
+            } else {
+                shrink();
+            }
+        } else if (M > K || (M == K && kth.is_inf())) {
+            shrink();
+        }
         // stop once the K-th key is closer than anything in the next shell
         if (!kth.is_inf()) {
             const double lb = double(r) * cs * (1.0 - 1e-4);
@@ -560,15 +646,32 @@ __device__ __forceinline__ void knn_query(const KnnArgs<R> &a, int64_t qi, int l
         const int rn = r + 1;
         if (ci - rn < 0 && ci + rn >= dim && cj - rn < 0 && cj + rn >= dim && ck - rn < 0 && ck + rn >= dim) break;
     }
+    // the K kept keys exactly (a search that stops after the first shell
+    // holds the keys up to its bound)
+    if (M > K) shrink();
     int *out = a.out + qi * int64_t(K);
     if constexpr (S == 1 && sizeof(R) == 4) {
         // K <= 64, one-word keys: each kept key's place is the number of
         // kept keys below it (keys are distinct); places past M are padding.
         // (With FP64 keys, two words, the sort below is faster.)
+        // The places come from the distances' bits (one 32-bit compare per
+        // key); keys of equal distance get one place, which one LDS write
+        // and read per lane detects, and then the full keys decide.
         const KeyT mine = lane < M ? KeyT::get(buf, lane) : KeyT::inf();
+        const uint32_t mh = uint32_t(mine.k >> 32);
+        const uint32_t *hi = reinterpret_cast<const uint32_t *>(buf.k) + 1;  // high words, every other one
         int rank = 0;
 #pragma unroll 4
-        for (int j = 0; j < M; ++j) rank += KeyT::get(buf, j) < mine ? 1 : 0;
+        for (int j = 0; j < M; ++j) rank += hi[2 * j] < mh ? 1 : 0;
+        int *place = reinterpret_cast<int *>(hist);  // free after the last selection (K <= 256 words)
+        if (lane < M) place[rank] = lane;
+        wave_lds_sync();
+        const bool clash = lane < M && place[rank] != lane;
+        wave_lds_sync();
+        if (ballot(clash)) {
+            rank = 0;
+            for (int j = 0; j < M; ++j) rank += KeyT::get(buf, j) < mine ? 1 : 0;
+        }
         if (lane < M)
             out[rank] = int(mine.s);
         else if (lane < K)
