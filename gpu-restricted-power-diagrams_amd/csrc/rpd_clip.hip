@@ -34,8 +34,11 @@
 // Forms: FP64 (all double); mixed: the plane-side tests and the vertex
 // registers are FP32 and each test also evaluates a certified bound on its
 // FP32 error against the FP64 planes, 2^-18 * sum of |terms| of the det4x4
-// expansion; a test inside the bound or an ill-conditioned new vertex ends the
-// cell with needs_exact_predicates and the host re-runs it in FP64.  The
+// expansion.  In the bisector kernel (FP32 planes in LDS) a test inside the
+// bound or an ill-conditioned new vertex ends the cell with
+// needs_exact_predicates and the host re-runs it in FP64; in the finish kernel
+// (FP64 planes in LDS) a test inside the bound is decided in FP64 from the
+// vertex's planes, as the FP64 form decides it.  The
 // planes are stored in FP64, and the cell geometry that produces the results
 // (bbox, cone planes, volume and moments) is evaluated in FP64 from them, so
 // an unflagged mixed cell has the FP64 combinatorics and FP64 geometry.  FP32
@@ -231,6 +234,17 @@ __device__ __forceinline__ void vertex_from_planes(const Pl<R> &p1, const Pl<R> 
     }
 }
 
+// Whether the vertex of planes tr lies strictly outside plane e, in FP64:
+// the vertex rebuilt from its FP64 planes (as the FP64 form builds it) and
+// the FP64 form's test.  (Out of line, the call cost the finish kernel more
+// spills than inlined.)
+__device__ __forceinline__ bool outside64(const Pl<double> *planes, uint32_t tr, Pl<double> e) {
+    double x, y, z, w;
+    float mx, mw;
+    vertex_from_planes<double, false>(planes[t0(tr)], planes[t1(tr)], planes[t2(tr)], x, y, z, w, mx, mw);
+    return e.x * x + e.y * y + e.z * z + e.w * w > 0.0;
+}
+
 // Error bound factor of the mixed form: 64 units in the last place of FP32
 // (first-order analysis gives 13: 4 from rounding the FP64 planes, 9 from the
 // cofactor and dot-product evaluation).  The bound of a test e.v is
@@ -335,7 +349,7 @@ struct WaveCell {
     // up to date.
     template <bool R2 = true>
     __device__ int clip(const Pl<R> &e, int wl, const Pl<PS> &eP) {
-        uint64_t rem[S];
+        uint64_t rem[S], unsure[S];
         int nr = 0, nvalid = 0;
         uint64_t flag = 0;
         const float exyz = fabsf(float(e.x)) + fabsf(float(e.y)) + fabsf(float(e.z)), ew = fabsf(float(e.w));
@@ -344,14 +358,33 @@ struct WaveCell {
             const bool v = (valid[s] >> lane) & 1;
             const R sv = e.x * X[s] + e.y * Y[s] + e.z * Z[s] + e.w * W[s];
             rem[s] = ballot(v && sv > R(0));
+            unsure[s] = 0;
             if (FILTER) {
                 const float bound = kFilterKappa * (exyz * MXYZ[s] + ew * MW[s]);
-                flag |= ballot(v && fabsf(float(sv)) <= bound);
+                unsure[s] = ballot(v && fabsf(float(sv)) <= bound);
+                flag |= unsure[s];
             }
             nr += __popcll(rem[s]);
             nvalid += __popcll(valid[s]);
         }
-        if (FILTER && flag) return RPD_NEEDS_EXACT_PREDICATES;
+        if (FILTER && flag) {
+            if constexpr (sizeof(PS) == 8) {
+                // FP64 planes at hand (the finish kernel): the vertices the
+                // filter leaves open are rebuilt from their planes in FP64
+                // and tested in FP64, the FP64 form's own evaluation, so the
+                // clip decides as the FP64 form does
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (!unsure[s]) continue;
+                    const bool out = ((unsure[s] >> lane) & 1) && outside64(planes, tri[s], eP);
+                    const uint64_t now = ballot(out);
+                    nr += __popcll(now) - __popcll(rem[s] & unsure[s]);
+                    rem[s] = (rem[s] & ~unsure[s]) | now;
+                }
+            } else {
+                return RPD_NEEDS_EXACT_PREDICATES;
+            }
+        }
         if (nr == 0) return kNoClip;
         if (nr == nvalid) return RPD_EMPTY_CELL;
         const int pid = nb_plane;
