@@ -139,6 +139,7 @@ struct rpd_ctx {
     bool solved = false;
     hipEvent_t ev[4][4] = {};  // per stream: before kNN, after kNN, after the bisector kernel, after the finish
     hipEvent_t dep[3] = {};    // per chain: orders work between its streams; [2]: pass 1's queue length copied
+    hipEvent_t evb[2] = {};    // a solve's binning (and weight minima): before, after
     int *pin = nullptr;        // kPinWords pinned host words
     int *pin_dev = nullptr;    // their device address (nullptr: copies go through the copy engine)
     double *pin_red = nullptr;  // 2 x kWminBlocks pinned doubles: per-block weight minima
@@ -234,6 +235,32 @@ __global__ __launch_bounds__(256) void wmin_kernel(const double4 *__restrict__ s
     if (threadIdx.x == 0) {
         out[blockIdx.x] = red[0][0];
         out[gridDim.x + blockIdx.x] = red[1][0];
+    }
+}
+
+// The two minima over the nb blocks' partials (non-border seeds, all seeds)
+// to out[0], out[1]: for the kernels queued before the host has them.
+__global__ __launch_bounds__(256) void wmin_final_kernel(const double *__restrict__ part, int nb,
+                                                         double *__restrict__ out) {
+    __shared__ double red[2][256];
+    double m = INFINITY, ma = INFINITY;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        m = fmin(m, part[i]);
+        ma = fmin(ma, part[nb + i]);
+    }
+    red[0][threadIdx.x] = m;
+    red[1][threadIdx.x] = ma;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+            red[1][threadIdx.x] = fmin(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
     }
 }
 
@@ -476,6 +503,11 @@ int rpd_create_streams(int device, int n_streams, rpd_ctx **out) {
             rpd_destroy(c);
             return RPD_E_DEVICE;
         }
+    for (auto &e : c->evb)
+        if (hipEventCreate(&e) != hipSuccess) {
+            rpd_destroy(c);
+            return RPD_E_DEVICE;
+        }
     if (hipHostMalloc(reinterpret_cast<void **>(&c->pin), kPinWords * sizeof(int), hipHostMallocDefault) !=
         hipSuccess) {
         c->pin = nullptr;
@@ -507,6 +539,8 @@ void rpd_destroy(rpd_ctx *c) {
         for (auto &e : es)
             if (e) hipEventDestroy(e);
     for (auto &e : c->dep)
+        if (e) hipEventDestroy(e);
+    for (auto &e : c->evb)
         if (e) hipEventDestroy(e);
     for (int i = 0; i < c->n_streams; ++i)
         if (c->streams[i]) hipStreamDestroy(c->streams[i]);
@@ -766,32 +800,30 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     st.n = n;
     auto t_start = std::chrono::steady_clock::now();
 
-    HIPCHK(c, hipEventRecord(c->ev[0][0], c->stream));
-    // the weight minima of new seeds, read after the binning's wait
+    HIPCHK(c, hipEventRecord(c->evb[0], c->stream));
+    // The weight minima of new seeds (over the non-border seeds, and over
+    // all, whose sign tells whether there are walls) and the binning are
+    // queued, and nothing waits for them here: the cover test reads the
+    // minima on the device (wdev), and the host reads them, with the
+    // binning's crowded-cell flag, after the solve's first wait (settle
+    // below), which the kNN launch has already overlapped.
     const bool new_wmin = !batch && prm->security_mode == 0 && !c->wmin_ok;
+    const double *wdev = nullptr;
     if (new_wmin) {
-        RPDCHK(c, c->wred.ensure(2 * kWminBlocks * sizeof(double)), "alloc wred");
-        wmin_kernel<<<kWminBlocks, 256, 0, c->stream>>>(c->seeds.as<double4>(), n, c->wred.as<double>());
-        HIPCHK(c, hipMemcpyAsync(c->pin_red, c->wred.ptr, 2 * kWminBlocks * sizeof(double), hipMemcpyDeviceToHost,
+        RPDCHK(c, c->wred.ensure((2 * kWminBlocks + 2) * sizeof(double)), "alloc wred");
+        double *wr = c->wred.as<double>();
+        wmin_kernel<<<kWminBlocks, 256, 0, c->stream>>>(c->seeds.as<double4>(), n, wr);
+        wmin_final_kernel<<<1, 256, 0, c->stream>>>(wr, kWminBlocks, wr + 2 * kWminBlocks);
+        HIPCHK(c, hipMemcpyAsync(c->pin_red, wr + 2 * kWminBlocks, 2 * sizeof(double), hipMemcpyDeviceToHost,
                                  c->stream));
+        wdev = wr + 2 * kWminBlocks;
     }
     if (batch) {
         if (c->bin_prec != kprec) return set_err(c, RPD_E_STATE, "rpd_solve_batch: binned in another precision");
     }
     bool bin_pending = false;
     if (!batch) RPDCHK(c, ensure_binning(c, kprec, &bin_pending), "binning");
-    HIPCHK(c, hipEventRecord(c->ev[0][1], c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (bin_pending) RPDCHK(c, finish_binning(c), "binning");
-    st.ms_bin = elapsed_ms(c->ev[0][0], c->ev[0][1]);
-    if (new_wmin) {
-        c->wmin = c->wall = INFINITY;
-        for (int b = 0; b < kWminBlocks; ++b) {
-            c->wmin = std::min(c->wmin, c->pin_red[b]);
-            c->wall = std::min(c->wall, c->pin_red[kWminBlocks + b]);
-        }
-        c->wmin_ok = true;
-    }
+    HIPCHK(c, hipEventRecord(c->evb[1], c->stream));
 
     RPDCHK(c, c->out.ensure(size_t(n) * 32), "alloc out");
     RPDCHK(c, c->status.ensure(size_t(n) * 4), "alloc status");
@@ -853,12 +885,37 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
                                    : nullptr;
     };
     const DomainDev *dd = dom ? &dom->dev : nullptr;
+    // the weight minimum the kernels subtract, and whether there are walls
+    // (border seeds): known here unless the minima are new (then at settle)
     double wmin = 0;
     bool border_seeds = true;
-    if (prm->security_mode == 0) {
-        wmin = c->wmin < INFINITY ? c->wmin : 0.0;
-        border_seeds = c->wall <= RPD_NO_CELL_LIMITS;
-    }
+    auto host_wmin = [&]() {
+        if (prm->security_mode == 0) {
+            wmin = c->wmin < INFINITY ? c->wmin : 0.0;
+            border_seeds = c->wall <= RPD_NO_CELL_LIMITS;
+        }
+    };
+    if (!new_wmin) host_wmin();
+    // After the solve's first wait: the binning's time, its crowded-cell
+    // path, the new weight minima.  *restart: the binning was redone (a
+    // crowded cell), so what was queued on the old one is to be redone.
+    auto settle = [&](bool *restart) -> int {
+        *restart = false;
+        if (hipEventSynchronize(c->evb[1]) != hipSuccess) return RPD_E_DEVICE;
+        st.ms_bin = elapsed_ms(c->evb[0], c->evb[1]);
+        if (new_wmin) {
+            c->wmin = c->pin_red[0];
+            c->wall = c->pin_red[1];
+            c->wmin_ok = true;
+            host_wmin();
+        }
+        if (bin_pending && c->pin[kPinBinFlag]) {
+            const int r = finish_binning(c);
+            if (r != RPD_OK) return r;
+            *restart = true;
+        }
+        return RPD_OK;
+    };
     // cells provably outside the domain skip pass 1 (exact stop, no walls);
     // in a batch, per request
     const int *ids1 = nullptr;
@@ -882,6 +939,7 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
             ca.vs = d->vs;
         }
         ca.wmin = wm;
+        ca.wdev = walls ? nullptr : wdev;
         ca.out = c->out.as<double4>();
         ca.status = c->status.as<int>();
         ca.kused = c->kused.as<int>();
@@ -893,7 +951,8 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
                                    prm->security_mode == 0 && q.wmin < INFINITY ? q.wmin : 0.0,
                                    prm->security_mode != 0 || q.wall <= RPD_NO_CELL_LIMITS));
     } else if (dd) {
-        cov.push_back(cover_of(dd, c->counters.as<int>(), c->dim, 0, n, wmin, border_seeds));
+        // new minima: the device tells walls (no cover test) from none
+        cov.push_back(cover_of(dd, c->counters.as<int>(), c->dim, 0, n, wmin, new_wmin ? false : border_seeds));
     }
     bool any_cover = false;
     for (const auto &ca : cov) any_cover = any_cover || ca.apply;
@@ -1343,6 +1402,15 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
                        "pass 1 knn (rest)");
                 HIPCHK(c, hipEventRecord(c->ev[0][1], s));
             }
+        }
+        {
+            // (the cover test's wait, if any, has covered the binning)
+            bool restart = false;
+            RPDCHK(c, settle(&restart), "binning");
+            // the seeds were rebinned: the cover test and the kNN queued
+            // so far read the old order, so the solve starts over (once:
+            // the seeds are binned now)
+            if (restart) return solve_impl(c, dom, prm, stats);
         }
         p1.slot = 0;
         p1.form = form0;

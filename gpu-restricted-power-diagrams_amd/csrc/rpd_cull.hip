@@ -38,7 +38,12 @@ constexpr double kCoverRel = 1e-9, kCoverAbs = 1e-6;
 __global__ __launch_bounds__(256) void coverage_kernel(const double4 *__restrict__ stored64,
                                                        const int *__restrict__ counters, int dim,
                                                        const int *__restrict__ dvox, int n_dvox, int g, double vs,
-                                                       double wmin, unsigned long long *__restrict__ mbits) {
+                                                       double wmin, const double *__restrict__ wdev,
+                                                       unsigned long long *__restrict__ mbits) {
+    if (wdev) {
+        if (wdev[1] <= RPD_NO_CELL_LIMITS) return;  // walls: no test (every block returns)
+        wmin = wdev[0] < INFINITY ? wdev[0] : 0.0;
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double best = INFINITY;
     if (i < n_dvox) {
@@ -82,11 +87,19 @@ __global__ __launch_bounds__(256) void coverage_kernel(const double4 *__restrict
 
 __global__ __launch_bounds__(256) void skip_kernel(const double4 *__restrict__ stored64, const int *__restrict__ perm,
                                                    int64_t n, const uint8_t *__restrict__ dist, int g, double vs,
-                                                   double wmin, const unsigned long long *__restrict__ mbits,
+                                                   double wmin, const double *__restrict__ wdev,
+                                                   const unsigned long long *__restrict__ mbits,
                                                    uint8_t *__restrict__ keep, double4 *__restrict__ out,
                                                    int *__restrict__ status, int *__restrict__ kused) {
     const int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     if (s >= n) return;
+    if (wdev) {
+        if (wdev[1] <= RPD_NO_CELL_LIMITS) {  // walls: every cell is kept
+            keep[s] = 1;
+            return;
+        }
+        wmin = wdev[0] < INFINITY ? wdev[0] : 0.0;
+    }
     const double M = __longlong_as_double((long long)*mbits);
     const double4 p = stored64[s];
     auto vox = [&](double x) { return int(fmin(fmax(x / vs, 0.0), double(g - 1))); };
@@ -136,11 +149,11 @@ int rpd_cover_skip(hipStream_t st, const CoverArgs *reqs, int nreq, int64_t n_to
             continue;
         }
         if (c.n_dvox > 0)
-            coverage_kernel<<<unsigned((c.n_dvox + 255) / 256), 256, 0, st>>>(c.stored64, c.counters, c.dim, c.dvox,
-                                                                               c.n_dvox, c.g, c.vs, c.wmin, mbits + r);
+            coverage_kernel<<<unsigned((c.n_dvox + 255) / 256), 256, 0, st>>>(
+                c.stored64, c.counters, c.dim, c.dvox, c.n_dvox, c.g, c.vs, c.wmin, c.wdev, mbits + r);
         skip_kernel<<<unsigned((c.n + 255) / 256), 256, 0, st>>>(c.stored64 + c.base, c.perm + c.base, c.n, c.dist,
-                                                                 c.g, c.vs, c.wmin, mbits + r, keep + c.base, c.out,
-                                                                 c.status, c.kused);
+                                                                 c.g, c.vs, c.wmin, c.wdev, mbits + r, keep + c.base,
+                                                                 c.out, c.status, c.kused);
     }
     size_t sb = sel_bytes;
     if (rocprim::select(base + off_tmp, sb, rocprim::counting_iterator<int>(0), keep, ids_out, cnt, size_t(n), st) !=

@@ -235,6 +235,10 @@ struct CoverArgs {
     int g;
     double vs;
     double wmin;              // minimum weight (no border seeds)
+    // nullable: the device pair (minimum weight over non-border seeds,
+    // minimum over all) replacing wmin; a second one <= RPD_NO_CELL_LIMITS
+    // (walls) turns the test off
+    const double *wdev;
     double4 *out;             // original order: skipped cells get 0
     int *status;
     int *kused;
