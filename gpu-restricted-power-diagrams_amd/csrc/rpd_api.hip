@@ -72,6 +72,8 @@ constexpr int kPinCounters = 16;
 constexpr int kPinBinFlag = kPinCounters - 4;
 // smallest triangle cap a solve uses
 constexpr int kTriangleFloor = 256;
+// the plane cap a solve raises a lower one to, where that costs no residency
+constexpr int kPlaneFloor = 48;
 constexpr int kPinHist = kPinCounters + 6 * kFailCounters;
 constexpr int kPinWords = kPinHist + 16;
 static_assert(RPD_STATUS_NUM <= 16, "status histogram room");
@@ -792,6 +794,14 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
     // finish kernel no residency, and a cell's result does not depend on the
     // cap it succeeds with; a lower cap only sends cells to re-queue passes.
     const bool hess = prm->H > 0;
+    // The plane cap likewise, toward kPlaneFloor as far as the bisector and
+    // finish kernels keep their residency: the restriction's cone clips add
+    // up to 4 planes to a cell, which pushed the cells closest to the cap
+    // into an FP64 finish pass of their own (6 cells at P = 40 at the 1M
+    // torus, which also grew the bisector failures' pass to P = 60).  Only
+    // with re-queue passes allowed: without them an overflow is the answer.
+    if (prm->max_extra_steps > 0 && !hess && P < kPlaneFloor)
+        P = rpd_clip_plane_room(form0, P, V, std::max(T, kTriangleFloor), kPlaneFloor);
     T = std::max(T, std::min(kTriangleFloor, rpd_clip_max_T(form0, P, V, hess)));
     // Likewise the vertex cap is the room of the vertex slots it needs
     // (64 per slot): a cap below it saves nothing.

@@ -1793,6 +1793,23 @@ int rpd_clip_max_T(int form, int P, int V, bool hess) {
     return lo;
 }
 
+int rpd_clip_plane_room(int form, int P, int V, int T, int floor) {
+    // LDS of a gfx950 CU, shared by the resident workgroups
+    constexpr size_t kLdsPerCu = 160 * 1024;
+    const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);
+    if (S != 1) return P;
+    // cells per CU at the kernels' residency: 4 waves per SIMD for the
+    // finish kernel, kClipWaves for the bisector kernel (one cell a wave)
+    const size_t fin_cells = 4 * 4;
+    const size_t bis_cells = 4 * size_t(form == RPD_PREC_FP64 ? kClipWaves<double, 1> : kClipWaves<float, 1>);
+    const size_t bis_pl = form == RPD_PREC_FP64 ? 32 : 16;  // the bisector kernel's planes, in the test precision
+    for (int p = std::min(floor, 255); p > P; --p)
+        if (finish_cell_lds(form, p, V, T, false) * fin_cells <= kLdsPerCu &&
+            finish_wave_lds(p, S, 0, bis_pl) * bis_cells <= kLdsPerCu)
+            return p;
+    return P;
+}
+
 // header | P planes | 64 S vertex triples | P plane neighbours (int)
 size_t rpd_clip_record_bytes(int form, int P, int V) {
     const int S = V <= 64 ? 1 : (V <= 128 ? 2 : 4);

@@ -256,6 +256,10 @@ size_t rpd_clip_record_bytes(int form, int P, int V);
 // Largest T whose finish-kernel cell fits the current device's workgroup LDS
 // (-1: none).
 int rpd_clip_max_T(int form, int P, int V, bool hess = false);
+// The plane cap P raised toward floor as far as the bisector and finish
+// kernels keep their residency (LDS per CU) for one-slot cells (V <= 64)
+// with T triangle entries; P itself otherwise.
+int rpd_clip_plane_room(int form, int P, int V, int T, int floor);
 // form: RPD_PREC_FP64 / RPD_PREC_MIXED / RPD_PREC_FP32.  The bisector-loop
 // kernel of L.n_ids cells, then copies to the host the number of cells it
 // Copies n0 ints src0 -> dst0 and n1 src1 -> dst1 with one one-wave kernel
