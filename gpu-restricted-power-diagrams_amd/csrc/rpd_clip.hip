@@ -37,8 +37,9 @@
 // expansion.  In the bisector kernel (FP32 planes in LDS) a test inside the
 // bound or an ill-conditioned new vertex ends the cell with
 // needs_exact_predicates and the host re-runs it in FP64; in the finish kernel
-// (FP64 planes in LDS) a test inside the bound is decided in FP64 from the
-// vertex's planes, as the FP64 form decides it.  The
+// (FP64 planes in LDS) a test inside the bound, and every test of an
+// ill-conditioned vertex, is decided in FP64 from the vertex's planes, as the
+// FP64 form decides it.  The
 // planes are stored in FP64, and the cell geometry that produces the results
 // (bbox, cone planes, volume and moments) is evaluated in FP64 from them, so
 // an unflagged mixed cell has the FP64 combinatorics and FP64 geometry.  FP32
@@ -288,7 +289,13 @@ struct WaveCell {
     R X[S], Y[S], Z[S], W[S], r2[S];
     float MXYZ[S], MW[S];
     uint64_t valid[S];
+    // (mixed finish kernel) vertices whose tests are all decided in FP64:
+    // ill-conditioned in FP32 (|W| small against its terms)
+    uint64_t x64[S];
     R maxr2;
+    // FP64 planes in LDS under the mixed filter (the finish kernel): tests
+    // the filter leaves open are decided in FP64 in place
+    static constexpr bool kInline64 = FILTER && sizeof(PS) == 8;
 
     // R2: also the vertex's squared distance from the seed (the bisector
     // loop's stop test reads it through maxr2; the finish kernel does not)
@@ -338,7 +345,9 @@ struct WaveCell {
             else
                 m = R(sizeof(R) == 8 ? 0x1.0p-33 : 0x1.0p-16) *
                     (fabs(e.x * X[s]) + fabs(e.y * Y[s]) + fabs(e.z * Z[s]) + fabs(e.w * W[s]));
-            keep |= v && !(sv > m);
+            bool k = !(sv > m);
+            if constexpr (kInline64) k = k || ((x64[s] >> lane) & 1);
+            keep |= v && k;
         }
         return ballot(keep) == 0;
     }
@@ -362,6 +371,7 @@ struct WaveCell {
             if (FILTER) {
                 const float bound = kFilterKappa * (exyz * MXYZ[s] + ew * MW[s]);
                 unsure[s] = ballot(v && fabsf(float(sv)) <= bound);
+                if constexpr (kInline64) unsure[s] |= x64[s] & valid[s];
                 flag |= unsure[s];
             }
             nr += __popcll(rem[s]);
@@ -526,7 +536,13 @@ struct WaveCell {
                 const uint32_t tr = nvs[q];
                 set_vertex<R2>(s, tr, e, cvt<R>(planes[t1(tr)]), cvt<R>(planes[t2(tr)]));
             }
-            if (FILTER) ill |= ballot(take && fabsf(float(W[s])) < kFilterMinW * MW[s]);
+            if (FILTER) {
+                const uint64_t il = ballot(take && fabsf(float(W[s])) < kFilterMinW * MW[s]);
+                if constexpr (kInline64)
+                    x64[s] = (x64[s] & ~newm) | il;  // decided in FP64 from now on
+                else
+                    ill |= il;
+            }
             carry += __popcll(fr);
             valid[s] |= newm;
         }
@@ -698,6 +714,7 @@ void clip_cells_kernel(ClipArgs a) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         c.valid[s] = 0;
+        c.x64[s] = 0;
         c.tri[s] = 0;
         c.X[s] = c.Y[s] = c.Z[s] = R(0);
         c.W[s] = R(1);
@@ -1077,6 +1094,7 @@ void finish_cells_kernel(ClipArgs a, int nq, int W) {
         c.MXYZ[s] = c.MW[s] = 0.f;
         if (j < nvalid) c.template set_vertex_lds<false>(s, rtr[j]);
         c.valid[s] = ballot(j < nvalid);
+        c.x64[s] = 0;
     }
     // (the vertices' distances from the seed are not kept here: the
     // restriction takes the cell's radius in FP64 below)
