@@ -147,7 +147,15 @@ typedef struct rpd_domain rpd_domain;
 
 /* Solve parameters: the public caps of VoroAlgo (voronoi.h:199-230). */
 typedef struct {
-    int32_t K, P, V, T;           /* caps of the first pass                   */
+    int32_t K, P, V, T;           /* caps of the first pass.  A solve may
+                                     raise caps where that costs the kernels
+                                     no residency and changes no result: T to
+                                     at least 256, V to its vertex slots'
+                                     room (64/128/256) and, with
+                                     max_extra_steps > 0, P toward 48.  A
+                                     cell's result does not depend on the cap
+                                     it succeeds with; the caps a solve used
+                                     are in rpd_stats.final_* / pass_caps    */
     int32_t precision;            /* enum rpd_precision                       */
     int32_t max_extra_steps;      /* re-queue passes over failed cells with
                                      grown caps (ref: 5)                      */
