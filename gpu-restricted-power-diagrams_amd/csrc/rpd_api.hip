@@ -1076,10 +1076,16 @@ static int solve_impl(rpd_ctx *c, rpd_domain *dom, const rpd_params *prm, rpd_st
         return rpd_launch_cells(s, ps.form, L, c->pin + sl, false);
     };
     // the finish kernel of a pass whose bisector kernel (and queue length) is done
+    // The FP64 passes of a mixed solve finish in the mixed form: over the
+    // same records (FP64 planes), its finish kernel decides in FP64, in
+    // place, the tests its filter leaves open and those of ill-conditioned
+    // vertices, so a cell gets the FP64 form's combinatorics for less work.
+    const int finish_form_fp64 = form0 == RPD_PREC_MIXED && !hess ? RPD_PREC_MIXED : RPD_PREC_FP64;
     auto pass_back = [&](Pass &ps, const ClipLaunch &L, int beside) -> int {
         hipStream_t s = c->streams[ps.slot];
         ps.nq = c->pin[ps.slot];
-        const int r = rpd_launch_finish(s, ps.form, L, ps.nq, beside);
+        const int r =
+            rpd_launch_finish(s, ps.form == RPD_PREC_FP64 ? finish_form_fp64 : ps.form, L, ps.nq, beside);
         if (r != RPD_OK) return r;
         return hipEventRecord(c->ev[ps.slot][3], s) == hipSuccess ? RPD_OK : RPD_E_DEVICE;
     };
